@@ -1,0 +1,170 @@
+/*
+ * rmesh.h -- C ABI of the MI355X (gfx950) tetrahedral tile rasterizer and hash-grid field.
+ *
+ * Every pointer argument named *_dev is device memory owned by the caller; every entry point
+ * enqueues work on the given HIP stream (0 = the null stream), returns 0 on success and a
+ * non-zero RM_ERR_* code otherwise. rm_last_error() returns a thread-local message for the last
+ * failure. No entry point allocates device memory: operations that need scratch take a
+ * caller-provided workspace sized by the matching *_workspace_size() query (two-phase, as
+ * CUB's temp-storage protocol). Forward passes are deterministic (no atomics); backward passes
+ * accumulate with float atomics into caller-zeroed gradient buffers.
+ *
+ * Camera convention (reference: delaunay_rasterization/internal/slang/camera.slang:13-76 and
+ * data/camera.py:72-91): w2v is the row-major 4x4 world->view matrix applied to column vectors,
+ * K the row-major 3x3 intrinsics [[fx,0,cx],[0,fy,cy],[0,0,1]], dist = (k1,k2,p1,p2),
+ * camera_type 0 = pinhole (optionally distorted), 1 = fisheye.
+ */
+#ifndef RMESH_H
+#define RMESH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* rm_stream_t; /* == hipStream_t */
+
+enum {
+    RM_OK = 0,
+    RM_ERR_INVALID_ARGUMENT = 1,
+    RM_ERR_UNSUPPORTED_TILE = 2,
+    RM_ERR_WORKSPACE_TOO_SMALL = 3,
+    RM_ERR_HIP = 4,
+    RM_ERR_CAPACITY = 5
+};
+
+typedef struct rm_camera {
+    float w2v[16];
+    float K[9];
+    float cam_pos[3];
+    float dist[4];
+    float min_t;
+    float fovy;
+    float fovx;
+    int32_t image_height;
+    int32_t image_width;
+    int32_t grid_height;
+    int32_t grid_width;
+    int32_t tile_height;
+    int32_t tile_width;
+    int32_t camera_type;
+    int32_t _pad;
+} rm_camera;
+
+/* ---- library info ---------------------------------------------------------------------- */
+const char* rm_last_error(void);
+int rm_version(void);
+
+/* ---- per-tet projection and tile binning ------------------------------------------------
+ * Replaces the Slang kernel vertex_shader launched by VertexShader.forward
+ * (delaunay_rasterization/internal/tile_shader_slang.py:75-125,
+ *  delaunay_rasterization/internal/slang/vertex_shader.slang:20-89).
+ * Writes every output element (culled tets get zeros), so outputs need no zero-init.
+ * vs_dev is [T,3] f32 of which only column 2 (the circumsphere power) is non-zero. */
+int rm_project_fwd(const rm_camera* cam, const int32_t* indices_dev, const float* vertices_dev,
+                   int64_t n_tets, int64_t n_verts,
+                   int32_t* tiles_touched_dev, int32_t* rect_dev, float* vs_dev,
+                   float* circumcenter_dev, rm_stream_t stream);
+
+/* Replaces vertex_shader.bwd (tile_shader_slang.py:127-150): gradients of vs[:,2] and the
+ * circumcenter w.r.t. the vertices (FP64 adjoint), atomically added into grad_vertices_dev. */
+int rm_project_bwd(const rm_camera* cam, const int32_t* indices_dev, const float* vertices_dev,
+                   int64_t n_tets, int64_t n_verts, const int32_t* tiles_touched_dev,
+                   const float* grad_vs_dev, const float* grad_circumcenter_dev,
+                   float* grad_vertices_dev, rm_stream_t stream);
+
+/* ---- scan / keys / sort / ranges: the reference's own decomposition ---------------------
+ * rm_inclusive_scan_i32 replaces torch.cumsum(tiles_touched) (tile_shader_slang.py:36). */
+size_t rm_scan_workspace_size(int64_t n);
+int rm_inclusive_scan_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n,
+                          void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+
+/* Replaces tile_shader.generate_keys (tile_shader.slang:49-97). */
+int rm_generate_keys(const float* vs_dev, const int32_t* rect_dev, const int32_t* offsets_dev,
+                     int64_t n_tets, int32_t grid_height, int32_t grid_width,
+                     int64_t* keys_dev, int32_t* values_dev, rm_stream_t stream);
+
+/* Replaces sort_by_keys_cub.sort_by_keys (sort_by_keys/sort_by_keys.cu:9-53): stable LSD
+ * radix sort of (u64 key, i32 value) pairs over key bits [0, end_bit). */
+size_t rm_sort_pairs_workspace_size(int64_t n);
+int rm_sort_pairs(const int64_t* keys_in_dev, const int32_t* values_in_dev,
+                  int64_t* keys_out_dev, int32_t* values_out_dev, int64_t n, int32_t end_bit,
+                  void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+
+/* Replaces tile_shader.compute_tile_ranges (tile_shader.slang:99-125). ranges_dev [G,2]
+ * is fully written: empty tiles get [0,0] as the reference's zero-initialised buffer. */
+int rm_tile_ranges(const int64_t* sorted_keys_dev, int64_t n, int32_t* ranges_dev,
+                   int64_t n_tiles, rm_stream_t stream);
+
+/* ---- fused binning: the MI355X path behind vertex_and_tile_shader ------------------------
+ * Replaces tile_shader_slang.py:36-68 (cumsum + generate_keys + CUB sort + ranges) by a
+ * stable radix sort of the visible tets by depth key followed by a stable radix sort of the
+ * (tile, tet) pairs by tile id. The result is bit-identical to the reference's stable sort of
+ * (tile << 32 | sortable(depth)) keys (equal keys keep tet-index order).
+ * Phase 1 (rm_bin_count) compacts visible tets, scans pair offsets and writes the totals
+ * {n_visible, n_pairs} to counts_dev (int64[2]); the caller reads n_pairs to size phase 2. */
+size_t rm_bin_workspace_size(int64_t n_tets, int64_t n_pairs, int64_t n_tiles);
+int rm_bin_count(const int32_t* tiles_touched_dev, int64_t n_tets, int64_t* counts_dev,
+                 void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect_dev, const float* vs_dev,
+                int64_t n_tets, int64_t n_visible, int64_t n_pairs,
+                int32_t grid_height, int32_t grid_width,
+                int32_t* sorted_tet_dev, int32_t* ranges_dev,
+                void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+
+/* ---- per-tile compositing ---------------------------------------------------------------
+ * Replaces splat_tiled (alphablend_shader_interp.slang:371-431, launched by
+ * AlphaBlendTiledRender.forward, alphablend_tiled_slang_interp.py:11-67).
+ * n_features = 7 ([density, rgb, grad]: one colour gradient shared by rgb, the reference's
+ * layout) or 13 ([density, rgb, 3x3 per-channel gradient rows]). Tiles: square 4, 8 or 16.
+ * out_img [H,W,4] = (rgb, log-transmittance), out_dist [H,W,5], out_ncontrib [H,W] i32.
+ * n_sorted is the length of sorted_tet_dev; tile ranges outside [0, n_sorted] count as empty. */
+int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                     const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
+                     const float* features_dev, int32_t n_features, const float* jitter_dev,
+                     float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
+                     rm_stream_t stream);
+
+/* Replaces splat_tiled.bwd / bwd_alpha_blend (alphablend_shader_interp.slang:273-369,
+ * alphablend_tiled_slang_interp.py:70-112). Walks each pixel's list back to front from the
+ * stored final state, reduces per-tet gradients across the wavefront, then adds them with
+ * one atomic per (wave, tet, parameter) into grad_vertices_dev [V,3] / grad_features_dev [T,F]. */
+int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                     const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
+                     const float* features_dev, int32_t n_features, const float* jitter_dev,
+                     const float* img_dev, const float* dist_dev, const int32_t* ncontrib_dev,
+                     const float* grad_img_dev, const float* grad_dist_dev,
+                     float* grad_vertices_dev, float* grad_features_dev, rm_stream_t stream);
+
+/* ---- hash-grid field ----------------------------------------------------------------------
+ * Replaces HashEmbedderOptimized.forward (utils/hashgrid.py:74-124): multi-level grid over
+ * [0,1]^3, per-level resolution res[l] and table offset offs[l] (entries, level-concatenated
+ * table of n_feat floats per entry; offs has n_levels + 1 entries); a level is dense (modulo wrap) when res^3 <= 2^log2_size,
+ * else hashed with primes (1, 2654435761, 805459861). Output [N, L*n_feat] level-major. */
+int rm_hashgrid_fwd(const float* x_dev, int64_t n, const float* table_dev, const int32_t* res,
+                    const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
+                    float* out_dev, rm_stream_t stream);
+int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_out_dev, const int32_t* res,
+                    const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
+                    float* grad_table_dev, rm_stream_t stream);
+
+/* Field MLP 32 -> 64 -> 64 -> n_out (n_out <= 16), SELU hidden activations, weights row-major
+ * [out,in] as torch.nn.Linear (models/ingp_color.py:141-151). precision 0 = FP32 operands,
+ * 1 = BF16 operands; both contract on MFMA with FP32 accumulation. The backward recomputes the
+ * hidden activations from x (the reference checkpoints the field, models/ingp_color.py:351)
+ * and adds weight and bias gradients into caller-zeroed gw1..gb3; grad_x_dev may be null. */
+int rm_mlp_fwd(const float* x_dev, int64_t n, const float* w1, const float* b1, const float* w2,
+               const float* b2, const float* w3, const float* b3, int32_t n_out, int32_t precision,
+               float* y_dev, rm_stream_t stream);
+int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const float* b1, const float* w2,
+               const float* b2, const float* w3, const float* b3, int32_t n_out, int32_t precision,
+               const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1, float* gw2,
+               float* gb2, float* gw3, float* gb3, rm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RMESH_H */

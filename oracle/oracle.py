@@ -1,0 +1,253 @@
+"""ctypes wrapper of the CPU oracle (librmesh_oracle.so) plus the numpy field oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline leg. The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librmesh_oracle.so")
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("w2v", C.c_float * 16), ("K", C.c_float * 9), ("cam_pos", C.c_float * 3),
+                ("dist", C.c_float * 4), ("min_t", C.c_float), ("fovy", C.c_float),
+                ("fovx", C.c_float), ("H", C.c_int32), ("W", C.c_int32), ("gh", C.c_int32),
+                ("gw", C.c_int32), ("th", C.c_int32), ("tw", C.c_int32),
+                ("camera_type", C.c_int32), ("pad_", C.c_int32)]
+
+
+def make_camera(cam, tile_size: int, min_t: float) -> OrcCamera:
+    c = OrcCamera()
+    c.w2v[:] = [float(v) for v in np.asarray(cam.w2v, np.float32).reshape(-1)]
+    c.K[:] = [float(v) for v in np.asarray(cam.K, np.float32).reshape(-1)]
+    c.cam_pos[:] = [float(v) for v in np.asarray(cam.cam_pos, np.float32).reshape(-1)]
+    c.dist[:] = [float(v) for v in np.asarray(cam.distortion, np.float32).reshape(-1)]
+    c.min_t = min_t
+    c.fovy = cam.fovy
+    c.fovx = cam.fovx
+    c.H, c.W = cam.height, cam.width
+    c.th = c.tw = tile_size
+    c.gh = (cam.height + tile_size - 1) // tile_size
+    c.gw = (cam.width + tile_size - 1) // tile_size
+    c.camera_type = cam.camera_type
+    return c
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_project.argtypes = [C.POINTER(OrcCamera), P, P, C.c_int64, P, P, P, P]
+        L.orc_bin.argtypes = [P, P, P, C.c_int64, C.c_int32, C.c_int32, P, P, P]
+        L.orc_bin.restype = C.c_int64
+        L.orc_sortable.argtypes = [C.c_float]
+        L.orc_sortable.restype = C.c_uint32
+        L.orc_composite_fwd.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, P, P, P]
+        L.orc_composite_bwd.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, P, P,
+                                        P, P, P, P]
+        L.orc_quadrature.argtypes = [C.POINTER(OrcCamera), P, P, C.c_int64, C.c_double, C.c_int,
+                                     P, P, P, C.c_int, C.c_int64, P]
+        L.orc_get_ray.argtypes = [C.POINTER(OrcCamera), C.c_double, C.c_double, P, P]
+        L.orc_intersect.argtypes = [P, P, P, P, P]
+        L.orc_intersect.restype = C.c_int
+        L.orc_power_of_circumsphere.argtypes = [P, P, P, P, P, P]
+        L.orc_power_of_circumsphere.restype = C.c_double
+        L.orc_power_bwd.argtypes = [P, P, C.c_double, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def project(cam: OrcCamera, indices, vertices):
+    idx = np.ascontiguousarray(indices, np.int32)
+    v = np.ascontiguousarray(vertices, np.float32)
+    T = idx.shape[0]
+    tiles = np.zeros(T, np.int32)
+    rect = np.zeros((T, 4), np.int32)
+    vs = np.zeros((T, 3), np.float32)
+    cc = np.zeros((T, 3), np.float32)
+    lib().orc_project(C.byref(cam), _p(idx), _p(v), T, _p(tiles), _p(rect), _p(vs), _p(cc))
+    return tiles, rect, vs, cc
+
+
+def bin_tiles(cam: OrcCamera, tiles, rect, vs):
+    P = int(tiles.astype(np.int64).sum())
+    keys = np.zeros(max(P, 1), np.int64)
+    vals = np.zeros(max(P, 1), np.int32)
+    ranges = np.zeros((cam.gh * cam.gw, 2), np.int32)
+    lib().orc_bin(_p(np.ascontiguousarray(tiles)), _p(np.ascontiguousarray(rect)),
+                  _p(np.ascontiguousarray(vs)), tiles.shape[0], cam.gh, cam.gw, _p(keys),
+                  _p(vals), _p(ranges))
+    return keys[:P], vals[:P], ranges
+
+
+def composite_fwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter):
+    H, W = cam.H, cam.W
+    F = feats.shape[1]
+    img = np.zeros((H, W, 4))
+    dist = np.zeros((H, W, 5))
+    nc = np.zeros((H, W), np.int32)
+    margin = np.zeros((H, W))
+    v = np.ascontiguousarray(vertices, np.float64)
+    f = np.ascontiguousarray(feats, np.float64)
+    j = np.ascontiguousarray(jitter, np.float32)
+    lib().orc_composite_fwd(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
+                            _p(np.ascontiguousarray(ranges, np.int32)),
+                            _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F, _p(j),
+                            _p(img), _p(dist), _p(nc), _p(margin))
+    return img, dist, nc, margin
+
+
+def composite_bwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter, img, dist,
+                  nc, gimg, gdist):
+    F = feats.shape[1]
+    v = np.ascontiguousarray(vertices, np.float64)
+    f = np.ascontiguousarray(feats, np.float64)
+    gv = np.zeros_like(v)
+    gf = np.zeros_like(f)
+    lib().orc_composite_bwd(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
+                            _p(np.ascontiguousarray(ranges, np.int32)),
+                            _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F,
+                            _p(np.ascontiguousarray(jitter, np.float32)),
+                            _p(np.ascontiguousarray(img, np.float64)),
+                            _p(np.ascontiguousarray(dist, np.float64)),
+                            _p(np.ascontiguousarray(nc, np.int32)),
+                            _p(np.ascontiguousarray(gimg, np.float64)),
+                            _p(np.ascontiguousarray(gdist, np.float64)), _p(gv), _p(gf))
+    return gv, gf
+
+
+def quadrature(cam: OrcCamera, jitter, pix, t_far, n_samples, indices, vertices, feats):
+    pix = np.ascontiguousarray(pix, np.int64)
+    out = np.zeros((len(pix), 4))
+    lib().orc_quadrature(C.byref(cam), _p(np.ascontiguousarray(jitter, np.float32)), _p(pix),
+                         len(pix), float(t_far), int(n_samples),
+                         _p(np.ascontiguousarray(indices, np.int32)),
+                         _p(np.ascontiguousarray(vertices, np.float64)),
+                         _p(np.ascontiguousarray(feats, np.float64)), feats.shape[1],
+                         indices.shape[0], _p(out))
+    return out
+
+
+def get_ray(cam: OrcCamera, px, py):
+    o = np.zeros(3)
+    d = np.zeros(3)
+    lib().orc_get_ray(C.byref(cam), float(px), float(py), _p(o), _p(d))
+    return o, d
+
+
+def intersect(v12, o, d):
+    v = np.ascontiguousarray(v12, np.float64).reshape(12)
+    e = np.zeros(1)
+    x = np.zeros(1)
+    hit = lib().orc_intersect(_p(v), _p(np.ascontiguousarray(o, np.float64)),
+                              _p(np.ascontiguousarray(d, np.float64)), _p(e), _p(x))
+    return bool(hit), float(e[0]), float(x[0])
+
+
+def power_of_circumsphere(A, B, C_, D, P):
+    ctr = np.zeros(3)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (A, B, C_, D, P)]
+    pw = lib().orc_power_of_circumsphere(*[_p(a) for a in arrs], _p(ctr))
+    return pw, ctr
+
+
+def power_bwd(V12, P, g_power, g_center):
+    g = np.zeros(12)
+    lib().orc_power_bwd(_p(np.ascontiguousarray(V12, np.float64).reshape(12)),
+                        _p(np.ascontiguousarray(P, np.float64)), float(g_power),
+                        _p(np.ascontiguousarray(g_center, np.float64)), _p(g))
+    return g.reshape(4, 3)
+
+
+def project_bwd(cam: OrcCamera, indices, vertices, tiles, grad_vs, grad_cc):
+    """FP64 adjoint of the projection outputs (vs[:,2] and circumcenter) w.r.t. vertices."""
+    v = np.asarray(vertices, np.float64)
+    gv = np.zeros_like(v)
+    P = np.array(cam.cam_pos[:], np.float64)
+    for t in np.nonzero(tiles > 0)[0]:
+        q = indices[t]
+        g = power_bwd(v[q].reshape(12), P, float(grad_vs[t, 2]), np.asarray(grad_cc[t], np.float64))
+        np.add.at(gv, q, g)
+    return gv
+
+
+# --------------------------------------------------------------------------- field oracle
+
+PRIMES = (1, 2654435761, 805459861)
+
+
+def hashgrid_levels(n_levels, base_res, growth, log2_size):
+    """Per-level resolution floor(base * growth**l) and table sizes min(2^log2, res^3)
+    (utils/hashgrid.py:35-48)."""
+    res = [int(np.floor(base_res * growth ** l)) for l in range(n_levels)]
+    sizes = [min(2 ** log2_size, r ** 3) for r in res]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    return np.array(res, np.int32), offs
+
+
+def hashgrid_fwd(x, table, res, offs, n_feat, log2_size):
+    """FP64 restatement of HashEmbedderOptimized.forward (utils/hashgrid.py:74-124) on the
+    [0,1]^3 box: clamp, pos = x*res + 0.5, floor, 8 corners, dense modulo index or XOR-prime
+    hash, trilinear interpolation x->y->z. Returns [N, L*n_feat] level-major."""
+    x = np.clip(np.asarray(x, np.float64), 0.0, 1.0)
+    N = x.shape[0]
+    L = len(res)
+    out = np.zeros((N, L * n_feat))
+    mask = (1 << log2_size) - 1
+    for l in range(L):
+        r = int(res[l])
+        pos = x * r + 0.5
+        fl = np.floor(pos).astype(np.int64)
+        fr = pos - fl
+        acc = np.zeros((N, n_feat))
+        for c in range(8):
+            off = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+            vi = fl + off
+            if r ** 3 <= (1 << log2_size):
+                w = vi % r
+                h = w[:, 0] + w[:, 1] * r + w[:, 2] * r * r
+            else:
+                u = vi.astype(np.uint64)
+                h = (u[:, 0] * np.uint64(PRIMES[0])) ^ (u[:, 1] * np.uint64(PRIMES[1])) ^ \
+                    (u[:, 2] * np.uint64(PRIMES[2]))
+                h = (h & np.uint64(mask)).astype(np.int64)
+            wt = np.prod(np.where(off[None, :] == 1, fr, 1.0 - fr), axis=1)
+            acc += wt[:, None] * table[offs[l] + h]
+        out[:, l * n_feat:(l + 1) * n_feat] = acc
+    return out
+
+
+SELU_ALPHA = 1.6732632423543772
+SELU_SCALE = 1.0507009873554805
+
+
+def selu(x):
+    return SELU_SCALE * np.where(x > 0, x, SELU_ALPHA * np.expm1(np.minimum(x, 0)))
+
+
+def mlp_fwd(x, w1, b1, w2, b2, w3, b3):
+    """FP64 Linear->SELU->Linear->SELU->Linear (models/ingp_color.py:141-151)."""
+    h1 = x @ w1.T + b1
+    h2 = selu(h1) @ w2.T + b2
+    return selu(h2) @ w3.T + b3

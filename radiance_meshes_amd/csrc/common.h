@@ -1,0 +1,72 @@
+// Shared device/host helpers for the rmesh HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/rmesh.h"
+
+namespace rmesh {
+
+constexpr int kWave = 64;
+
+// Thread-local error message behind rm_last_error().
+void set_error(const std::string& msg);
+
+#define RM_CHECK_ARG(cond, msg)                         \
+    do {                                                \
+        if (!(cond)) {                                  \
+            ::rmesh::set_error(msg);                    \
+            return RM_ERR_INVALID_ARGUMENT;             \
+        }                                               \
+    } while (0)
+
+#define RM_CHECK_LAUNCH(what)                                                         \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess) {                                                       \
+            ::rmesh::set_error(std::string(what) + ": " + hipGetErrorString(e_));     \
+            return RM_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+inline hipStream_t as_stream(rm_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Camera as passed to kernels by value. ray_rot is transpose(w2v[:3,:3]) (the view->world
+// rotation of camera.slang:200-207), precomputed on the host.
+struct CamArgs {
+    float w2v[16];
+    float K[9];
+    float cam_pos[3];
+    float dist[4];
+    float ray_rot[9];
+    float min_t;
+    int H, W, gh, gw, th, tw, type;
+    int distorted;   // dot(dist, dist) >= 1e-9 (camera.slang:244,360)
+};
+
+inline CamArgs make_cam_args(const rm_camera& c) {
+    CamArgs a;
+    for (int i = 0; i < 16; ++i) a.w2v[i] = c.w2v[i];
+    for (int i = 0; i < 9; ++i) a.K[i] = c.K[i];
+    for (int i = 0; i < 3; ++i) a.cam_pos[i] = c.cam_pos[i];
+    for (int i = 0; i < 4; ++i) a.dist[i] = c.dist[i];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) a.ray_rot[3 * r + k] = c.w2v[4 * k + r];
+    a.min_t = c.min_t;
+    a.H = c.image_height;
+    a.W = c.image_width;
+    a.gh = c.grid_height;
+    a.gw = c.grid_width;
+    a.th = c.tile_height;
+    a.tw = c.tile_width;
+    a.type = c.camera_type;
+    const float* k = c.dist;
+    a.distorted = !((k[0] * k[0] + k[1] * k[1] + k[2] * k[2] + k[3] * k[3]) < 1e-9f);
+    return a;
+}
+
+}  // namespace rmesh

@@ -1,0 +1,738 @@
+// Per-tile front-to-back compositing of tetrahedra and its backward.
+//
+// Forward semantics: splat_tiled / alpha_blend (alphablend_shader_interp.slang:194-271,371-431),
+// ray_tetrahedron_intersect_fused (intersect.slang:393-624), integrate_tetra_segment
+// (interp_version.slang:21-98), get_ray (camera.slang:298-346). Per pixel, the tile's tets are
+// visited in sorted order; a hit with sigma*dt > 0 is blended with the exact integral of a
+// linear colour under constant density, n_contrib counts every visited tet up to and including
+// the one after which log-transmittance drops below -5.5.
+//
+// MI355X layout: a workgroup of max(64, tile^2) lanes holds 1, 1 or 4 tiles (16x16, 8x8, 4x4).
+// Each chunk of a tile's list is staged in LDS as per-view plane records computed once per
+// (tile, tet): the four outward face planes n_i with offsets e_i = n_i . (cam - v_a), and
+// the colour folded to c(s) = base + g.(cam - v0) + s * g.d, where s is the distance from the
+// camera centre along the unit ray. A plane crossing is then s_i = -e_i / (n_i . d) and the
+// reference's ray parameter is t = s - min_t (its origin is cam + min_t * d).
+//
+// Backward: the reference's structure (bwd_alpha_blend, alphablend_shader_interp.slang:273-369):
+// each pixel walks its list back to front from the stored final state, undoing log-transmittance
+// and the distortion moments, skipping entries at or beyond its n_contrib. Per-tet gradients
+// are accumulated as adjoints of the plane records (per plane: sum of ds/den and of
+// (ds*s/den)*d), summed across the wavefront (or 16-lane tile group) with cross-lane shuffles,
+// and converted once per (wave, tet) to vertex and feature gradients, which lanes of the
+// wave then add with one float atomic each.
+#include "common.h"
+
+namespace rmesh {
+namespace {
+
+template <int TS>
+struct Cfg {
+    static constexpr int GROUP = TS * TS;                 // pixels per tile
+    static constexpr int WG = GROUP < 64 ? 64 : GROUP;    // lanes per workgroup
+    static constexpr int NG = WG / GROUP;                 // tiles per workgroup
+    static constexpr int GX = NG >= 2 ? 2 : 1;            // tiles per workgroup along x
+    static constexpr int GY = NG / GX;
+    static constexpr int RED = GROUP < 64 ? GROUP : 64;   // lanes summed per reduction
+};
+
+template <int F>
+struct NG_ {
+    static constexpr int NGRD = F == 7 ? 3 : 9;           // colour-gradient floats per tet
+};
+
+struct Ray {
+    float dx, dy, dz;
+};
+
+__device__ inline float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ inline float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline float3 operator*(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float3 cross3(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// camera.slang:241-296 undistort (Newton, 10 iterations) and 298-346 get_ray.
+__device__ inline Ray make_ray(const CamArgs& c, float px, float py) {
+    float x = (px - c.K[2]) / c.K[0];
+    float y = (py - c.K[5]) / c.K[4];
+    if ((c.type == 0 || c.type == 1) && c.distorted) {
+        const float k1 = c.dist[0], k2 = c.dist[1], p1 = c.dist[2], p2 = c.dist[3];
+        const float xd = x, yd = y;
+        for (int i = 0; i < 10; ++i) {
+            float r2 = x * x + y * y;
+            float d = 1.0f + r2 * (k1 + r2 * k2);
+            float fx = d * x + 2.0f * p1 * x * y + p2 * (r2 + 2.0f * x * x) - xd;
+            float fy = d * y + 2.0f * p2 * x * y + p1 * (r2 + 2.0f * y * y) - yd;
+            float d_r = k1 + r2 * (2.0f * k2);
+            float d_x = 2.0f * x * d_r, d_y = 2.0f * y * d_r;
+            float fx_x = d + d_x * x + 2.0f * p1 * y + 6.0f * p2 * x;
+            float fx_y = d_y * x + 2.0f * p1 * x + 2.0f * p2 * y;
+            float fy_x = d_x * y + 2.0f * p2 * y + 2.0f * p1 * x;
+            float fy_y = d + d_y * y + 2.0f * p2 * x + 6.0f * p1 * y;
+            float den = fy_x * fx_y - fx_x * fy_y;
+            if (fabsf(den) < 1e-9f) break;
+            x += (fx * fy_y - fy * fx_y) / den;
+            y += (fy * fx_x - fx * fy_x) / den;
+        }
+    }
+    float3 dc;
+    if (c.type == 1) {
+        float th = fminf(3.14159265f, sqrtf(x * x + y * y));
+        float s = th < 1e-6f ? 1.0f : sinf(th) / th;
+        dc = f3(x * s, y * s, cosf(th));
+    } else {
+        float inv = 1.0f / sqrtf(x * x + y * y + 1.0f);
+        dc = f3(x * inv, y * inv, inv);
+    }
+    const float* R = c.ray_rot;
+    float3 d = f3(R[0] * dc.x + R[1] * dc.y + R[2] * dc.z, R[3] * dc.x + R[4] * dc.y + R[5] * dc.z,
+                  R[6] * dc.x + R[7] * dc.y + R[8] * dc.z);
+    float inv = 1.0f / sqrtf(dot3(d, d));
+    return {d.x * inv, d.y * inv, d.z * inv};
+}
+
+__device__ inline float3 load_v(const float* __restrict__ v, int i) {
+    const float* p = v + 3 * (int64_t)i;
+    return f3(p[0], p[1], p[2]);
+}
+
+// Outward plane of face (a, b, c) opposite vertex o: n = +-cross(b - a, c - a), flipped when it
+// points towards o (intersect.slang:455-479). Returns the sign applied.
+__device__ inline float face_plane(float3 a, float3 b, float3 c, float3 o, float3& n) {
+    n = cross3(b - a, c - a);
+    if (dot3(n, o - a) > 0.f) {
+        n = f3(-n.x, -n.y, -n.z);
+        return -1.f;
+    }
+    return 1.f;
+}
+
+struct Hit {
+    bool hit;
+    float se, sx;      // clamped entry / exit distance from the camera along the ray
+    float se_raw;      // entering plane distance before clamping at min_t
+    float den_e, den_x;
+    int ie, ix;
+};
+
+// ray_tetrahedron_intersect_fused on plane records (n_i, e_i) for a ray from cam along d.
+__device__ inline Hit intersect(const float4 pl[4], Ray r, float mt) {
+    Hit h;
+    float se = -1e30f, sx = 1e30f, de = 1.f, dx = 1.f;
+    int ie = -1, ix = -1;
+    bool miss = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
+        if (fabsf(den) < 1e-10f) {
+            if (pl[i].w + mt * den > 0.f) miss = true;
+        } else {
+            float s = -pl[i].w * __builtin_amdgcn_rcpf(den);
+            if (den < 0.f) {
+                if (s > se) { se = s; de = den; ie = i; }
+            } else {
+                if (s < sx) { sx = s; dx = den; ix = i; }
+            }
+        }
+    }
+    h.hit = !miss && !(se > sx) && (sx > mt);
+    h.se_raw = se;
+    h.se = fmaxf(se, mt);
+    h.sx = fmaxf(sx, h.se);
+    h.den_e = de;
+    h.den_x = dx;
+    h.ie = ie;
+    h.ix = ix;
+    return h;
+}
+
+// phi(x) = (1 - e^-x) / x with the reference's small-x branch (interp_version.slang:21-28).
+__device__ inline float phi_of(float x, float em1) { return x < 1e-6f ? 1.0f - 0.5f * x : -em1 / x; }
+
+// phi'(x): series below 0.05 (truncation < 1e-9 relative), closed form above.
+__device__ inline float dphi_of(float x, float em1, float alpha) {
+    if (x < 1e-6f) return -0.5f;
+    if (x < 0.05f) return -0.5f + x * (1.f / 3.f + x * (-1.f / 8.f + x * (1.f / 30.f - x * (1.f / 144.f))));
+    return (x * alpha + em1) / (x * x);
+}
+
+// XCD-aware workgroup order: consecutive dispatch slots go round-robin over the 8 XCDs, so
+// slot b is remapped to make each XCD own a contiguous band of tiles (shared tets stay in its L2).
+__device__ inline int remap_block(int b, int nb) {
+    int per = nb >> 3;
+    if (b < (per << 3)) return (b & 7) * per + (b >> 3);
+    return b;
+}
+
+// A tile's [start, end) range; ranges outside [0, n_sorted] are treated as empty.
+__device__ inline int2 load_range(const int2* __restrict__ ranges, bool tile_ok, int tile, int64_t n_sorted) {
+    if (!tile_ok) return make_int2(0, 0);
+    int2 rg = ranges[tile];
+    if (rg.x < 0 || rg.y < rg.x || (int64_t)rg.y > n_sorted) return make_int2(0, 0);
+    return rg;
+}
+
+template <int TS>
+__device__ inline void locate(const CamArgs& cam, int& g, int& r, int& tx, int& ty, int& px, int& py,
+                              bool& tile_ok, bool& inside) {
+    using C = Cfg<TS>;
+    int b = remap_block(blockIdx.x, gridDim.x);
+    int sgw = (cam.gw + C::GX - 1) / C::GX;
+    int sx = b % sgw, sy = b / sgw;
+    g = threadIdx.x / C::GROUP;
+    r = threadIdx.x % C::GROUP;
+    tx = sx * C::GX + (g % C::GX);
+    ty = sy * C::GY + (g / C::GX);
+    tile_ok = tx < cam.gw && ty < cam.gh;
+    px = tx * TS + (r % TS);
+    py = ty * TS + (r / TS);
+    inside = tile_ok && px < cam.W && py < cam.H;
+}
+
+// LDS staging of one chunk: per slot, 4 planes, (sigma, folded base rgb), colour gradients.
+template <int WG, int F, bool BWD>
+struct Stage {
+    float4 pl[4][WG];
+    float4 fa[WG];                     // sigma, base' r, g, b
+    float4 fg[(NG_<F>::NGRD + 3) / 4][WG];
+    float4 vtx[BWD ? 3 : 1][BWD ? WG : 1];  // v0..v3 packed (backward epilogue)
+    int meta[BWD ? 2 : 1][BWD ? WG : 1];     // tet id, face sign bits
+};
+
+template <int WG, int F, bool BWD>
+__device__ inline void stage_tet(Stage<WG, F, BWD>& S, int slot, int tet, const CamArgs& cam,
+                                 const int4* __restrict__ indices, const float* __restrict__ verts,
+                                 const float* __restrict__ feats) {
+    constexpr int NGRD = NG_<F>::NGRD;
+    int4 q = indices[tet];
+    float3 v0 = load_v(verts, q.x), v1 = load_v(verts, q.y), v2 = load_v(verts, q.z), v3 = load_v(verts, q.w);
+    float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
+    float3 n;
+    int signs = 0;
+    float sg;
+    sg = face_plane(v0, v1, v2, v3, n);
+    S.pl[0][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
+    signs |= (sg < 0) << 0;
+    sg = face_plane(v0, v1, v3, v2, n);
+    S.pl[1][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
+    signs |= (sg < 0) << 1;
+    sg = face_plane(v0, v2, v3, v1, n);
+    S.pl[2][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
+    signs |= (sg < 0) << 2;
+    sg = face_plane(v1, v2, v3, v0, n);
+    S.pl[3][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v1));
+    signs |= (sg < 0) << 3;
+    const float* f = feats + (int64_t)F * tet;
+    float g[NGRD];
+#pragma unroll
+    for (int k = 0; k < NGRD; ++k) g[k] = f[4 + k];
+    float3 cv = cp - v0;
+    float b0, b1, b2;
+    if (F == 7) {
+        float g0 = g[0] * cv.x + g[1] * cv.y + g[2] * cv.z;
+        b0 = f[1] + g0; b1 = f[2] + g0; b2 = f[3] + g0;
+    } else {
+        b0 = f[1] + (g[0] * cv.x + g[1] * cv.y + g[2] * cv.z);
+        b1 = f[2] + (g[3 % NGRD] * cv.x + g[4 % NGRD] * cv.y + g[5 % NGRD] * cv.z);
+        b2 = f[3] + (g[6 % NGRD] * cv.x + g[7 % NGRD] * cv.y + g[8 % NGRD] * cv.z);
+    }
+    S.fa[slot] = make_float4(fmaxf(f[0], 0.f), b0, b1, b2);
+#pragma unroll
+    for (int k = 0; k < (NGRD + 3) / 4; ++k)
+        S.fg[k][slot] = make_float4(g[4 * k], 4 * k + 1 < NGRD ? g[(4 * k + 1) % NGRD] : 0.f,
+                                    4 * k + 2 < NGRD ? g[(4 * k + 2) % NGRD] : 0.f,
+                                    4 * k + 3 < NGRD ? g[(4 * k + 3) % NGRD] : 0.f);
+    if constexpr (BWD) {
+        S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
+        S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
+        S.vtx[2][slot] = make_float4(v2.z, v3.x, v3.y, v3.z);
+        S.meta[0][slot] = tet;
+        S.meta[1][slot] = signs;
+    }
+}
+
+template <int WG, int F, bool BWD>
+__device__ inline void read_grads(const Stage<WG, F, BWD>& S, int slot, float* g) {
+    constexpr int NGRD = NG_<F>::NGRD;
+#pragma unroll
+    for (int k = 0; k < (NGRD + 3) / 4; ++k) {
+        float4 v = S.fg[k][slot];
+        if (4 * k + 0 < NGRD) g[4 * k + 0] = v.x;
+        if (4 * k + 1 < NGRD) g[(4 * k + 1) % NGRD] = v.y;
+        if (4 * k + 2 < NGRD) g[(4 * k + 2) % NGRD] = v.z;
+        if (4 * k + 3 < NGRD) g[(4 * k + 3) % NGRD] = v.w;
+    }
+}
+
+// ------------------------------------------------------------------------------- forward --
+
+template <int TS, int F>
+__global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
+    int* __restrict__ out_nc) {
+    using C = Cfg<TS>;
+    constexpr int NGRD = NG_<F>::NGRD;
+    __shared__ Stage<C::WG, F, false> S;
+    __shared__ int s_chunks;
+
+    int g, r, tx, ty, px, py;
+    bool tile_ok, inside;
+    locate<TS>(cam, g, r, tx, ty, px, py, tile_ok, inside);
+    int2 rg = load_range(ranges, tile_ok, ty * cam.gw + tx, n_sorted);
+    int n = rg.y - rg.x;
+    if (threadIdx.x == 0) s_chunks = 0;
+    __syncthreads();
+    if (r == 0) atomicMax(&s_chunks, (n + C::GROUP - 1) / C::GROUP);
+    const int64_t pix = (int64_t)py * cam.W + px;
+    Ray ray = {0.f, 0.f, 1.f};
+    if (inside) {
+        float2 j = jitter[pix];
+        ray = make_ray(cam, (float)px + j.x, (float)py + j.y);
+    }
+    const float mt = cam.min_t;
+    float cr = 0.f, cg = 0.f, cb = 0.f, logT = 0.f;
+    float X = 0.f, Y = 0.f, Z = 0.f, W = 0.f, V = 0.f;
+    int nc = 0;
+    bool alive = inside;
+    __syncthreads();
+    const int nchunks = s_chunks;
+    for (int c = 0; c < nchunks; ++c) {
+        if (!__syncthreads_or(alive)) break;
+        int k = c * C::GROUP + r;
+        if (k < n) stage_tet<C::WG, F, false>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+        __syncthreads();
+        if (alive) {
+            int cnt = n - c * C::GROUP;
+            cnt = cnt < C::GROUP ? cnt : C::GROUP;
+            const int base = g * C::GROUP;
+            for (int jj = 0; jj < cnt; ++jj) {
+                const int slot = base + jj;
+                ++nc;
+                float4 pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
+                Hit h = intersect(pl, ray, mt);
+                float4 fa = S.fa[slot];
+                float dt = h.sx - h.se;
+                float x = fa.x * dt;
+                if (!(h.hit && x > 0.f)) continue;
+                float gr[NGRD];
+                read_grads<C::WG, F, false>(S, slot, gr);
+                float em1 = expm1f(-x);
+                float alpha = 1.0f + em1;
+                float ph = phi_of(x, em1);
+                float w0 = ph - alpha, w1 = 1.0f - ph;
+                float T = __expf(logT);
+                float gd0 = gr[0] * ray.dx + gr[1] * ray.dy + gr[2] * ray.dz;
+                float gd1 = gd0, gd2 = gd0;
+                if (F == 13) {
+                    gd1 = gr[3 % NGRD] * ray.dx + gr[4 % NGRD] * ray.dy + gr[5 % NGRD] * ray.dz;
+                    gd2 = gr[6 % NGRD] * ray.dx + gr[7 % NGRD] * ray.dy + gr[8 % NGRD] * ray.dz;
+                }
+                // w0 * c(s_x) + w1 * c(s_e) with c(s) = base' + s * gd
+                float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
+                cr += T * (wsum * fa.y + wsl * gd0);
+                cg += T * (wsum * fa.z + wsl * gd1);
+                cb += T * (wsum * fa.w + wsl * gd2);
+                logT -= x;
+                float te = h.se - mt, tx_ = h.sx - mt;
+                float m1 = fa.x * (te + tx_) * 0.5f;
+                X += 2.0f * m1 * Z;
+                Y += 2.0f * fa.x * W;
+                Z += fa.x;
+                W += m1;
+                V += fa.x * fa.x * dt * (1.0f / 3.0f);
+                if (logT < -5.5f) {
+                    alive = false;
+                    break;
+                }
+            }
+        }
+    }
+    if (inside) {
+        out_img[pix] = make_float4(cr, cg, cb, logT);
+        float* d = out_dist + 5 * pix;
+        d[0] = X; d[1] = Y; d[2] = Z; d[3] = W; d[4] = V;
+        out_nc[pix] = nc;
+    }
+}
+
+// ------------------------------------------------------------------------------ backward --
+
+template <int N>
+__device__ inline void group_sum(float (&v)[N], int width) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        if (o >= width) break;
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], o, 64);
+    }
+}
+
+template <int TS, int F>
+__global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
+    const int* __restrict__ ncontrib, const float4* __restrict__ gimg, const float* __restrict__ gdist,
+    float* __restrict__ gverts, float* __restrict__ gfeats) {
+    using C = Cfg<TS>;
+    constexpr int NGRD = NG_<F>::NGRD;
+    constexpr int NCH = F == 7 ? 1 : 3;        // independent colour-gradient rows
+    // reduced per-tet values: dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], r[12]
+    constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;
+    constexpr int NOUT = 12 + F;                // vertex + feature gradients per tet
+    __shared__ Stage<C::WG, F, true> S;
+    __shared__ int s_nc[C::NG];
+
+    int g, r, tx, ty, px, py;
+    bool tile_ok, inside;
+    locate<TS>(cam, g, r, tx, ty, px, py, tile_ok, inside);
+    int2 rg = load_range(ranges, tile_ok, ty * cam.gw + tx, n_sorted);
+    const int64_t pix = (int64_t)py * cam.W + px;
+    if (threadIdx.x < C::NG) s_nc[threadIdx.x] = 0;
+    __syncthreads();
+    int nc = 0;
+    Ray ray = {0.f, 0.f, 1.f};
+    float A = 0.f, Z = 0.f, W = 0.f;
+    float dCr = 0.f, dCg = 0.f, dCb = 0.f, dA = 0.f, dX = 0.f, dY = 0.f, dZ = 0.f, dW = 0.f, dV = 0.f;
+    if (inside) {
+        nc = ncontrib[pix];
+        float2 j = jitter[pix];
+        ray = make_ray(cam, (float)px + j.x, (float)py + j.y);
+        float4 im = img[pix];
+        A = im.w;
+        Z = dist[5 * pix + 2];
+        W = dist[5 * pix + 3];
+        float4 gi = gimg[pix];
+        dCr = gi.x; dCg = gi.y; dCb = gi.z; dA = gi.w;
+        const float* gd = gdist + 5 * pix;
+        dX = gd[0]; dY = gd[1]; dZ = gd[2]; dW = gd[3]; dV = gd[4];
+        atomicMax(&s_nc[g], nc);
+    }
+    __syncthreads();
+    const int nmax_g = min(s_nc[g], rg.y - rg.x);   // list entries this tile must revisit
+    int nchunks_wg = 0;
+#pragma unroll
+    for (int q = 0; q < C::NG; ++q) {
+        int v = (s_nc[q] + C::GROUP - 1) / C::GROUP;
+        nchunks_wg = v > nchunks_wg ? v : nchunks_wg;
+    }
+    const int nchunks_g = (nmax_g + C::GROUP - 1) / C::GROUP;
+    const float mt = cam.min_t;
+    const float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
+    const int lane = threadIdx.x & 63;
+    const int red_base = lane & ~(C::RED - 1);
+    const uint64_t red_mask = (C::RED == 64) ? ~0ull : (((1ull << C::RED) - 1ull) << red_base);
+
+    for (int c = nchunks_wg - 1; c >= 0; --c) {
+        __syncthreads();
+        int k = c * C::GROUP + r;
+        if (c < nchunks_g && k < nmax_g)
+            stage_tet<C::WG, F, true>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+        __syncthreads();
+        if (c >= nchunks_g) continue;               // uniform per tile group
+        int cnt = nmax_g - c * C::GROUP;
+        cnt = cnt < C::GROUP ? cnt : C::GROUP;
+        const int base = g * C::GROUP;
+        for (int jj = cnt - 1; jj >= 0; --jj) {
+            const int slot = base + jj;
+            const int kk = c * C::GROUP + jj;
+            float acc[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] = 0.f;
+            bool active = false;
+            if (inside && kk < nc) {
+                float4 pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
+                Hit h = intersect(pl, ray, mt);
+                float4 fa = S.fa[slot];
+                const float sig = fa.x;
+                float dt = h.sx - h.se;
+                float x = sig * dt;
+                if (h.hit && x > 0.f) {
+                    active = true;
+                    float gr[NGRD];
+                    read_grads<C::WG, F, true>(S, slot, gr);
+                    float em1 = expm1f(-x);
+                    float alpha = 1.0f + em1;
+                    float ph = phi_of(x, em1);
+                    float w0 = ph - alpha, w1 = 1.0f - ph;
+                    float gd[3];
+                    gd[0] = gr[0] * ray.dx + gr[1] * ray.dy + gr[2] * ray.dz;
+                    gd[1] = gd[0];
+                    gd[2] = gd[0];
+                    if (F == 13) {
+                        gd[1] = gr[3 % NGRD] * ray.dx + gr[4 % NGRD] * ray.dy + gr[5 % NGRD] * ray.dz;
+                        gd[2] = gr[6 % NGRD] * ray.dx + gr[7 % NGRD] * ray.dy + gr[8 % NGRD] * ray.dz;
+                    }
+                    float bse[3] = {fa.y, fa.z, fa.w};
+                    float ce[3], cx[3], rgb[3];
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        ce[ch] = bse[ch] + h.se * gd[ch];
+                        cx[ch] = bse[ch] + h.sx * gd[ch];
+                        rgb[ch] = w0 * cx[ch] + w1 * ce[ch];
+                    }
+                    // undo this tet's blend
+                    const float Ap = A + x;
+                    const float T = __expf(Ap);
+                    const float te = h.se - mt, tx_ = h.sx - mt;
+                    const float m1 = sig * (te + tx_) * 0.5f;
+                    const float zp = Z - sig, wp = W - m1;
+                    const float dC[3] = {dCr, dCg, dCb};
+                    float drgb[3];
+                    float dT = 0.f;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        drgb[ch] = dC[ch] * T;
+                        dT += dC[ch] * rgb[ch];
+                    }
+                    float dxx = -dA;
+                    const float dm1 = dX * 2.0f * zp + dW;
+                    const float dm0 = dY * 2.0f * wp + dZ;
+                    const float dZn = dZ + dX * 2.0f * m1;
+                    const float dWn = dW + dY * 2.0f * sig;
+                    float dsig = dm0 + dm1 * (te + tx_) * 0.5f + dV * 2.0f * sig * dt * (1.0f / 3.0f);
+                    float ddt = dV * sig * sig * (1.0f / 3.0f);
+                    float dse = dm1 * sig * 0.5f, dsx = dm1 * sig * 0.5f;
+                    float dw0 = 0.f, dw1 = 0.f;
+                    float dqe[3], dqx[3];
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        acc[1 + ch] = drgb[ch] * (w0 + w1);
+                        dqx[ch] = drgb[ch] * w0;
+                        dqe[ch] = drgb[ch] * w1;
+                        dw0 += drgb[ch] * cx[ch];
+                        dw1 += drgb[ch] * ce[ch];
+                    }
+                    dxx += alpha * dw0 + (dw0 - dw1) * dphi_of(x, em1, alpha);
+                    dsig += dxx * dt;
+                    ddt += dxx * sig;
+                    dsx += ddt;
+                    dse -= ddt;
+                    acc[0] = dsig;
+                    if (F == 7) {
+                        float qe = dqe[0] + dqe[1] + dqe[2], qx = dqx[0] + dqx[1] + dqx[2];
+                        acc[4] = qe + qx;
+                        float rs = qe * h.se + qx * h.sx;
+                        acc[5] = rs * ray.dx; acc[6] = rs * ray.dy; acc[7] = rs * ray.dz;
+                        dse += qe * gd[0];
+                        dsx += qx * gd[0];
+                    } else {
+#pragma unroll
+                        for (int ch = 0; ch < 3; ++ch) {
+                            acc[4 + ch] = dqe[ch] + dqx[ch];
+                            float rs = dqe[ch] * h.se + dqx[ch] * h.sx;
+                            acc[7 + 3 * ch] = rs * ray.dx;
+                            acc[8 + 3 * ch] = rs * ray.dy;
+                            acc[9 + 3 * ch] = rs * ray.dz;
+                            dse += dqe[ch] * gd[ch];
+                            dsx += dqx[ch] * gd[ch];
+                        }
+                    }
+                    // plane adjoints: s_i = -e_i / den_i
+                    const int PU = 4 + NCH + 3 * NCH;
+                    float ue = (h.se_raw > mt) ? dse / h.den_e : 0.f;
+                    float ux = dsx / h.den_x;
+                    float re = ue * h.se_raw, rx = ux * h.sx;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float u = (h.ie == i ? ue : 0.f) + (h.ix == i ? ux : 0.f);
+                        float rr = (h.ie == i ? re : 0.f) + (h.ix == i ? rx : 0.f);
+                        acc[PU + i] = u;
+                        acc[PU + 4 + 3 * i] = rr * ray.dx;
+                        acc[PU + 5 + 3 * i] = rr * ray.dy;
+                        acc[PU + 6 + 3 * i] = rr * ray.dz;
+                    }
+                    // step the pixel state back over this tet
+                    A = Ap; Z = zp; W = wp;
+                    dA += dT * T;
+                    dZ = dZn; dW = dWn;
+                }
+            }
+            uint64_t any = __ballot(active) & red_mask;
+            if (any == 0) continue;
+            group_sum<NV>(acc, C::RED);
+            // per-tet epilogue (uniform across the reduction group)
+            const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
+            const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x),
+                                  f3(q2.y, q2.z, q2.w)};
+            const int tet = S.meta[0][slot], signs = S.meta[1][slot];
+            float gr[NGRD];
+            read_grads<C::WG, F, true>(S, slot, gr);
+            float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
+            const float3 cv0 = cp - V4[0];
+            float outv[NOUT];
+            outv[12] = acc[0];
+            outv[13] = acc[1]; outv[14] = acc[2]; outv[15] = acc[3];
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                float Sq = acc[4 + ch];
+                float3 grow = f3(gr[3 * ch], gr[(3 * ch + 1) % NGRD], gr[(3 * ch + 2) % NGRD]);
+                gv[0] = gv[0] - Sq * grow;
+                outv[16 + 3 * ch] = Sq * cv0.x + acc[4 + NCH + 3 * ch];
+                outv[17 + 3 * ch] = Sq * cv0.y + acc[5 + NCH + 3 * ch];
+                outv[18 + 3 * ch] = Sq * cv0.z + acc[6 + NCH + 3 * ch];
+            }
+            const int PU = 4 + NCH + 3 * NCH;
+            const int fa_[4] = {0, 0, 0, 1}, fb_[4] = {1, 1, 2, 2}, fc_[4] = {2, 3, 3, 3};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float u = acc[PU + i];
+                const float3 rr = f3(acc[PU + 4 + 3 * i], acc[PU + 5 + 3 * i], acc[PU + 6 + 3 * i]);
+                const float4 pl = S.pl[i][slot];
+                const float3 nn = f3(pl.x, pl.y, pl.z);
+                const int a = fa_[i], b = fb_[i], cc = fc_[i];
+                const float3 gn = (-u) * (cp - V4[a]) - rr;
+                gv[a] = gv[a] + u * nn;
+                const float sgn = ((signs >> i) & 1) ? -1.f : 1.f;
+                const float3 gcr = sgn * gn;
+                const float3 e1 = V4[b] - V4[a], e2 = V4[cc] - V4[a];
+                const float3 ge1 = cross3(e2, gcr), ge2 = cross3(gcr, e1);
+                gv[b] = gv[b] + ge1;
+                gv[cc] = gv[cc] + ge2;
+                gv[a] = gv[a] - (ge1 + ge2);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                outv[3 * v] = gv[v].x;
+                outv[3 * v + 1] = gv[v].y;
+                outv[3 * v + 2] = gv[v].z;
+            }
+            const int l = lane - red_base;
+            if (l < NOUT) {
+                float val = 0.f;
+#pragma unroll
+                for (int i = 0; i < NOUT; ++i) val = (l == i) ? outv[i] : val;
+                if (val != 0.f) {
+                    if (l < 12) {
+                        const int4 qi = indices[tet];
+                        const int vsel = l / 3;
+                        const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
+                        unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
+                    } else {
+                        unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int TS, int F>
+int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
+               const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st) {
+    using C = Cfg<TS>;
+    int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
+    hipLaunchKernelGGL((composite_fwd_kernel<TS, F>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+                       n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
+                       (float4*)img, dist, nc);
+    return 0;
+}
+
+template <int TS, int F>
+int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
+               const float* feats, const float* jitter, const float* img, const float* dist, const int* nc,
+               const float* gimg, const float* gdist, float* gverts, float* gfeats, hipStream_t st) {
+    using C = Cfg<TS>;
+    int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
+    hipLaunchKernelGGL((composite_bwd_kernel<TS, F>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+                       n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
+                       (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats);
+    return 0;
+}
+
+int check_common(const rm_camera* cam, int32_t nf, const void* sorted, const void* ranges, const void* indices,
+                 const void* verts, const void* feats, const void* jitter, const char* who) {
+    std::string w(who);
+    if (!cam) { set_error(w + ": camera is null"); return RM_ERR_INVALID_ARGUMENT; }
+    if (nf != 7 && nf != 13) { set_error(w + ": n_features must be 7 or 13"); return RM_ERR_INVALID_ARGUMENT; }
+    if (cam->tile_width != cam->tile_height ||
+        (cam->tile_width != 4 && cam->tile_width != 8 && cam->tile_width != 16)) {
+        set_error(w + ": tile size must be square 4, 8 or 16");
+        return RM_ERR_UNSUPPORTED_TILE;
+    }
+    if (cam->image_height <= 0 || cam->image_width <= 0 ||
+        cam->grid_height != (cam->image_height + cam->tile_height - 1) / cam->tile_height ||
+        cam->grid_width != (cam->image_width + cam->tile_width - 1) / cam->tile_width) {
+        set_error(w + ": grid does not match image and tile size");
+        return RM_ERR_INVALID_ARGUMENT;
+    }
+    if (!sorted || !ranges || !indices || !verts || !feats || !jitter) {
+        set_error(w + ": null buffer");
+        return RM_ERR_INVALID_ARGUMENT;
+    }
+    if (((uintptr_t)indices & 15) || ((uintptr_t)ranges & 7) || ((uintptr_t)jitter & 7)) {
+        set_error(w + ": misaligned indices/ranges/jitter");
+        return RM_ERR_INVALID_ARGUMENT;
+    }
+    return RM_OK;
+}
+
+}  // namespace
+}  // namespace rmesh
+
+using namespace rmesh;
+
+extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                const int32_t* ranges_dev,
+                                const int32_t* indices_dev, const float* vertices_dev, const float* features_dev,
+                                int32_t n_features, const float* jitter_dev, float* out_img_dev,
+                                float* out_dist_dev, int32_t* out_ncontrib_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(n_sorted >= 0, "rm_composite_fwd: negative n_sorted");
+    if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
+    int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
+                         jitter_dev, "rm_composite_fwd");
+    if (e) return e;
+    RM_CHECK_ARG(out_img_dev && out_dist_dev && out_ncontrib_dev, "rm_composite_fwd: null output");
+    RM_CHECK_ARG(((uintptr_t)out_img_dev & 15) == 0, "rm_composite_fwd: out_img must be 16-byte aligned");
+    CamArgs a = make_cam_args(*cam);
+    hipStream_t st = as_stream(stream);
+    const int ts = cam->tile_width;
+#define RM_FWD(TS, F) launch_fwd<TS, F>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev, \
+                                        jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st)
+    if (n_features == 7) {
+        if (ts == 4) RM_FWD(4, 7); else if (ts == 8) RM_FWD(8, 7); else RM_FWD(16, 7);
+    } else {
+        if (ts == 4) RM_FWD(4, 13); else if (ts == 8) RM_FWD(8, 13); else RM_FWD(16, 13);
+    }
+#undef RM_FWD
+    RM_CHECK_LAUNCH("composite_fwd_kernel");
+    return RM_OK;
+}
+
+extern "C" int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                const int32_t* ranges_dev,
+                                const int32_t* indices_dev, const float* vertices_dev, const float* features_dev,
+                                int32_t n_features, const float* jitter_dev, const float* img_dev,
+                                const float* dist_dev, const int32_t* ncontrib_dev, const float* grad_img_dev,
+                                const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
+                                rm_stream_t stream) {
+    RM_CHECK_ARG(n_sorted >= 0, "rm_composite_bwd: negative n_sorted");
+    if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
+    int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
+                         jitter_dev, "rm_composite_bwd");
+    if (e) return e;
+    RM_CHECK_ARG(img_dev && dist_dev && ncontrib_dev && grad_img_dev && grad_dist_dev && grad_vertices_dev &&
+                     grad_features_dev,
+                 "rm_composite_bwd: null buffer");
+    RM_CHECK_ARG(((uintptr_t)img_dev & 15) == 0 && ((uintptr_t)grad_img_dev & 15) == 0,
+                 "rm_composite_bwd: img and grad_img must be 16-byte aligned");
+    CamArgs a = make_cam_args(*cam);
+    hipStream_t st = as_stream(stream);
+    const int ts = cam->tile_width;
+#define RM_BWD(TS, F) launch_bwd<TS, F>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev, \
+                                        jitter_dev, img_dev, dist_dev, ncontrib_dev, grad_img_dev, grad_dist_dev, \
+                                        grad_vertices_dev, grad_features_dev, st)
+    if (n_features == 7) {
+        if (ts == 4) RM_BWD(4, 7); else if (ts == 8) RM_BWD(8, 7); else RM_BWD(16, 7);
+    } else {
+        if (ts == 4) RM_BWD(4, 13); else if (ts == 8) RM_BWD(8, 13); else RM_BWD(16, 13);
+    }
+#undef RM_BWD
+    RM_CHECK_LAUNCH("composite_bwd_kernel");
+    return RM_OK;
+}

@@ -1,0 +1,177 @@
+// Hash-grid field: multi-level grid encode (forward) and its scatter-add adjoint (backward).
+//
+// Semantics: HashEmbedderOptimized (utils/hashgrid.py:5-124) on the [0,1]^3 box with
+// conditional hashing: level l has resolution res[l]; a level with res^3 <= 2^log2_size is
+// dense (index = x%r + (y%r) r + (z%r) r^2), otherwise hashed as
+// (x * 1) ^ (y * 2654435761) ^ (z * 805459861) & (2^log2_size - 1) (the low 32 bits of the
+// reference's int64 arithmetic). pos = x * res + 0.5 rounded as two float ops, as torch does;
+// trilinear interpolation in the reference's x -> y -> z order. Output [N, L * n_feat],
+// level-major. One thread per (point, level): consecutive threads write consecutive features.
+#include "common.h"
+
+namespace rmesh {
+namespace {
+
+constexpr int kMaxLevels = 32;
+
+struct GridArgs {
+    int res[kMaxLevels];
+    int64_t offs[kMaxLevels];
+    int dense[kMaxLevels];
+    int L, F;
+    uint32_t mask;
+};
+
+__device__ inline uint32_t corner_index(const GridArgs& g, int l, int cx, int cy, int cz) {
+    if (g.dense[l]) {
+        int r = g.res[l];
+        return (uint32_t)((cx % r) + (cy % r) * r + (cz % r) * r * r);
+    }
+    return ((uint32_t)cx * 1u ^ (uint32_t)cy * 2654435761u ^ (uint32_t)cz * 805459861u) & g.mask;
+}
+
+__device__ inline void cell(const float* __restrict__ x, int64_t n, const GridArgs& g, int l, int& bx, int& by,
+                            int& bz, float& fx, float& fy, float& fz) {
+    const float r = (float)g.res[l];
+    float p[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float v = fminf(fmaxf(x[3 * n + k], 0.0f), 1.0f);
+        p[k] = __fadd_rn(__fmul_rn(v, r), 0.5f);
+    }
+    float f0 = floorf(p[0]), f1 = floorf(p[1]), f2 = floorf(p[2]);
+    bx = (int)f0; by = (int)f1; bz = (int)f2;
+    fx = p[0] - f0; fy = p[1] - f1; fz = p[2] - f2;
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void hashgrid_fwd_kernel(const float* __restrict__ x, int64_t N,
+                                                           const float* __restrict__ table, GridArgs g,
+                                                           float* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * g.L) return;
+    int64_t n = t / g.L;
+    int l = (int)(t - n * g.L);
+    int bx, by, bz;
+    float fx, fy, fz;
+    cell(x, n, g, l, bx, by, bz, fx, fy, fz);
+    const float* tb = table + g.offs[l] * F;
+    float e[8][F];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        uint32_t h = corner_index(g, l, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
+#pragma unroll
+        for (int f = 0; f < F; ++f) e[c][f] = tb[(int64_t)h * F + f];
+    }
+    const float wx0 = 1.0f - fx, wy0 = 1.0f - fy, wz0 = 1.0f - fz;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        float c0 = e[0][f] * wx0 + e[4][f] * fx;
+        float c1 = e[1][f] * wx0 + e[5][f] * fx;
+        float c2 = e[2][f] * wx0 + e[6][f] * fx;
+        float c3 = e[3][f] * wx0 + e[7][f] * fx;
+        c0 = c0 * wy0 + c2 * fy;
+        c1 = c1 * wy0 + c3 * fy;
+        out[(n * g.L + l) * F + f] = c0 * wz0 + c1 * fz;
+    }
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restrict__ x, int64_t N,
+                                                           const float* __restrict__ gout, GridArgs g,
+                                                           float* __restrict__ gtable) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * g.L) return;
+    int64_t n = t / g.L;
+    int l = (int)(t - n * g.L);
+    float go[F];
+    bool any = false;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        go[f] = gout[(n * g.L + l) * F + f];
+        any |= go[f] != 0.f;
+    }
+    if (!any) return;
+    int bx, by, bz;
+    float fx, fy, fz;
+    cell(x, n, g, l, bx, by, bz, fx, fy, fz);
+    float* tb = gtable + g.offs[l] * F;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+        float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+        uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
+#pragma unroll
+        for (int f = 0; f < F; ++f) unsafeAtomicAdd(tb + (int64_t)h * F + f, w * go[f]);
+    }
+}
+
+int make_grid_args(const int32_t* res, const int64_t* offs, int32_t L, int32_t F, int32_t log2_size, GridArgs& g,
+                   const char* who) {
+    std::string w(who);
+    if (L <= 0 || L > kMaxLevels) { set_error(w + ": n_levels must be in [1, 32]"); return RM_ERR_INVALID_ARGUMENT; }
+    if (F != 1 && F != 2 && F != 4 && F != 8) { set_error(w + ": n_feat must be 1, 2, 4 or 8"); return RM_ERR_INVALID_ARGUMENT; }
+    if (log2_size <= 0 || log2_size > 31) { set_error(w + ": log2_size out of range"); return RM_ERR_INVALID_ARGUMENT; }
+    if (!res || !offs) { set_error(w + ": null level table"); return RM_ERR_INVALID_ARGUMENT; }
+    g.L = L;
+    g.F = F;
+    g.mask = (uint32_t)((1ull << log2_size) - 1ull);
+    for (int l = 0; l < L; ++l) {
+        if (res[l] <= 0) { set_error(w + ": resolution must be positive"); return RM_ERR_INVALID_ARGUMENT; }
+        g.res[l] = res[l];
+        g.offs[l] = offs[l];
+        double cells = (double)res[l] * res[l] * res[l];
+        g.dense[l] = cells <= (double)(1ull << log2_size);
+        int64_t need = g.dense[l] ? (int64_t)res[l] * res[l] * res[l] : (1ll << log2_size);
+        if (offs[l + 1] - offs[l] < need) { set_error(w + ": level table smaller than the level needs"); return RM_ERR_INVALID_ARGUMENT; }
+    }
+    return RM_OK;
+}
+
+}  // namespace
+}  // namespace rmesh
+
+using namespace rmesh;
+
+#define RM_FEAT_DISPATCH(F_, KERNEL, ...)                                                     \
+    switch (F_) {                                                                             \
+        case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                            \
+        case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                            \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                            \
+        default: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                           \
+    }
+
+extern "C" int rm_hashgrid_fwd(const float* x_dev, int64_t n, const float* table_dev, const int32_t* res,
+                               const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
+                               float* out_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0, "rm_hashgrid_fwd: negative n");
+    GridArgs g;
+    int e = make_grid_args(res, offs, n_levels, n_feat, log2_size, g, "rm_hashgrid_fwd");
+    if (e) return e;
+    if (n == 0) return RM_OK;
+    RM_CHECK_ARG(x_dev && table_dev && out_dev, "rm_hashgrid_fwd: null buffer");
+    int64_t total = n * n_levels;
+    dim3 grid((unsigned)ceil_div(total, 256)), block(256);
+    RM_FEAT_DISPATCH(n_feat, hashgrid_fwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, table_dev, g, out_dev);
+    RM_CHECK_LAUNCH("hashgrid_fwd_kernel");
+    return RM_OK;
+}
+
+extern "C" int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_out_dev, const int32_t* res,
+                               const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
+                               float* grad_table_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0, "rm_hashgrid_bwd: negative n");
+    GridArgs g;
+    int e = make_grid_args(res, offs, n_levels, n_feat, log2_size, g, "rm_hashgrid_bwd");
+    if (e) return e;
+    if (n == 0) return RM_OK;
+    RM_CHECK_ARG(x_dev && grad_out_dev && grad_table_dev, "rm_hashgrid_bwd: null buffer");
+    int64_t total = n * n_levels;
+    dim3 grid((unsigned)ceil_div(total, 256)), block(256);
+    RM_FEAT_DISPATCH(n_feat, hashgrid_bwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
+                     grad_table_dev);
+    RM_CHECK_LAUNCH("hashgrid_bwd_kernel");
+    return RM_OK;
+}
+
+// Field MLP entry points: filled in by mlp.hip.

@@ -1,0 +1,638 @@
+// Scans, stable LSD radix sort, key generation, tile ranges and the fused binning path.
+//
+// Radix sort: 8-bit (or narrower) digits, 2048 keys per 256-thread workgroup. Each pass is
+//   (1) per-workgroup digit histogram in LDS,
+//   (2) device-wide exclusive scan of the digit-major [digit][workgroup] count table,
+//   (3) stable scatter: wave-level digit matching with ballots gives each key its rank among
+//       equal digits in original order, keys are first sorted by digit in LDS, then written
+//       out so that consecutive lanes store consecutive addresses within a digit run.
+// The result equals a stable sort (equal keys keep input order), the contract of
+// cub::DeviceRadixSort::SortPairs used by sort_by_keys.cu:36-48.
+#include "common.h"
+
+namespace rmesh {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kScanItems = 4;                       // per thread
+constexpr int kScanTile = kThreads * kScanItems;    // 1024
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kThreads * kRsItems;        // 2048
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+template <typename T>
+__device__ inline T wave_incl_scan(T v) {
+    int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; lds must hold 4 elements.
+template <typename T>
+__device__ inline T block_excl_scan(T v, T& total, T* lds) {
+    int lane = lane_id(), w = threadIdx.x >> 6;
+    T inc = wave_incl_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        T s = lds[i];
+        if (i < w) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + inc - v;
+}
+
+// ---------------------------------------------------------------- device-wide scan ------
+
+struct LoadU32 {
+    const uint32_t* p;
+    __device__ uint32_t operator()(int64_t i) const { return p[i]; }
+};
+struct LoadI32 {
+    const int32_t* p;
+    __device__ uint32_t operator()(int64_t i) const { return (uint32_t)p[i]; }
+};
+struct LoadGather {  // tiles_touched[order[i]]
+    const int32_t* tiles;
+    const uint32_t* order;
+    __device__ uint32_t operator()(int64_t i) const { return (uint32_t)tiles[order[i]]; }
+};
+
+template <typename Load>
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, int64_t n, uint32_t* partial) {
+    __shared__ uint32_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += ld(base + k);
+    uint32_t tot;
+    block_excl_scan(s, tot, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of partial[0..m) in place; writes the grand total.
+__global__ __launch_bounds__(kThreads) void scan_partials_kernel(uint32_t* partial, int64_t m,
+                                                                 int64_t* total_out) {
+    __shared__ uint32_t lds[4];
+    uint32_t carry = 0;
+    for (int64_t c = 0; c < m; c += kScanTile) {
+        int64_t base = c + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (base + k < m) ? partial[base + k] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t pre = block_excl_scan(s, tot, lds) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (base + k < m) partial[base + k] = pre;
+            pre += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = (int64_t)carry;
+}
+
+template <typename Load, typename Out>
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, int64_t n, const uint32_t* partial,
+                                                             Out* out, int inclusive) {
+    __shared__ uint32_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? ld(base + k) : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(s, tot, lds) + partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = (Out)(inclusive ? pre + v[k] : pre);
+        pre += v[k];
+    }
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline int64_t scan_blocks(int64_t n) { return ceil_div(n, kScanTile); }
+inline size_t scan_ws_bytes(int64_t n) { return align256(sizeof(uint32_t) * (size_t)(scan_blocks(n) + 1)); }
+
+template <typename Load, typename Out>
+hipError_t device_scan(Load ld, int64_t n, Out* out, bool inclusive, uint32_t* partial,
+                       int64_t* total_out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int64_t nb = scan_blocks(n);
+    hipLaunchKernelGGL(scan_reduce_kernel<Load>, dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n, partial);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, total_out);
+    hipLaunchKernelGGL((scan_down_kernel<Load, Out>), dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n,
+                       (const uint32_t*)partial, out, inclusive ? 1 : 0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- radix sort ------------
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t n,
+                                                           int shift, int bits, uint32_t* counts, int64_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t mask = (1u << bits) - 1u;
+    int64_t base = (int64_t)blockIdx.x * kRsTile + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        int64_t i = base + r * kThreads;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < (1 << bits)) counts[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift, int bits, const uint32_t* __restrict__ offsets,
+    int64_t nb) {
+    __shared__ uint32_t s_cnt[256];
+    __shared__ uint32_t s_wcnt[4][256];
+    __shared__ uint32_t s_loc[256];
+    __shared__ uint32_t s_glob[256];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ K s_keys[kRsTile];
+    __shared__ uint32_t s_vals[kRsTile];
+
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int nbins = 1 << bits;
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    s_cnt[tid] = 0;
+    s_glob[tid] = (tid < nbins) ? offsets[(int64_t)tid * nb + blockIdx.x] : 0u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    K key[kRsItems];
+    uint32_t val[kRsItems], rank[kRsItems], dig[kRsItems];
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        int64_t i = base + r * kThreads + tid;
+        bool valid = i < n;
+        key[r] = valid ? kin[i] : K(0);
+        val[r] = valid ? vin[i] : 0u;
+        dig[r] = valid ? ((uint32_t)(key[r] >> shift) & mask) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_wcnt[q][tid] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            bool bit = valid && ((dig[r] >> b) & 1u);
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t wr = (uint32_t)__popcll(peers & lt);
+        if (valid && wr == 0) s_wcnt[w][dig[r]] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = s_cnt[dig[r]];
+            for (int q = 0; q < w; ++q) pre += s_wcnt[q][dig[r]];
+            rank[r] = pre + wr;
+        } else {
+            rank[r] = 0;
+        }
+        __syncthreads();
+        s_cnt[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+    }
+    __syncthreads();
+    uint32_t tot;
+    s_loc[tid] = block_excl_scan(s_cnt[tid], tot, s_tmp);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        if (dig[r] != 0xFFFFFFFFu) {
+            uint32_t pos = s_loc[dig[r]] + rank[r];
+            s_keys[pos] = key[r];
+            s_vals[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    int64_t cnt = n - base < kRsTile ? n - base : kRsTile;
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        int j = r * kThreads + tid;
+        if (j < cnt) {
+            K k = s_keys[j];
+            uint32_t d = (uint32_t)(k >> shift) & mask;
+            uint32_t o = s_glob[d] + ((uint32_t)j - s_loc[d]);
+            kout[o] = k;
+            vout[o] = s_vals[j];
+        }
+    }
+}
+
+inline int64_t rs_blocks(int64_t n) { return ceil_div(n, kRsTile); }
+inline size_t rs_counts_bytes(int64_t n) { return align256(sizeof(uint32_t) * 256 * (size_t)rs_blocks(n)); }
+
+// Stable LSD sort of (key, value) over key bits [begin_bit, end_bit). kin/vin are not modified;
+// the result lands in kout/vout. ktmp/vtmp are a ping-pong buffer of n entries; all three
+// buffer pairs must be distinct.
+template <typename K>
+hipError_t radix_sort(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, K* ktmp, uint32_t* vtmp,
+                      int64_t n, int begin_bit, int end_bit, uint32_t* counts, uint32_t* partial,
+                      hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int nbits = end_bit - begin_bit;
+    if (nbits <= 0) {
+        hipMemcpyAsync(kout, kin, sizeof(K) * n, hipMemcpyDeviceToDevice, st);
+        hipMemcpyAsync(vout, vin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
+        return hipGetLastError();
+    }
+    int passes = (nbits + 7) / 8;
+    int64_t nb = rs_blocks(n);
+    const K* ks = kin;
+    const uint32_t* vs = vin;
+    int shift = begin_bit;
+    for (int p = 0; p < passes; ++p) {
+        int bits = (nbits - (shift - begin_bit) + (passes - p) - 1) / (passes - p);  // split evenly
+        bool to_out = ((passes - 1 - p) % 2) == 0;
+        K* kd = to_out ? kout : ktmp;
+        uint32_t* vd = to_out ? vout : vtmp;
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, n, shift, bits,
+                           counts, nb);
+        hipError_t e = device_scan(LoadU32{counts}, (int64_t)(1 << bits) * nb, counts, false, partial,
+                                   (int64_t*)nullptr, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, vs, kd, vd, n,
+                           shift, bits, (const uint32_t*)counts, nb);
+        ks = kd;
+        vs = vd;
+        shift += bits;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- keys / ranges ----------
+
+__device__ inline uint32_t sortable_key(float v) {  // tile_shader.slang:18-47
+    uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b ^ 0x80000000u);
+}
+
+__global__ __launch_bounds__(kThreads) void generate_keys_kernel(
+    const float* __restrict__ vs, const int4* __restrict__ rect, const int32_t* __restrict__ offs,
+    int64_t T, int32_t gw, int64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int64_t o = (t == 0) ? 0 : offs[t - 1];
+    int4 r = rect[t];
+    uint64_t fk = sortable_key(vs[3 * t + 2]);
+    for (int32_t y = r.y; y < r.w; ++y)
+        for (int32_t x = r.x; x < r.z; ++x) {
+            uint64_t key = (uint64_t)(uint32_t)(y * gw + x);
+            keys[o] = (int64_t)((key << 32) | fk);
+            vals[o] = (int32_t)t;
+            ++o;
+        }
+}
+
+__global__ void zero_ranges_kernel(int2* ranges, int64_t G) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < G) ranges[i] = make_int2(0, 0);
+}
+
+template <typename TileOf>
+__global__ __launch_bounds__(kThreads) void ranges_kernel(TileOf tile_of, int64_t n, int32_t* ranges) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cur = tile_of(i);
+    if (i == 0) ranges[2 * (int64_t)cur] = 0;
+    else {
+        uint32_t prev = tile_of(i - 1);
+        if (cur != prev) {
+            ranges[2 * (int64_t)prev + 1] = (int32_t)i;
+            ranges[2 * (int64_t)cur] = (int32_t)i;
+        }
+    }
+    if (i == n - 1) ranges[2 * (int64_t)cur + 1] = (int32_t)n;
+}
+
+struct TileOfKey {
+    const int64_t* k;
+    __device__ uint32_t operator()(int64_t i) const { return (uint32_t)((uint64_t)k[i] >> 32); }
+};
+struct TileOfU32 {
+    const uint32_t* t;
+    __device__ uint32_t operator()(int64_t i) const { return t[i]; }
+};
+
+// ---------------------------------------------------------------- fused binning ----------
+
+// Per 1024-tet block: number of visible tets and their pair count.
+__global__ __launch_bounds__(kThreads) void bin_count_kernel(const int32_t* __restrict__ tiles, int64_t T,
+                                                             uint32_t* blk_vis, uint32_t* blk_pairs) {
+    __shared__ uint32_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t nv = 0, np = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < T) {
+            int32_t c = tiles[base + k];
+            nv += c > 0;
+            np += c > 0 ? (uint32_t)c : 0u;
+        }
+    uint32_t tv, tp;
+    block_excl_scan(nv, tv, lds);
+    block_excl_scan(np, tp, lds);
+    if (threadIdx.x == 0) {
+        blk_vis[blockIdx.x] = tv;
+        blk_pairs[blockIdx.x] = tp;
+    }
+}
+
+// Order-preserving compaction of the visible tets into (sortable depth key, tet id).
+__global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __restrict__ tiles,
+                                                           const float* __restrict__ vs, int64_t T,
+                                                           const uint32_t* __restrict__ blk_vis_off,
+                                                           uint32_t* __restrict__ vkey, uint32_t* __restrict__ vidx) {
+    __shared__ uint32_t lds[4];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    bool vis[kScanItems];
+    uint32_t nv = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        vis[k] = (base + k < T) && tiles[base + k] > 0;
+        nv += vis[k];
+    }
+    uint32_t tot;
+    uint32_t pos = block_excl_scan(nv, tot, lds) + blk_vis_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (vis[k]) {
+            vkey[pos] = sortable_key(vs[3 * (base + k) + 2]);
+            vidx[pos] = (uint32_t)(base + k);
+            ++pos;
+        }
+}
+
+// Expands depth-ordered visible tets into (tile id, tet id) pairs, row-major over each rect.
+// A workgroup owns 256 consecutive tets and all their pairs; lanes take pairs round-robin and
+// locate their tet by binary search over the workgroup's offsets in LDS.
+__global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ poff, const int4* __restrict__ rect,
+    int64_t nvis, int64_t P, int32_t gw, uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet) {
+    __shared__ uint32_t s_off[kThreads + 1];
+    __shared__ int4 s_rect[kThreads];
+    __shared__ uint32_t s_tet[kThreads];
+    int64_t i0 = (int64_t)blockIdx.x * kThreads;
+    int64_t i = i0 + threadIdx.x;
+    if (i < nvis) {
+        uint32_t t = order[i];
+        s_tet[threadIdx.x] = t;
+        s_rect[threadIdx.x] = rect[t];
+        s_off[threadIdx.x] = poff[i];
+    }
+    if (threadIdx.x == 0) {
+        int64_t iend = i0 + kThreads;
+        s_off[kThreads] = iend < nvis ? poff[iend] : (uint32_t)P;
+    }
+    int nloc = (int)((nvis - i0) < kThreads ? (nvis - i0) : kThreads);
+    __syncthreads();
+    if (nloc < kThreads && threadIdx.x == 0) s_off[nloc] = s_off[kThreads];
+    __syncthreads();
+    uint32_t p0 = s_off[0], p1 = s_off[nloc];
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
+        int lo = 0, hi = nloc - 1;  // last j with s_off[j] <= p
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        int4 r = s_rect[lo];
+        uint32_t m = p - s_off[lo];
+        uint32_t wdt = (uint32_t)(r.z - r.x);
+        uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
+        ptile[p] = y * (uint32_t)gw + x;
+        ptet[p] = s_tet[lo];
+    }
+}
+
+// Fused-path workspace layout (offsets in bytes). The count phase only uses the first region.
+struct BinLayout {
+    size_t blk_vis, blk_pairs, partial, counts2, vkey, vidx, vkey2, vidx2, vkey3, vidx3, poff, ptile, ptet,
+        ptile2, ptile3, ptet3, rcounts, total;
+};
+
+inline BinLayout bin_layout(int64_t T, int64_t P) {
+    BinLayout L;
+    size_t o = 0;
+    int64_t nbT = scan_blocks(T > 0 ? T : 1);
+    int64_t big = T > P ? T : P;
+    int64_t nb_rs = rs_blocks(big > 0 ? big : 1);
+    int64_t scan_n = (256 * nb_rs) > (T + 1) ? 256 * nb_rs : T + 1;
+    L.blk_vis = o; o += align256(4 * (size_t)nbT);
+    L.blk_pairs = o; o += align256(4 * (size_t)nbT);
+    L.partial = o; o += scan_ws_bytes(scan_n);
+    L.counts2 = o; o += align256(16);
+    L.vkey = o; o += align256(4 * (size_t)T);
+    L.vidx = o; o += align256(4 * (size_t)T);
+    L.vkey2 = o; o += align256(4 * (size_t)T);
+    L.vidx2 = o; o += align256(4 * (size_t)T);
+    L.vkey3 = o; o += align256(4 * (size_t)T);
+    L.vidx3 = o; o += align256(4 * (size_t)T);
+    L.poff = o; o += align256(4 * (size_t)T);
+    L.rcounts = o; o += rs_counts_bytes(big > 0 ? big : 1);
+    L.ptile = o; o += align256(4 * (size_t)P);
+    L.ptet = o; o += align256(4 * (size_t)P);
+    L.ptile2 = o; o += align256(4 * (size_t)P);
+    L.ptile3 = o; o += align256(4 * (size_t)P);
+    L.ptet3 = o; o += align256(4 * (size_t)P);
+    L.total = o;
+    return L;
+}
+
+inline int bit_length(int64_t v) {
+    int b = 0;
+    while (v > 0) { ++b; v >>= 1; }
+    return b;
+}
+
+}  // namespace
+}  // namespace rmesh
+
+using namespace rmesh;
+
+#define WS(T_, off) reinterpret_cast<T_*>(static_cast<char*>(workspace_dev) + (off))
+
+extern "C" size_t rm_scan_workspace_size(int64_t n) { return scan_ws_bytes(n > 0 ? n : 1); }
+
+extern "C" int rm_inclusive_scan_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n, void* workspace_dev,
+                                     size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0 && n < (1ll << 31), "rm_inclusive_scan_i32: n out of range");
+    if (n == 0) return RM_OK;
+    RM_CHECK_ARG(in_dev && out_dev && workspace_dev, "rm_inclusive_scan_i32: null buffer");
+    if (workspace_bytes < rm_scan_workspace_size(n)) {
+        set_error("rm_inclusive_scan_i32: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    device_scan(LoadI32{in_dev}, n, out_dev, true, (uint32_t*)workspace_dev, (int64_t*)nullptr,
+                as_stream(stream));
+    RM_CHECK_LAUNCH("rm_inclusive_scan_i32");
+    return RM_OK;
+}
+
+extern "C" int rm_generate_keys(const float* vs_dev, const int32_t* rect_dev, const int32_t* offsets_dev,
+                                int64_t n_tets, int32_t grid_height, int32_t grid_width, int64_t* keys_dev,
+                                int32_t* values_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(n_tets >= 0 && grid_height > 0 && grid_width > 0, "rm_generate_keys: bad size");
+    if (n_tets == 0) return RM_OK;
+    RM_CHECK_ARG(vs_dev && rect_dev && offsets_dev && keys_dev && values_dev, "rm_generate_keys: null buffer");
+    RM_CHECK_ARG(((uintptr_t)rect_dev & 15) == 0, "rm_generate_keys: rect must be 16-byte aligned");
+    hipLaunchKernelGGL(generate_keys_kernel, dim3((unsigned)ceil_div(n_tets, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), vs_dev, (const int4*)rect_dev, offsets_dev, n_tets, grid_width,
+                       keys_dev, values_dev);
+    RM_CHECK_LAUNCH("generate_keys_kernel");
+    return RM_OK;
+}
+
+extern "C" size_t rm_sort_pairs_workspace_size(int64_t n) {
+    if (n < 1) n = 1;
+    return align256(8 * (size_t)n) + align256(4 * (size_t)n) + rs_counts_bytes(n) +
+           scan_ws_bytes(256 * rs_blocks(n));
+}
+
+extern "C" int rm_sort_pairs(const int64_t* keys_in_dev, const int32_t* values_in_dev, int64_t* keys_out_dev,
+                             int32_t* values_out_dev, int64_t n, int32_t end_bit, void* workspace_dev,
+                             size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0 && n < (1ll << 31), "rm_sort_pairs: n out of range");
+    RM_CHECK_ARG(end_bit >= 0 && end_bit <= 64, "rm_sort_pairs: end_bit must be in [0, 64]");
+    if (n == 0) return RM_OK;
+    RM_CHECK_ARG(keys_in_dev && values_in_dev && keys_out_dev && values_out_dev && workspace_dev,
+                 "rm_sort_pairs: null buffer");
+    if (workspace_bytes < rm_sort_pairs_workspace_size(n)) {
+        set_error("rm_sort_pairs: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    size_t o = 0;
+    uint64_t* ktmp = WS(uint64_t, o); o += align256(8 * (size_t)n);
+    uint32_t* vtmp = WS(uint32_t, o); o += align256(4 * (size_t)n);
+    uint32_t* counts = WS(uint32_t, o); o += rs_counts_bytes(n);
+    uint32_t* partial = WS(uint32_t, o);
+    radix_sort<uint64_t>((const uint64_t*)keys_in_dev, (const uint32_t*)values_in_dev, (uint64_t*)keys_out_dev,
+                         (uint32_t*)values_out_dev, ktmp, vtmp, n, 0, end_bit, counts, partial,
+                         as_stream(stream));
+    RM_CHECK_LAUNCH("rm_sort_pairs");
+    return RM_OK;
+}
+
+extern "C" int rm_tile_ranges(const int64_t* sorted_keys_dev, int64_t n, int32_t* ranges_dev, int64_t n_tiles,
+                              rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0 && n_tiles > 0, "rm_tile_ranges: bad size");
+    RM_CHECK_ARG(ranges_dev != nullptr, "rm_tile_ranges: null ranges");
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(n_tiles, kThreads)), dim3(kThreads), 0, st,
+                       (int2*)ranges_dev, n_tiles);
+    if (n > 0) {
+        RM_CHECK_ARG(sorted_keys_dev != nullptr, "rm_tile_ranges: null keys");
+        hipLaunchKernelGGL(ranges_kernel<TileOfKey>, dim3((unsigned)ceil_div(n, kThreads)), dim3(kThreads), 0,
+                           st, TileOfKey{sorted_keys_dev}, n, ranges_dev);
+    }
+    RM_CHECK_LAUNCH("rm_tile_ranges");
+    return RM_OK;
+}
+
+extern "C" size_t rm_bin_workspace_size(int64_t n_tets, int64_t n_pairs, int64_t n_tiles) {
+    (void)n_tiles;
+    return bin_layout(n_tets, n_pairs).total;
+}
+
+extern "C" int rm_bin_count(const int32_t* tiles_touched_dev, int64_t n_tets, int64_t* counts_dev,
+                            void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n_tets >= 0 && n_tets < (1ll << 31), "rm_bin_count: n_tets out of range");
+    RM_CHECK_ARG(counts_dev && workspace_dev, "rm_bin_count: null buffer");
+    BinLayout L = bin_layout(n_tets, 0);
+    if (workspace_bytes < L.total) {
+        set_error("rm_bin_count: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    hipStream_t st = as_stream(stream);
+    if (n_tets == 0) {
+        hipMemsetAsync(counts_dev, 0, 2 * sizeof(int64_t), st);
+        RM_CHECK_LAUNCH("rm_bin_count");
+        return RM_OK;
+    }
+    RM_CHECK_ARG(tiles_touched_dev != nullptr, "rm_bin_count: null tiles");
+    int64_t nb = scan_blocks(n_tets);
+    hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles_touched_dev, n_tets,
+                       WS(uint32_t, L.blk_vis), WS(uint32_t, L.blk_pairs));
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, WS(uint32_t, L.blk_vis), nb,
+                       counts_dev);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, WS(uint32_t, L.blk_pairs), nb,
+                       counts_dev + 1);
+    RM_CHECK_LAUNCH("rm_bin_count");
+    return RM_OK;
+}
+
+extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect_dev, const float* vs_dev,
+                           int64_t n_tets, int64_t n_visible, int64_t n_pairs, int32_t grid_height,
+                           int32_t grid_width, int32_t* sorted_tet_dev, int32_t* ranges_dev, void* workspace_dev,
+                           size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n_tets >= 0 && n_visible >= 0 && n_visible <= n_tets, "rm_bin_sort: bad n_visible");
+    RM_CHECK_ARG(n_pairs >= 0 && n_pairs < (1ll << 31), "rm_bin_sort: n_pairs out of range");
+    RM_CHECK_ARG(grid_height > 0 && grid_width > 0, "rm_bin_sort: bad grid");
+    RM_CHECK_ARG(ranges_dev && workspace_dev, "rm_bin_sort: null buffer");
+    BinLayout L = bin_layout(n_tets, n_pairs);
+    if (workspace_bytes < L.total) {
+        set_error("rm_bin_sort: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    hipStream_t st = as_stream(stream);
+    int64_t G = (int64_t)grid_height * grid_width;
+    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (int2*)ranges_dev, G);
+    if (n_pairs == 0 || n_visible == 0) {
+        RM_CHECK_LAUNCH("rm_bin_sort");
+        return RM_OK;
+    }
+    RM_CHECK_ARG(tiles_touched_dev && rect_dev && vs_dev && sorted_tet_dev, "rm_bin_sort: null buffer");
+    RM_CHECK_ARG(((uintptr_t)rect_dev & 15) == 0, "rm_bin_sort: rect must be 16-byte aligned");
+    int64_t nb = scan_blocks(n_tets);
+    uint32_t* partial = WS(uint32_t, L.partial);
+    uint32_t* rcounts = WS(uint32_t, L.rcounts);
+    // 1. compact visible tets (blk_vis holds exclusive offsets after rm_bin_count)
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles_touched_dev, vs_dev,
+                       n_tets, (const uint32_t*)WS(uint32_t, L.blk_vis), WS(uint32_t, L.vkey),
+                       WS(uint32_t, L.vidx));
+    // 2. stable sort of the visible tets by depth key (32 bits)
+    radix_sort<uint32_t>(WS(uint32_t, L.vkey), WS(uint32_t, L.vidx), WS(uint32_t, L.vkey2), WS(uint32_t, L.vidx2),
+                         WS(uint32_t, L.vkey3), WS(uint32_t, L.vidx3), n_visible, 0, 32, rcounts, partial, st);
+    const uint32_t* order = WS(uint32_t, L.vidx2);
+    // 3. pair offsets in depth order
+    device_scan(LoadGather{tiles_touched_dev, order}, n_visible, WS(uint32_t, L.poff), false, partial,
+                (int64_t*)nullptr, st);
+    // 4. expand pairs
+    hipLaunchKernelGGL(emit_pairs_kernel, dim3((unsigned)ceil_div(n_visible, kThreads)), dim3(kThreads), 0, st,
+                       order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev, n_visible, n_pairs,
+                       grid_width, WS(uint32_t, L.ptile), WS(uint32_t, L.ptet));
+    // 5. stable sort of pairs by tile id over bits [0, bit_length(G))
+    int msb = bit_length(G);
+    radix_sort<uint32_t>(WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), WS(uint32_t, L.ptile2),
+                         (uint32_t*)sorted_tet_dev, WS(uint32_t, L.ptile3), WS(uint32_t, L.ptet3), n_pairs, 0,
+                         msb, rcounts, partial, st);
+    // 6. ranges from the sorted tile ids
+    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)ceil_div(n_pairs, kThreads)), dim3(kThreads), 0,
+                       st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, ranges_dev);
+    RM_CHECK_LAUNCH("rm_bin_sort");
+    return RM_OK;
+}
