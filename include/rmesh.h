@@ -127,6 +127,13 @@ int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_
                      float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
                      rm_stream_t stream);
 
+/* Precision of the per-pixel ray/plane intersection in rm_composite_fwd/bwd (process-wide):
+ * 1 (default) = FP64 plane records and crossing distances, rounded to FP32 once the segment
+ * [entry, exit] is known; 0 = FP32 throughout (faster; rays that graze a face then carry the FP32
+ * conditioning of the crossing, up to ~1e-4 in colour on the test scenes). Blending, colour and
+ * transmittance are FP32 in both modes. No counterpart in the reference (always FP32). */
+int rm_set_intersect_precision(int fp64);
+
 /* Replaces splat_tiled.bwd / bwd_alpha_blend (alphablend_shader_interp.slang:273-369,
  * alphablend_tiled_slang_interp.py:70-112). Walks each pixel's list back to front from the
  * stored final state, reduces per-tet gradients across the wavefront, then adds them with

@@ -48,6 +48,7 @@ SIGNATURES = {
     "rm_bin_count": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "rm_bin_sort": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, _I32, _I32, _P, _P, _P, _SZ, _P]),
     "rm_composite_fwd": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P]),
+    "rm_set_intersect_precision": (C.c_int, [C.c_int]),
     "rm_composite_bwd": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P,
                                    _P, _P, _P, _P, _P]),
     "rm_hashgrid_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P]),

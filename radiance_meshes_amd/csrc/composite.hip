@@ -41,8 +41,23 @@ struct NG_ {
     static constexpr int NGRD = F == 7 ? 3 : 9;           // colour-gradient floats per tet
 };
 
-struct Ray {
-    float dx, dy, dz;
+template <typename T>
+struct RayT {
+    T dx, dy, dz;
+};
+
+// Plane records: float4 (n, e) in FP32, or double4 in FP64 for the accurate intersection path.
+template <bool P64>
+struct Prec;
+template <>
+struct Prec<false> {
+    using R = float;
+    using PL = float4;
+};
+template <>
+struct Prec<true> {
+    using R = double;
+    using PL = double4;
 };
 
 __device__ inline float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
@@ -54,44 +69,48 @@ __device__ inline float3 cross3(float3 a, float3 b) {
     return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-// camera.slang:241-296 undistort (Newton, 10 iterations) and 298-346 get_ray.
-__device__ inline Ray make_ray(const CamArgs& c, float px, float py) {
-    float x = (px - c.K[2]) / c.K[0];
-    float y = (py - c.K[5]) / c.K[4];
+// camera.slang:241-296 undistort (Newton, 10 iterations) and 298-346 get_ray, evaluated in T.
+template <typename T>
+__device__ inline RayT<T> make_ray(const CamArgs& c, int ix, int iy, float jx, float jy) {
+    const T px = (T)ix + (T)jx, py = (T)iy + (T)jy;
+    T x = (px - (T)c.K[2]) / (T)c.K[0];
+    T y = (py - (T)c.K[5]) / (T)c.K[4];
     if ((c.type == 0 || c.type == 1) && c.distorted) {
-        const float k1 = c.dist[0], k2 = c.dist[1], p1 = c.dist[2], p2 = c.dist[3];
-        const float xd = x, yd = y;
+        const T k1 = c.dist[0], k2 = c.dist[1], p1 = c.dist[2], p2 = c.dist[3];
+        const T xd = x, yd = y;
         for (int i = 0; i < 10; ++i) {
-            float r2 = x * x + y * y;
-            float d = 1.0f + r2 * (k1 + r2 * k2);
-            float fx = d * x + 2.0f * p1 * x * y + p2 * (r2 + 2.0f * x * x) - xd;
-            float fy = d * y + 2.0f * p2 * x * y + p1 * (r2 + 2.0f * y * y) - yd;
-            float d_r = k1 + r2 * (2.0f * k2);
-            float d_x = 2.0f * x * d_r, d_y = 2.0f * y * d_r;
-            float fx_x = d + d_x * x + 2.0f * p1 * y + 6.0f * p2 * x;
-            float fx_y = d_y * x + 2.0f * p1 * x + 2.0f * p2 * y;
-            float fy_x = d_x * y + 2.0f * p2 * y + 2.0f * p1 * x;
-            float fy_y = d + d_y * y + 2.0f * p2 * x + 6.0f * p1 * y;
-            float den = fy_x * fx_y - fx_x * fy_y;
-            if (fabsf(den) < 1e-9f) break;
+            T r2 = x * x + y * y;
+            T d = (T)1 + r2 * (k1 + r2 * k2);
+            T fx = d * x + (T)2 * p1 * x * y + p2 * (r2 + (T)2 * x * x) - xd;
+            T fy = d * y + (T)2 * p2 * x * y + p1 * (r2 + (T)2 * y * y) - yd;
+            T d_r = k1 + r2 * ((T)2 * k2);
+            T d_x = (T)2 * x * d_r, d_y = (T)2 * y * d_r;
+            T fx_x = d + d_x * x + (T)2 * p1 * y + (T)6 * p2 * x;
+            T fx_y = d_y * x + (T)2 * p1 * x + (T)2 * p2 * y;
+            T fy_x = d_x * y + (T)2 * p2 * y + (T)2 * p1 * x;
+            T fy_y = d + d_y * y + (T)2 * p2 * x + (T)6 * p1 * y;
+            T den = fy_x * fx_y - fx_x * fy_y;
+            if (fabs(den) < (T)1e-9) break;
             x += (fx * fy_y - fy * fx_y) / den;
             y += (fy * fx_x - fx * fy_x) / den;
         }
     }
-    float3 dc;
+    T dcx, dcy, dcz;
     if (c.type == 1) {
-        float th = fminf(3.14159265f, sqrtf(x * x + y * y));
-        float s = th < 1e-6f ? 1.0f : sinf(th) / th;
-        dc = f3(x * s, y * s, cosf(th));
+        T th = sqrt(x * x + y * y);
+        th = th < (T)3.14159265 ? th : (T)3.14159265;
+        T s = th < (T)1e-6 ? (T)1 : sin(th) / th;
+        dcx = x * s; dcy = y * s; dcz = cos(th);
     } else {
-        float inv = 1.0f / sqrtf(x * x + y * y + 1.0f);
-        dc = f3(x * inv, y * inv, inv);
+        T inv = (T)1 / sqrt(x * x + y * y + (T)1);
+        dcx = x * inv; dcy = y * inv; dcz = inv;
     }
     const float* R = c.ray_rot;
-    float3 d = f3(R[0] * dc.x + R[1] * dc.y + R[2] * dc.z, R[3] * dc.x + R[4] * dc.y + R[5] * dc.z,
-                  R[6] * dc.x + R[7] * dc.y + R[8] * dc.z);
-    float inv = 1.0f / sqrtf(dot3(d, d));
-    return {d.x * inv, d.y * inv, d.z * inv};
+    T dx = (T)R[0] * dcx + (T)R[1] * dcy + (T)R[2] * dcz;
+    T dy = (T)R[3] * dcx + (T)R[4] * dcy + (T)R[5] * dcz;
+    T dz = (T)R[6] * dcx + (T)R[7] * dcy + (T)R[8] * dcz;
+    T inv = (T)1 / sqrt(dx * dx + dy * dy + dz * dz);
+    return {dx * inv, dy * inv, dz * inv};
 }
 
 __device__ inline float3 load_v(const float* __restrict__ v, int i) {
@@ -99,39 +118,38 @@ __device__ inline float3 load_v(const float* __restrict__ v, int i) {
     return f3(p[0], p[1], p[2]);
 }
 
-// Outward plane of face (a, b, c) opposite vertex o: n = +-cross(b - a, c - a), flipped when it
-// points towards o (intersect.slang:455-479). Returns the sign applied.
-__device__ inline float face_plane(float3 a, float3 b, float3 c, float3 o, float3& n) {
-    n = cross3(b - a, c - a);
-    if (dot3(n, o - a) > 0.f) {
-        n = f3(-n.x, -n.y, -n.z);
-        return -1.f;
-    }
-    return 1.f;
-}
-
 struct Hit {
     bool hit;
     float se, sx;      // clamped entry / exit distance from the camera along the ray
+    float dt;          // sx - se, formed in the plane precision before rounding
     float se_raw;      // entering plane distance before clamping at min_t
     float den_e, den_x;
     int ie, ix;
 };
 
+__device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp_(double x) {  // v_rcp_f64 refined by two Newton steps
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
 // ray_tetrahedron_intersect_fused on plane records (n_i, e_i) for a ray from cam along d.
-__device__ inline Hit intersect(const float4 pl[4], Ray r, float mt) {
+template <typename PL, typename T>
+__device__ inline Hit intersect(const PL pl[4], RayT<T> r, float mtf) {
     Hit h;
-    float se = -1e30f, sx = 1e30f, de = 1.f, dx = 1.f;
+    const T mt = (T)mtf;
+    T se = (T)-1e30, sx = (T)1e30, de = 1, dx = 1;
     int ie = -1, ix = -1;
     bool miss = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        float den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
-        if (fabsf(den) < 1e-10f) {
-            if (pl[i].w + mt * den > 0.f) miss = true;
+        T den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
+        if (fabs(den) < (T)1e-10) {
+            if (pl[i].w + mt * den > (T)0) miss = true;
         } else {
-            float s = -pl[i].w * __builtin_amdgcn_rcpf(den);
-            if (den < 0.f) {
+            T s = -pl[i].w * rcp_(den);
+            if (den < (T)0) {
                 if (s > se) { se = s; de = den; ie = i; }
             } else {
                 if (s < sx) { sx = s; dx = den; ix = i; }
@@ -139,11 +157,14 @@ __device__ inline Hit intersect(const float4 pl[4], Ray r, float mt) {
         }
     }
     h.hit = !miss && !(se > sx) && (sx > mt);
-    h.se_raw = se;
-    h.se = fmaxf(se, mt);
-    h.sx = fmaxf(sx, h.se);
-    h.den_e = de;
-    h.den_x = dx;
+    T sec = se > mt ? se : mt;
+    T sxc = sx > sec ? sx : sec;
+    h.se_raw = (float)se;
+    h.se = (float)sec;
+    h.sx = (float)sxc;
+    h.dt = (float)(sxc - sec);
+    h.den_e = (float)de;
+    h.den_x = (float)dx;
     h.ie = ie;
     h.ix = ix;
     return h;
@@ -193,38 +214,57 @@ __device__ inline void locate(const CamArgs& cam, int& g, int& r, int& tx, int& 
 }
 
 // LDS staging of one chunk: per slot, 4 planes, (sigma, folded base rgb), colour gradients.
-template <int WG, int F, bool BWD>
+template <int WG, int F, bool BWD, bool P64>
 struct Stage {
-    float4 pl[4][WG];
+    typename Prec<P64>::PL pl[4][WG];
     float4 fa[WG];                     // sigma, base' r, g, b
     float4 fg[(NG_<F>::NGRD + 3) / 4][WG];
     float4 vtx[BWD ? 3 : 1][BWD ? WG : 1];  // v0..v3 packed (backward epilogue)
     int meta[BWD ? 2 : 1][BWD ? WG : 1];     // tet id, face sign bits
 };
 
-template <int WG, int F, bool BWD>
-__device__ inline void stage_tet(Stage<WG, F, BWD>& S, int slot, int tet, const CamArgs& cam,
+template <typename T>
+struct V3 {
+    T x, y, z;
+};
+template <typename T>
+__device__ inline V3<T> v3sub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T>
+__device__ inline T v3dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T>
+__device__ inline V3<T> v3cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Outward plane record of face (a, b, c) opposite o, anchored at a, evaluated in T:
+// n = +-cross(b - a, c - a) flipped away from o (intersect.slang:455-479), e = n . (cam - a).
+template <typename T, typename PL>
+__device__ inline int plane_rec(float3 af, float3 bf, float3 cf, float3 of, float3 camf, PL& out) {
+    V3<T> a = {(T)af.x, (T)af.y, (T)af.z}, b = {(T)bf.x, (T)bf.y, (T)bf.z};
+    V3<T> c = {(T)cf.x, (T)cf.y, (T)cf.z}, o = {(T)of.x, (T)of.y, (T)of.z};
+    V3<T> cam = {(T)camf.x, (T)camf.y, (T)camf.z};
+    V3<T> n = v3cross(v3sub(b, a), v3sub(c, a));
+    int flip = v3dot(n, v3sub(o, a)) > (T)0;
+    if (flip) n = {-n.x, -n.y, -n.z};
+    out.x = n.x; out.y = n.y; out.z = n.z;
+    out.w = v3dot(n, v3sub(cam, a));
+    return flip;
+}
+
+template <int WG, int F, bool BWD, bool P64>
+__device__ inline void stage_tet(Stage<WG, F, BWD, P64>& S, int slot, int tet, const CamArgs& cam,
                                  const int4* __restrict__ indices, const float* __restrict__ verts,
                                  const float* __restrict__ feats) {
+    using T = typename Prec<P64>::R;
     constexpr int NGRD = NG_<F>::NGRD;
     int4 q = indices[tet];
     float3 v0 = load_v(verts, q.x), v1 = load_v(verts, q.y), v2 = load_v(verts, q.z), v3 = load_v(verts, q.w);
     float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
-    float3 n;
     int signs = 0;
-    float sg;
-    sg = face_plane(v0, v1, v2, v3, n);
-    S.pl[0][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
-    signs |= (sg < 0) << 0;
-    sg = face_plane(v0, v1, v3, v2, n);
-    S.pl[1][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
-    signs |= (sg < 0) << 1;
-    sg = face_plane(v0, v2, v3, v1, n);
-    S.pl[2][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v0));
-    signs |= (sg < 0) << 2;
-    sg = face_plane(v1, v2, v3, v0, n);
-    S.pl[3][slot] = make_float4(n.x, n.y, n.z, dot3(n, cp - v1));
-    signs |= (sg < 0) << 3;
+    signs |= plane_rec<T>(v0, v1, v2, v3, cp, S.pl[0][slot]) << 0;
+    signs |= plane_rec<T>(v0, v1, v3, v2, cp, S.pl[1][slot]) << 1;
+    signs |= plane_rec<T>(v0, v2, v3, v1, cp, S.pl[2][slot]) << 2;
+    signs |= plane_rec<T>(v1, v2, v3, v0, cp, S.pl[3][slot]) << 3;
     const float* f = feats + (int64_t)F * tet;
     float g[NGRD];
 #pragma unroll
@@ -254,8 +294,8 @@ __device__ inline void stage_tet(Stage<WG, F, BWD>& S, int slot, int tet, const 
     }
 }
 
-template <int WG, int F, bool BWD>
-__device__ inline void read_grads(const Stage<WG, F, BWD>& S, int slot, float* g) {
+template <int WG, int F, bool BWD, bool P64>
+__device__ inline void read_grads(const Stage<WG, F, BWD, P64>& S, int slot, float* g) {
     constexpr int NGRD = NG_<F>::NGRD;
 #pragma unroll
     for (int k = 0; k < (NGRD + 3) / 4; ++k) {
@@ -269,7 +309,7 @@ __device__ inline void read_grads(const Stage<WG, F, BWD>& S, int slot, float* g
 
 // ------------------------------------------------------------------------------- forward --
 
-template <int TS, int F>
+template <int TS, int F, bool P64>
 __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
@@ -277,7 +317,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     int* __restrict__ out_nc) {
     using C = Cfg<TS>;
     constexpr int NGRD = NG_<F>::NGRD;
-    __shared__ Stage<C::WG, F, false> S;
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    __shared__ Stage<C::WG, F, false, P64> S;
     __shared__ int s_chunks;
 
     int g, r, tx, ty, px, py;
@@ -289,11 +331,12 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     __syncthreads();
     if (r == 0) atomicMax(&s_chunks, (n + C::GROUP - 1) / C::GROUP);
     const int64_t pix = (int64_t)py * cam.W + px;
-    Ray ray = {0.f, 0.f, 1.f};
+    RayT<T> ray = {0, 0, 1};
     if (inside) {
         float2 j = jitter[pix];
-        ray = make_ray(cam, (float)px + j.x, (float)py + j.y);
+        ray = make_ray<T>(cam, px, py, j.x, j.y);
     }
+    const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
     const float mt = cam.min_t;
     float cr = 0.f, cg = 0.f, cb = 0.f, logT = 0.f;
     float X = 0.f, Y = 0.f, Z = 0.f, W = 0.f, V = 0.f;
@@ -304,7 +347,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     for (int c = 0; c < nchunks; ++c) {
         if (!__syncthreads_or(alive)) break;
         int k = c * C::GROUP + r;
-        if (k < n) stage_tet<C::WG, F, false>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+        if (k < n) stage_tet<C::WG, F, false, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
         __syncthreads();
         if (alive) {
             int cnt = n - c * C::GROUP;
@@ -313,24 +356,24 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
             for (int jj = 0; jj < cnt; ++jj) {
                 const int slot = base + jj;
                 ++nc;
-                float4 pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
+                const PL pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
                 Hit h = intersect(pl, ray, mt);
                 float4 fa = S.fa[slot];
-                float dt = h.sx - h.se;
+                float dt = h.dt;
                 float x = fa.x * dt;
                 if (!(h.hit && x > 0.f)) continue;
                 float gr[NGRD];
-                read_grads<C::WG, F, false>(S, slot, gr);
+                read_grads<C::WG, F, false, P64>(S, slot, gr);
                 float em1 = expm1f(-x);
                 float alpha = 1.0f + em1;
                 float ph = phi_of(x, em1);
                 float w0 = ph - alpha, w1 = 1.0f - ph;
                 float T = __expf(logT);
-                float gd0 = gr[0] * ray.dx + gr[1] * ray.dy + gr[2] * ray.dz;
+                float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
                 float gd1 = gd0, gd2 = gd0;
                 if (F == 13) {
-                    gd1 = gr[3 % NGRD] * ray.dx + gr[4 % NGRD] * ray.dy + gr[5 % NGRD] * ray.dz;
-                    gd2 = gr[6 % NGRD] * ray.dx + gr[7 % NGRD] * ray.dy + gr[8 % NGRD] * ray.dz;
+                    gd1 = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
+                    gd2 = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
                 }
                 // w0 * c(s_x) + w1 * c(s_e) with c(s) = base' + s * gd
                 float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
@@ -372,7 +415,7 @@ __device__ inline void group_sum(float (&v)[N], int width) {
     }
 }
 
-template <int TS, int F>
+template <int TS, int F, bool P64>
 __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
@@ -385,7 +428,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     // reduced per-tet values: dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], r[12]
     constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;
     constexpr int NOUT = 12 + F;                // vertex + feature gradients per tet
-    __shared__ Stage<C::WG, F, true> S;
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    __shared__ Stage<C::WG, F, true, P64> S;
     __shared__ int s_nc[C::NG];
 
     int g, r, tx, ty, px, py;
@@ -396,13 +441,13 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     if (threadIdx.x < C::NG) s_nc[threadIdx.x] = 0;
     __syncthreads();
     int nc = 0;
-    Ray ray = {0.f, 0.f, 1.f};
+    RayT<T> ray = {0, 0, 1};
     float A = 0.f, Z = 0.f, W = 0.f;
     float dCr = 0.f, dCg = 0.f, dCb = 0.f, dA = 0.f, dX = 0.f, dY = 0.f, dZ = 0.f, dW = 0.f, dV = 0.f;
     if (inside) {
         nc = ncontrib[pix];
         float2 j = jitter[pix];
-        ray = make_ray(cam, (float)px + j.x, (float)py + j.y);
+        ray = make_ray<T>(cam, px, py, j.x, j.y);
         float4 im = img[pix];
         A = im.w;
         Z = dist[5 * pix + 2];
@@ -414,6 +459,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         atomicMax(&s_nc[g], nc);
     }
     __syncthreads();
+    const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
     const int nmax_g = min(s_nc[g], rg.y - rg.x);   // list entries this tile must revisit
     int nchunks_wg = 0;
 #pragma unroll
@@ -432,7 +478,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         __syncthreads();
         int k = c * C::GROUP + r;
         if (c < nchunks_g && k < nmax_g)
-            stage_tet<C::WG, F, true>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+            stage_tet<C::WG, F, true, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
         __syncthreads();
         if (c >= nchunks_g) continue;               // uniform per tile group
         int cnt = nmax_g - c * C::GROUP;
@@ -446,27 +492,27 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             for (int i = 0; i < NV; ++i) acc[i] = 0.f;
             bool active = false;
             if (inside && kk < nc) {
-                float4 pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
+                const PL pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
                 Hit h = intersect(pl, ray, mt);
                 float4 fa = S.fa[slot];
                 const float sig = fa.x;
-                float dt = h.sx - h.se;
+                float dt = h.dt;
                 float x = sig * dt;
                 if (h.hit && x > 0.f) {
                     active = true;
                     float gr[NGRD];
-                    read_grads<C::WG, F, true>(S, slot, gr);
+                    read_grads<C::WG, F, true, P64>(S, slot, gr);
                     float em1 = expm1f(-x);
                     float alpha = 1.0f + em1;
                     float ph = phi_of(x, em1);
                     float w0 = ph - alpha, w1 = 1.0f - ph;
                     float gd[3];
-                    gd[0] = gr[0] * ray.dx + gr[1] * ray.dy + gr[2] * ray.dz;
+                    gd[0] = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
                     gd[1] = gd[0];
                     gd[2] = gd[0];
                     if (F == 13) {
-                        gd[1] = gr[3 % NGRD] * ray.dx + gr[4 % NGRD] * ray.dy + gr[5 % NGRD] * ray.dz;
-                        gd[2] = gr[6 % NGRD] * ray.dx + gr[7 % NGRD] * ray.dy + gr[8 % NGRD] * ray.dz;
+                        gd[1] = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
+                        gd[2] = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
                     }
                     float bse[3] = {fa.y, fa.z, fa.w};
                     float ce[3], cx[3], rgb[3];
@@ -518,7 +564,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         float qe = dqe[0] + dqe[1] + dqe[2], qx = dqx[0] + dqx[1] + dqx[2];
                         acc[4] = qe + qx;
                         float rs = qe * h.se + qx * h.sx;
-                        acc[5] = rs * ray.dx; acc[6] = rs * ray.dy; acc[7] = rs * ray.dz;
+                        acc[5] = rs * rdx; acc[6] = rs * rdy; acc[7] = rs * rdz;
                         dse += qe * gd[0];
                         dsx += qx * gd[0];
                     } else {
@@ -526,9 +572,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         for (int ch = 0; ch < 3; ++ch) {
                             acc[4 + ch] = dqe[ch] + dqx[ch];
                             float rs = dqe[ch] * h.se + dqx[ch] * h.sx;
-                            acc[7 + 3 * ch] = rs * ray.dx;
-                            acc[8 + 3 * ch] = rs * ray.dy;
-                            acc[9 + 3 * ch] = rs * ray.dz;
+                            acc[7 + 3 * ch] = rs * rdx;
+                            acc[8 + 3 * ch] = rs * rdy;
+                            acc[9 + 3 * ch] = rs * rdz;
                             dse += dqe[ch] * gd[ch];
                             dsx += dqx[ch] * gd[ch];
                         }
@@ -543,9 +589,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         float u = (h.ie == i ? ue : 0.f) + (h.ix == i ? ux : 0.f);
                         float rr = (h.ie == i ? re : 0.f) + (h.ix == i ? rx : 0.f);
                         acc[PU + i] = u;
-                        acc[PU + 4 + 3 * i] = rr * ray.dx;
-                        acc[PU + 5 + 3 * i] = rr * ray.dy;
-                        acc[PU + 6 + 3 * i] = rr * ray.dz;
+                        acc[PU + 4 + 3 * i] = rr * rdx;
+                        acc[PU + 5 + 3 * i] = rr * rdy;
+                        acc[PU + 6 + 3 * i] = rr * rdz;
                     }
                     // step the pixel state back over this tet
                     A = Ap; Z = zp; W = wp;
@@ -562,7 +608,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                                   f3(q2.y, q2.z, q2.w)};
             const int tet = S.meta[0][slot], signs = S.meta[1][slot];
             float gr[NGRD];
-            read_grads<C::WG, F, true>(S, slot, gr);
+            read_grads<C::WG, F, true, P64>(S, slot, gr);
             float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
             const float3 cv0 = cp - V4[0];
             float outv[NOUT];
@@ -583,8 +629,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             for (int i = 0; i < 4; ++i) {
                 const float u = acc[PU + i];
                 const float3 rr = f3(acc[PU + 4 + 3 * i], acc[PU + 5 + 3 * i], acc[PU + 6 + 3 * i]);
-                const float4 pl = S.pl[i][slot];
-                const float3 nn = f3(pl.x, pl.y, pl.z);
+                const PL pl = S.pl[i][slot];
+                const float3 nn = f3((float)pl.x, (float)pl.y, (float)pl.z);
                 const int a = fa_[i], b = fb_[i], cc = fc_[i];
                 const float3 gn = (-u) * (cp - V4[a]) - rr;
                 gv[a] = gv[a] + u * nn;
@@ -602,19 +648,23 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                 outv[3 * v + 1] = gv[v].y;
                 outv[3 * v + 2] = gv[v].z;
             }
-            const int l = lane - red_base;
-            if (l < NOUT) {
-                float val = 0.f;
+            // lane l of the reduction group adds outputs l, l + RED, ... (NOUT may exceed RED)
 #pragma unroll
-                for (int i = 0; i < NOUT; ++i) val = (l == i) ? outv[i] : val;
-                if (val != 0.f) {
-                    if (l < 12) {
-                        const int4 qi = indices[tet];
-                        const int vsel = l / 3;
-                        const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
-                        unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
-                    } else {
-                        unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+            for (int l0 = 0; l0 < NOUT; l0 += C::RED) {
+                const int l = lane - red_base + l0;
+                if (l < NOUT) {
+                    float val = 0.f;
+#pragma unroll
+                    for (int i = 0; i < NOUT; ++i) val = (l == i) ? outv[i] : val;
+                    if (val != 0.f) {
+                        if (l < 12) {
+                            const int4 qi = indices[tet];
+                            const int vsel = l / 3;
+                            const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
+                            unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
+                        } else {
+                            unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                        }
                     }
                 }
             }
@@ -622,24 +672,24 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     }
 }
 
-template <int TS, int F>
+template <int TS, int F, bool P64>
 int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
-    hipLaunchKernelGGL((composite_fwd_kernel<TS, F>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+    hipLaunchKernelGGL((composite_fwd_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
                        n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
                        (float4*)img, dist, nc);
     return 0;
 }
 
-template <int TS, int F>
+template <int TS, int F, bool P64>
 int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, const float* img, const float* dist, const int* nc,
                const float* gimg, const float* gdist, float* gverts, float* gfeats, hipStream_t st) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
-    hipLaunchKernelGGL((composite_bwd_kernel<TS, F>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+    hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
                        n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
                        (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats);
     return 0;
@@ -672,10 +722,18 @@ int check_common(const rm_camera* cam, int32_t nf, const void* sorted, const voi
     return RM_OK;
 }
 
+// Ray/plane intersection precision: FP64 plane records (default) or FP32.
+int g_fp64_planes = 1;
+
 }  // namespace
 }  // namespace rmesh
 
 using namespace rmesh;
+
+extern "C" int rm_set_intersect_precision(int fp64) {
+    g_fp64_planes = fp64 ? 1 : 0;
+    return RM_OK;
+}
 
 extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
                                 const int32_t* ranges_dev,
@@ -692,13 +750,15 @@ extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_
     CamArgs a = make_cam_args(*cam);
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
-#define RM_FWD(TS, F) launch_fwd<TS, F>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev, \
-                                        jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st)
+#define RM_FWD(TS, F, P) launch_fwd<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
+                                        features_dev, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st)
+#define RM_FWD_P(TS, F) (g_fp64_planes ? RM_FWD(TS, F, true) : RM_FWD(TS, F, false))
     if (n_features == 7) {
-        if (ts == 4) RM_FWD(4, 7); else if (ts == 8) RM_FWD(8, 7); else RM_FWD(16, 7);
+        if (ts == 4) RM_FWD_P(4, 7); else if (ts == 8) RM_FWD_P(8, 7); else RM_FWD_P(16, 7);
     } else {
-        if (ts == 4) RM_FWD(4, 13); else if (ts == 8) RM_FWD(8, 13); else RM_FWD(16, 13);
+        if (ts == 4) RM_FWD_P(4, 13); else if (ts == 8) RM_FWD_P(8, 13); else RM_FWD_P(16, 13);
     }
+#undef RM_FWD_P
 #undef RM_FWD
     RM_CHECK_LAUNCH("composite_fwd_kernel");
     return RM_OK;
@@ -724,14 +784,16 @@ extern "C" int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_
     CamArgs a = make_cam_args(*cam);
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
-#define RM_BWD(TS, F) launch_bwd<TS, F>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev, \
-                                        jitter_dev, img_dev, dist_dev, ncontrib_dev, grad_img_dev, grad_dist_dev, \
-                                        grad_vertices_dev, grad_features_dev, st)
+#define RM_BWD(TS, F, P) launch_bwd<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
+                                        features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, grad_img_dev, \
+                                        grad_dist_dev, grad_vertices_dev, grad_features_dev, st)
+#define RM_BWD_P(TS, F) (g_fp64_planes ? RM_BWD(TS, F, true) : RM_BWD(TS, F, false))
     if (n_features == 7) {
-        if (ts == 4) RM_BWD(4, 7); else if (ts == 8) RM_BWD(8, 7); else RM_BWD(16, 7);
+        if (ts == 4) RM_BWD_P(4, 7); else if (ts == 8) RM_BWD_P(8, 7); else RM_BWD_P(16, 7);
     } else {
-        if (ts == 4) RM_BWD(4, 13); else if (ts == 8) RM_BWD(8, 13); else RM_BWD(16, 13);
+        if (ts == 4) RM_BWD_P(4, 13); else if (ts == 8) RM_BWD_P(8, 13); else RM_BWD_P(16, 13);
     }
+#undef RM_BWD_P
 #undef RM_BWD
     RM_CHECK_LAUNCH("composite_bwd_kernel");
     return RM_OK;

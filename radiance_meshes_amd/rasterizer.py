@@ -278,6 +278,11 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         return None, None, None, vertices_grad, tet_density_grad, None, None, None
 
 
+def set_intersect_precision(fp64: bool):
+    """FP64 (default) or FP32 ray/plane intersection in the compositing kernels."""
+    check(lib().rm_set_intersect_precision(1 if fp64 else 0), "rm_set_intersect_precision")
+
+
 def distortion_loss(distortion_img):
     """utils/train_util.py:66-67."""
     total_density = (distortion_img[:, :, 2] ** 2).clip(min=1e-6)

@@ -146,3 +146,23 @@ def test_composite_backward_fp64(oracle_lib, kind, tile, nf):
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         print(f"{name}: rel L2 err vs FP64 adjoint = {rel:.3e} (|ref| = {np.linalg.norm(b):.3e})")
         assert rel <= 1e-3, name
+
+
+@pytest.mark.parametrize("kind,tile,nf", [("plain", 16, 7), ("ball", 8, 13)])
+def test_composite_forward_fp32_planes(oracle_lib, kind, tile, nf):
+    """The all-FP32 intersection mode: same images up to the FP32 conditioning of grazing
+    ray/face crossings (the reference's own precision)."""
+    from radiance_meshes_amd.rasterizer import set_intersect_precision
+    O = oracle_lib
+    sc = scene(kind, tile, nf)
+    set_intersect_precision(False)
+    try:
+        sidx, ranges, img, dist = _composite(sc, jit_seed=7)
+    finally:
+        set_intersect_precision(True)
+    cam, vals, o_ranges, o_img, o_dist, o_nc, margin = _oracle_fwd(O, sc, jit_seed=7)
+    flip = margin < 1e-3
+    err = np.abs(img.cpu().numpy().astype(np.float64) - o_img)[~flip][:, :3]
+    print(f"fp32 planes: rgb max-abs {err.max():.3e}, 99.9th pct {np.percentile(err, 99.9):.3e}")
+    assert np.percentile(err, 99.9) <= 5e-5
+    assert err.max() <= 2e-3
