@@ -1,0 +1,264 @@
+#!/usr/bin/env python
+"""Training-step benchmark of the MI355X tetrahedral rasterizer + hash-grid field.
+
+One step = one pass of the hot path over one synthetic view: per-tet projection and tile binning,
+field query at the visible tets' centroids (hash grid 16 x 2 features, 2^19 entries/level, base 16,
+growth 1.38, MLP 32-64-64-13), per-tile compositing of a 1280x840 image, L2 loss against a seeded
+target, backward through compositing, MLP and hash grid, and an Adam update of the field and the
+vertices. The mesh is 1M points from a seeded 32-component Gaussian mixture plus 8 corners at +-20,
+Delaunay-tetrahedralised on the CPU before timing starts.
+
+Prints one JSON line (rank 0): metric fwd+bwd Mrays/s (rays = pixels per step / step time), the
+roofline of the dominant kernel (HIP events around its launches inside the timed region), and a
+CPU baseline: the reference's CPU method (per-ray midpoint quadrature testing every tet at every
+sample, jaxutil/tetra_quad.py) restated in C, timed on a bounded sample of rays of the same view.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_VALU_PEAK_TFLOPS = 78.6   # non-packed FP32 vector FMA rate (packed FP32 ops are not used)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--points", type=int, default=1_000_000)
+    p.add_argument("--width", type=int, default=1280)
+    p.add_argument("--height", type=int, default=840)
+    p.add_argument("--tile", type=int, default=4)
+    p.add_argument("--sweep-tiles", default="", help="comma-separated extra tile sizes timed on the same mesh "
+                   "(one JSON line each on stderr)")
+    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    p.add_argument("--intersect", default="fp64", choices=["fp64", "fp32"])
+    p.add_argument("--cpu-baseline-rays", type=int, default=32)
+    p.add_argument("--cpu-baseline-samples", type=int, default=128)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-layout", action="store_true", help="keep the Delaunay order of vertices and tets")
+    p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
+    p.add_argument("--profile-steps", type=int, default=0,
+                   help="run only this many steps after warmup, without the CPU baseline (for rocprof)")
+    return p.parse_args()
+
+
+def build_workload(args):
+    from radiance_meshes_amd import scenes
+    cam = scenes.orbit_camera(6.0, 30.0, 20.0, args.width, args.height, 50.0)
+    if args.mesh_cache and os.path.exists(args.mesh_cache):
+        z = np.load(args.mesh_cache)
+        if int(z["points"]) == args.points and bool(z["layout"]) == (not args.no_layout):
+            log(f"[bench] mesh from {args.mesh_cache}")
+            return z["pts"], z["tets"], cam
+    t0 = time.time()
+    pts = scenes.gmm_points(args.points, 0)
+    log(f"[bench] {len(pts)} points; tetrahedralising on the CPU ...")
+    tets = scenes.tetrahedralize(pts)
+    log(f"[bench] {len(tets)} tets ({time.time() - t0:.1f} s)")
+    if not args.no_layout:
+        from radiance_meshes_amd.layout import spatial_layout
+        pts, tets, _, _ = spatial_layout(pts, tets)
+    pts = pts.astype(np.float32)
+    if args.mesh_cache:
+        np.savez(args.mesh_cache, pts=pts, tets=tets, points=args.points, layout=not args.no_layout)
+    return pts, tets, cam
+
+
+def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats):
+    """Sum over pixels of n_contrib: (pixel, tet) evaluations of one forward pass."""
+    import torch
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    from radiance_meshes_amd.rasterizer import _cam
+    H, W = rg.image_height, rg.image_width
+    img = torch.empty((H, W, 4), device="cuda")
+    dist = torch.empty((H, W, 5), device="cuda")
+    nc = torch.empty((H, W), device="cuda", dtype=torch.int32)
+    check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(indices), ptr(vertices),
+                                 ptr(feats), feats.shape[1], ptr(jitter), ptr(img), ptr(dist), ptr(nc),
+                                 stream_of()), "rm_composite_fwd")
+    return int(nc.sum().item())
+
+
+def run_config(args, pts, tets, cam, tile):
+    import torch
+    from radiance_meshes_amd import field as fieldmod, timing
+    from radiance_meshes_amd.rasterizer import (AlphaBlendTiledRender, RenderGrid, distortion_loss,
+                                                set_intersect_precision, vertex_and_tile_shader)
+    dev = "cuda"
+    torch.manual_seed(0)
+    vertices = torch.nn.Parameter(torch.from_numpy(pts).to(dev))
+    indices = torch.from_numpy(tets).to(dev)
+    T = indices.shape[0]
+    H, W = args.height, args.width
+    tcam = {k: (v.to(dev) if hasattr(v, "to") else v) for k, v in cam.tcam(tile, 0.1).items()}
+    rg = RenderGrid(H, W, tile, tile)
+    field = fieldmod.Field(n_features=13, precision=args.precision, scale=2.0, device=dev)
+    target = torch.rand((H, W, 3), generator=torch.Generator().manual_seed(1)).to(dev)
+    jitter = torch.full((H, W, 2), 0.5, device=dev)
+    opt = torch.optim.Adam([{"params": field.parameters(), "lr": 1e-3},
+                            {"params": [vertices], "lr": 1e-6}], foreach=True)
+    set_intersect_precision(args.intersect == "fp64")
+    stats = {}
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(indices, vertices, tcam, rg)
+        feats = torch.zeros((T, 13), device=dev)
+        feats[mask] = field(vertices, indices, mask)
+        img, dist, _ = AlphaBlendTiledRender.apply(sidx, ranges, indices, vertices, feats, rg, tcam, jitter)
+        loss = ((img[..., :3] - target) ** 2).mean() + 0.01 * distortion_loss(dist).mean()
+        loss.backward()
+        opt.step()
+        stats["P"] = sidx.numel()
+        stats["mask"] = mask
+        stats["last"] = (sidx, ranges, feats.detach())
+        return loss
+
+    log("[bench] warmup")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    steps = args.profile_steps or args.steps
+    log(f"[bench] timing {steps} steps")
+    timing.reset()
+    timing.enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    timing.enable(False)
+    kern = timing.summary()
+    ms = dt / steps * 1e3
+    mrays = H * W / (dt / steps) / 1e6
+    n_vis = int(stats["mask"].sum().item())
+    n_eval = count_evaluations(indices, vertices.detach(), tcam, rg, jitter, *stats["last"])
+    P = int(stats["P"])
+    G = rg.grid_height * rg.grid_width
+    N = H * W
+    V = vertices.shape[0]
+
+    # algorithmic bytes per launch (FP32 values, int32 indices)
+    L, Fh = field.grid.n_levels, field.grid.n_feat
+    algo = {
+        "project_fwd": 16 * T + 12 * V + 44 * T,
+        "bin_count": 4 * T,
+        "bin_sort": (24 * T) + (12 * P + 24 * P + 8 * P) + 8 * G,
+        "composite_fwd": 96 * P + 8 * G + 8 * N + 40 * N,
+        "composite_bwd": 96 * P + 8 * G + 8 * N + 40 * N + 36 * N + 76 * P,
+        "hashgrid_fwd": n_vis * (12 + L * 8 * Fh * 4 + 4 * L * Fh),
+        "hashgrid_bwd": n_vis * (12 + 4 * L * Fh + L * 8 * Fh * 4),
+        "mlp_fwd": n_vis * (128 + 52),
+        "mlp_bwd": n_vis * (128 + 52 + 128),
+    }
+    dominant = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dominant]
+    achieved = algo.get(dominant, 0) / (d["mean_ms"] * 1e-3) / 1e9
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "mean_ms": round(d["mean_ms"], 4),
+                "algorithmic_bytes": int(algo.get(dominant, 0))}
+    total_algo = sum(algo[k] * kern[k]["launches"] / steps for k in algo if k in kern)
+    breakdown = {k: round(v["total_ms"] / steps, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])}
+
+    return {"value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
+            "hbm_frac_all_kernels": round(total_algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "breakdown_ms": breakdown,
+            "config": {"workload": f"gmm_{args.points}pts_{W}x{H}_fwd_bwd_train_step", "points": int(V),
+                       "tets": int(T), "visible_tets": n_vis, "tile_pairs": P, "evaluations": n_eval,
+                       "width": W, "height": H, "tile": tile,
+                       "field": f"hashgrid L16 F2 2^19 base16 x1.38 + mlp 32-64-64-13 {args.precision}",
+                       "intersect": args.intersect, "layout": "delaunay" if args.no_layout else "morton",
+                       "parallelism": "single_gpu"}}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: this build runs the rasterizer on a single GPU "
+              "(multi-GPU image-space sharding is a design note, not built); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pts, tets, cam = build_workload(args)
+    for extra in [int(t) for t in args.sweep_tiles.split(",") if t.strip()]:
+        r = run_config(args, pts, tets, cam, extra)
+        log(json.dumps({"sweep_tile": extra, **{k: r[k] for k in ("value", "ms_per_step", "breakdown_ms", "config")}}))
+    res = run_config(args, pts, tets, cam, args.tile)
+    cpu = None
+    if not args.no_cpu_baseline and not args.profile_steps:
+        cpu = cpu_baseline(args, pts, tets, cam)
+
+    out = {
+        "metric": "fwd+bwd Mrays/s @1M verts",
+        "value": res["value"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": res["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": res["config"],
+        "roofline": res["roofline"],
+        "hbm_frac_all_kernels": res["hbm_frac_all_kernels"],
+        "breakdown_ms": res["breakdown_ms"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, pts, tets, cam):
+    """The reference's CPU method (midpoint quadrature over all tets per sample) on host cores."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    from radiance_meshes_amd import scenes
+    import torch
+    feats = scenes.tet_features(len(tets), 1, 13, np.float64)
+    ocam = O.make_camera(cam, args.tile, 0.1)
+    rng = np.random.default_rng(0)
+    pix = rng.choice(args.width * args.height, args.cpu_baseline_rays, replace=False)
+    jit = np.full((args.height, args.width, 2), 0.5, np.float32)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[bench] CPU baseline: {len(pix)} rays x {args.cpu_baseline_samples} samples x {len(tets)} tets "
+        f"on {cores} threads")
+    t0 = time.perf_counter()
+    O.quadrature(ocam, jit, pix, 12.0, args.cpu_baseline_samples, tets, pts.astype(np.float64), feats)
+    t = time.perf_counter() - t0
+    cpu_model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(len(pix) / t / 1e6, 9), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{len(pix)} random pixels of the {args.width}x{args.height} view, "
+                      f"{args.cpu_baseline_samples} midpoint samples per ray over t in [0, 12], every one of "
+                      f"the {len(tets)} tets tested at every sample (forward only), {t:.1f} s",
+            "cpu": cpu_model}
+
+
+if __name__ == "__main__":
+    main()

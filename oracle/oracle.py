@@ -206,36 +206,58 @@ def hashgrid_levels(n_levels, base_res, growth, log2_size):
     return np.array(res, np.int32), offs
 
 
+def _grid_cells(x, r):
+    """Cell corner and fractions at resolution r, with the reference's FP32 position arithmetic
+    (utils/hashgrid.py:78-84: clamp, x * r, + 0.5, floor, frac = pos - floor, all in float32)."""
+    xf = np.clip(np.asarray(x, np.float32), np.float32(0), np.float32(1))
+    pos = (xf * np.float32(r)).astype(np.float32) + np.float32(0.5)
+    fl = np.floor(pos)
+    return fl.astype(np.int64), (pos - fl).astype(np.float64)
+
+
+def _corner_hash(vi, r, log2_size):
+    if r ** 3 <= (1 << log2_size):
+        w = vi % r
+        return w[:, 0] + w[:, 1] * r + w[:, 2] * r * r
+    u = vi.astype(np.uint64)
+    h = (u[:, 0] * np.uint64(PRIMES[0])) ^ (u[:, 1] * np.uint64(PRIMES[1])) ^ (u[:, 2] * np.uint64(PRIMES[2]))
+    return (h & np.uint64((1 << log2_size) - 1)).astype(np.int64)
+
+
 def hashgrid_fwd(x, table, res, offs, n_feat, log2_size):
-    """FP64 restatement of HashEmbedderOptimized.forward (utils/hashgrid.py:74-124) on the
-    [0,1]^3 box: clamp, pos = x*res + 0.5, floor, 8 corners, dense modulo index or XOR-prime
-    hash, trilinear interpolation x->y->z. Returns [N, L*n_feat] level-major."""
-    x = np.clip(np.asarray(x, np.float64), 0.0, 1.0)
-    N = x.shape[0]
+    """Restatement of HashEmbedderOptimized.forward (utils/hashgrid.py:74-124) on the [0,1]^3 box:
+    FP32 cell position as the reference, then 8 corners with the dense modulo index or the XOR-prime
+    hash, trilinear interpolation in FP64. Returns [N, L*n_feat] level-major."""
+    N = np.asarray(x).shape[0]
     L = len(res)
     out = np.zeros((N, L * n_feat))
-    mask = (1 << log2_size) - 1
     for l in range(L):
         r = int(res[l])
-        pos = x * r + 0.5
-        fl = np.floor(pos).astype(np.int64)
-        fr = pos - fl
+        fl, fr = _grid_cells(x, r)
         acc = np.zeros((N, n_feat))
         for c in range(8):
             off = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
-            vi = fl + off
-            if r ** 3 <= (1 << log2_size):
-                w = vi % r
-                h = w[:, 0] + w[:, 1] * r + w[:, 2] * r * r
-            else:
-                u = vi.astype(np.uint64)
-                h = (u[:, 0] * np.uint64(PRIMES[0])) ^ (u[:, 1] * np.uint64(PRIMES[1])) ^ \
-                    (u[:, 2] * np.uint64(PRIMES[2]))
-                h = (h & np.uint64(mask)).astype(np.int64)
+            h = _corner_hash(fl + off, r, log2_size)
             wt = np.prod(np.where(off[None, :] == 1, fr, 1.0 - fr), axis=1)
-            acc += wt[:, None] * table[offs[l] + h]
+            acc += wt[:, None] * np.asarray(table, np.float64)[offs[l] + h]
         out[:, l * n_feat:(l + 1) * n_feat] = acc
     return out
+
+
+def hashgrid_bwd(x, grad_out, res, offs, n_feat, log2_size, table_rows):
+    """Adjoint of hashgrid_fwd w.r.t. the table (the encode is linear in the table): scatter-add of
+    corner weight * output gradient, in FP64."""
+    g = np.zeros((table_rows, n_feat))
+    go = np.asarray(grad_out, np.float64)
+    for l in range(len(res)):
+        r = int(res[l])
+        fl, fr = _grid_cells(x, r)
+        for c in range(8):
+            off = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+            h = _corner_hash(fl + off, r, log2_size)
+            wt = np.prod(np.where(off[None, :] == 1, fr, 1.0 - fr), axis=1)
+            np.add.at(g, offs[l] + h, wt[:, None] * go[:, l * n_feat:(l + 1) * n_feat])
+    return g
 
 
 SELU_ALPHA = 1.6732632423543772

@@ -405,13 +405,29 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
 
 // ------------------------------------------------------------------------------ backward --
 
+// DPP lane permutes within a row of 16 lanes (full-rate VALU, no LDS traffic).
+template <int CTRL>
+__device__ inline float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum of v over the reduction group, returned in every lane of the group. Within a row of 16:
+// quad_perm xor 1 and xor 2 give each lane its quad's sum, row_ror 4 and 8 add the other quads.
+// A 64-lane group then adds the other rows with two cross-row shuffles.
 template <int N>
 __device__ inline void group_sum(float (&v)[N], int width) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        if (o >= width) break;
-#pragma unroll
-        for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], o, 64);
+    for (int i = 0; i < N; ++i) {
+        float x = v[i];
+        x += dpp<0xB1>(x);   // quad_perm [1,0,3,2]
+        x += dpp<0x4E>(x);   // quad_perm [2,3,0,1]
+        x += dpp<0x124>(x);  // row_ror:4
+        x += dpp<0x128>(x);  // row_ror:8
+        if (width > 16) {
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+        }
+        v[i] = x;
     }
 }
 

@@ -32,12 +32,15 @@ __device__ inline uint32_t corner_index(const GridArgs& g, int l, int cx, int cy
 
 __device__ inline void cell(const float* __restrict__ x, int64_t n, const GridArgs& g, int l, int& bx, int& by,
                             int& bz, float& fx, float& fy, float& fz) {
+#pragma clang fp contract(off)
+    // x * r and + 0.5 round separately, as the reference's float32 torch ops do
     const float r = (float)g.res[l];
     float p[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         float v = fminf(fmaxf(x[3 * n + k], 0.0f), 1.0f);
-        p[k] = __fadd_rn(__fmul_rn(v, r), 0.5f);
+        const float xr = v * r;
+        p[k] = xr + 0.5f;
     }
     float f0 = floorf(p[0]), f1 = floorf(p[1]), f2 = floorf(p[2]);
     bx = (int)f0; by = (int)f1; bz = (int)f2;
@@ -76,33 +79,63 @@ __global__ __launch_bounds__(256) void hashgrid_fwd_kernel(const float* __restri
     }
 }
 
+// Backward, level-major: blockIdx.y is the level and consecutive lanes take consecutive query
+// points, so spatially ordered queries give adjacent lanes the same table slots at coarse levels.
+// Per corner, a wave whose lanes hit at most 32 distinct slot runs sums each run of equal slot
+// indices with a segmented scan and issues one atomic per run (its last lane); otherwise every
+// lane adds its own contribution directly.
 template <int F>
 __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restrict__ x, int64_t N,
                                                            const float* __restrict__ gout, GridArgs g,
                                                            float* __restrict__ gtable) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N * g.L) return;
-    int64_t n = t / g.L;
-    int l = (int)(t - n * g.L);
+    const int l = blockIdx.y;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = n < N;
     float go[F];
-    bool any = false;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-        go[f] = gout[(n * g.L + l) * F + f];
-        any |= go[f] != 0.f;
-    }
-    if (!any) return;
-    int bx, by, bz;
-    float fx, fy, fz;
-    cell(x, n, g, l, bx, by, bz, fx, fy, fz);
+    for (int f = 0; f < F; ++f) go[f] = valid ? gout[(n * g.L + l) * F + f] : 0.f;
+    int bx = 0, by = 0, bz = 0;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (valid) cell(x, n, g, l, bx, by, bz, fx, fy, fz);
     float* tb = gtable + g.offs[l] * F;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
-        float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-        uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
+        const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+        const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+        const uint32_t h = valid ? corner_index(g, l, bx + ox, by + oy, bz + oz) : 0xFFFFFFFFu;
+        float v[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) unsafeAtomicAdd(tb + (int64_t)h * F + f, w * go[f]);
+        for (int f = 0; f < F; ++f) v[f] = w * go[f];
+        const uint32_t hprev = __shfl_up(h, 1, 64);
+        const bool head = lane == 0 || h != hprev;
+        const uint64_t heads = __ballot(head);
+        if (__popcll(heads) > 32) {
+            if (valid)
+#pragma unroll
+                for (int f = 0; f < F; ++f)
+                    if (v[f] != 0.f) unsafeAtomicAdd(tb + (int64_t)h * F + f, v[f]);
+            continue;
+        }
+        // segmented inclusive scan over runs of equal h
+        bool seg = head;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            float u[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) u[f] = __shfl_up(v[f], o, 64);
+            const bool su = __shfl_up((int)seg, o, 64);
+            if (lane >= o && !seg) {
+#pragma unroll
+                for (int f = 0; f < F; ++f) v[f] += u[f];
+            }
+            if (lane >= o) seg = seg || su;
+        }
+        const bool next_head = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
+        if (next_head && h != 0xFFFFFFFFu)
+#pragma unroll
+            for (int f = 0; f < F; ++f)
+                if (v[f] != 0.f) unsafeAtomicAdd(tb + (int64_t)h * F + f, v[f]);
     }
 }
 
@@ -166,8 +199,7 @@ extern "C" int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_
     if (e) return e;
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && grad_out_dev && grad_table_dev, "rm_hashgrid_bwd: null buffer");
-    int64_t total = n * n_levels;
-    dim3 grid((unsigned)ceil_div(total, 256)), block(256);
+    dim3 grid((unsigned)ceil_div(n, 256), (unsigned)n_levels), block(256);
     RM_FEAT_DISPATCH(n_feat, hashgrid_bwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
                      grad_table_dev);
     RM_CHECK_LAUNCH("hashgrid_bwd_kernel");

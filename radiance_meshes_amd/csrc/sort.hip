@@ -387,44 +387,63 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __rest
 }
 
 // Expands depth-ordered visible tets into (tile id, tet id) pairs, row-major over each rect.
-// A workgroup owns 256 consecutive tets and all their pairs; lanes take pairs round-robin and
-// locate their tet by binary search over the workgroup's offsets in LDS.
+// Work is balanced by pairs: workgroup b owns pairs [b*K, (b+1)*K); its first lane finds the
+// tets covering that range by binary search over the exclusive pair offsets, their offsets and
+// rects are staged in LDS (at most K tets, each owning at least one pair), and lanes take the
+// range's pairs round-robin, locating their tet by binary search in LDS. A screen-filling tet
+// (tens of thousands of pairs) is thus spread over many workgroups.
+constexpr int kPairsPerBlock = 1024;
+
+__device__ inline int64_t last_le(const uint32_t* __restrict__ a, int64_t n, uint32_t v) {
+    int64_t lo = 0, hi = n - 1;  // a[0] == 0 <= v
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ poff, const int4* __restrict__ rect,
     int64_t nvis, int64_t P, int32_t gw, uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet) {
-    __shared__ uint32_t s_off[kThreads + 1];
-    __shared__ int4 s_rect[kThreads];
-    __shared__ uint32_t s_tet[kThreads];
-    int64_t i0 = (int64_t)blockIdx.x * kThreads;
-    int64_t i = i0 + threadIdx.x;
-    if (i < nvis) {
-        uint32_t t = order[i];
-        s_tet[threadIdx.x] = t;
-        s_rect[threadIdx.x] = rect[t];
-        s_off[threadIdx.x] = poff[i];
-    }
+    __shared__ uint32_t s_off[kPairsPerBlock];
+    __shared__ int4 s_rect[kPairsPerBlock];
+    __shared__ uint32_t s_tet[kPairsPerBlock];
+    __shared__ int64_t s_lo;
+    __shared__ int s_n;
+    const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
+    const int64_t p1 = (p0 + kPairsPerBlock < P) ? p0 + kPairsPerBlock : P;
     if (threadIdx.x == 0) {
-        int64_t iend = i0 + kThreads;
-        s_off[kThreads] = iend < nvis ? poff[iend] : (uint32_t)P;
+        int64_t lo = last_le(poff, nvis, (uint32_t)p0);
+        int64_t hi = last_le(poff, nvis, (uint32_t)(p1 - 1));
+        s_lo = lo;
+        s_n = (int)(hi - lo + 1);
     }
-    int nloc = (int)((nvis - i0) < kThreads ? (nvis - i0) : kThreads);
     __syncthreads();
-    if (nloc < kThreads && threadIdx.x == 0) s_off[nloc] = s_off[kThreads];
+    const int64_t lo = s_lo;
+    const int n = s_n;
+    for (int j = threadIdx.x; j < n; j += kThreads) {
+        uint32_t t = order[lo + j];
+        s_off[j] = poff[lo + j];
+        s_tet[j] = t;
+        s_rect[j] = rect[t];
+    }
     __syncthreads();
-    uint32_t p0 = s_off[0], p1 = s_off[nloc];
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
-        int lo = 0, hi = nloc - 1;  // last j with s_off[j] <= p
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= p) lo = mid;
-            else hi = mid - 1;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
+        const uint32_t pp = (uint32_t)p;
+        int a = 0, b = n - 1;  // last j with s_off[j] <= p
+        while (a < b) {
+            int mid = (a + b + 1) >> 1;
+            if (s_off[mid] <= pp) a = mid;
+            else b = mid - 1;
         }
-        int4 r = s_rect[lo];
-        uint32_t m = p - s_off[lo];
-        uint32_t wdt = (uint32_t)(r.z - r.x);
-        uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
+        const int4 r = s_rect[a];
+        const uint32_t m = pp - s_off[a];
+        const uint32_t wdt = (uint32_t)(r.z - r.x);
+        const uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
         ptile[p] = y * (uint32_t)gw + x;
-        ptet[p] = s_tet[lo];
+        ptet[p] = s_tet[a];
     }
 }
 
@@ -622,7 +641,7 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     device_scan(LoadGather{tiles_touched_dev, order}, n_visible, WS(uint32_t, L.poff), false, partial,
                 (int64_t*)nullptr, st);
     // 4. expand pairs
-    hipLaunchKernelGGL(emit_pairs_kernel, dim3((unsigned)ceil_div(n_visible, kThreads)), dim3(kThreads), 0, st,
+    hipLaunchKernelGGL(emit_pairs_kernel, dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)), dim3(kThreads), 0, st,
                        order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev, n_visible, n_pairs,
                        grid_width, WS(uint32_t, L.ptile), WS(uint32_t, L.ptet));
     // 5. stable sort of pairs by tile id over bits [0, bit_length(G))

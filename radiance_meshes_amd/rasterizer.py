@@ -17,7 +17,7 @@ import math
 
 import torch
 
-from . import _capi
+from . import _capi, timing
 from ._capi import check, lib, ptr, stream_of
 
 SUPPORTED_TILES = (4, 8, 16)
@@ -93,8 +93,9 @@ class VertexShader(torch.autograd.Function):
         vs = torch.empty((T, 3), device=dev, dtype=torch.float32)
         cc = torch.empty((T, 3), device=dev, dtype=torch.float32)
         cam = _cam(tcam, render_grid)
-        check(lib().rm_project_fwd(cam, ptr(indices), ptr(vertices), T, vertices.shape[0], ptr(tiles),
-                                   ptr(rect), ptr(vs), ptr(cc), stream_of(dev)), "rm_project_fwd")
+        with timing.span("project_fwd"):
+            check(lib().rm_project_fwd(cam, ptr(indices), ptr(vertices), T, vertices.shape[0], ptr(tiles),
+                                       ptr(rect), ptr(vs), ptr(cc), stream_of(dev)), "rm_project_fwd")
         non_tensor, tensor_data = split_tensors(tcam)
         ctx.non_tensor_data = non_tensor
         ctx.render_grid = render_grid
@@ -111,7 +112,8 @@ class VertexShader(torch.autograd.Function):
             return None, grad_vertices, None, None
         gvs = grad_vs.contiguous() if grad_vs is not None else None
         gcc = grad_cc.contiguous() if grad_cc is not None else None
-        check(lib().rm_project_bwd(ctx.cam, ptr(indices), ptr(vertices), indices.shape[0], vertices.shape[0],
+        with timing.span("project_bwd"):
+            check(lib().rm_project_bwd(ctx.cam, ptr(indices), ptr(vertices), indices.shape[0], vertices.shape[0],
                                    ptr(tiles), ptr(gvs), ptr(gcc), ptr(grad_vertices),
                                    stream_of(vertices.device)), "rm_project_bwd")
         return None, grad_vertices, None, None
@@ -128,7 +130,8 @@ def bin_and_sort(tiles_touched, rect_tile_space, vs_tetra, render_grid):
     ws_bytes = L.rm_bin_workspace_size(T, 0, G)
     ws = torch.empty((max(ws_bytes, 1),), device=dev, dtype=torch.uint8)
     counts = torch.empty((2,), device=dev, dtype=torch.int64)
-    check(L.rm_bin_count(ptr(tiles_touched), T, ptr(counts), ptr(ws), ws_bytes, st), "rm_bin_count")
+    with timing.span("bin_count"):
+        check(L.rm_bin_count(ptr(tiles_touched), T, ptr(counts), ptr(ws), ws_bytes, st), "rm_bin_count")
     n_vis, P = (int(v) for v in counts.tolist())  # host sync, as the reference's index_buffer_offset[-1]
     full = L.rm_bin_workspace_size(T, P, G)
     if full > ws.numel():
@@ -137,7 +140,8 @@ def bin_and_sort(tiles_touched, rect_tile_space, vs_tetra, render_grid):
         ws = ws2
     sorted_idx = torch.empty((P,), device=dev, dtype=torch.int32)
     ranges = torch.empty((G, 2), device=dev, dtype=torch.int32)
-    check(L.rm_bin_sort(ptr(tiles_touched), ptr(rect_tile_space), ptr(vs_tetra), T, n_vis, P,
+    with timing.span("bin_sort"):
+        check(L.rm_bin_sort(ptr(tiles_touched), ptr(rect_tile_space), ptr(vs_tetra), T, n_vis, P,
                         render_grid.grid_height, render_grid.grid_width, ptr(sorted_idx), ptr(ranges),
                         ptr(ws), ws.numel(), st), "rm_bin_sort")
     return sorted_idx, ranges
@@ -246,7 +250,8 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         tet_alive = torch.zeros((indices.shape[0],), dtype=torch.bool, device=dev)
         cam = _cam(tcam, render_grid)
         F = tet_density.shape[1]
-        check(lib().rm_composite_fwd(cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices), ptr(vertices),
+        with timing.span("composite_fwd"):
+            check(lib().rm_composite_fwd(cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices), ptr(vertices),
                                      ptr(tet_density), F, ptr(ray_jitter), ptr(output_img), ptr(distortion_img),
                                      ptr(n_contributors), stream_of(dev)), "rm_composite_fwd")
         non_tensor, tensor_data = split_tensors(tcam)
@@ -270,7 +275,8 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         grad_distortion_img = grad_distortion_img.contiguous().float()
         vertices_grad = torch.zeros_like(vertices)
         tet_density_grad = torch.zeros_like(tet_density)
-        check(lib().rm_composite_bwd(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices),
+        with timing.span("composite_bwd"):
+            check(lib().rm_composite_bwd(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices),
                                      ptr(vertices), ptr(tet_density), tet_density.shape[1], ptr(ray_jitter),
                                      ptr(output_img), ptr(distortion_img), ptr(n_contributors),
                                      ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),

@@ -1,0 +1,52 @@
+"""The C ABI library loads without a GPU and exports every entry point include/rmesh.h declares;
+argument checking that needs no device memory reports errors through rm_last_error."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "rmesh.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rm_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from radiance_meshes_amd import _capi
+    path = _capi.LIB_PATH
+    if not os.path.exists(path):
+        from radiance_meshes_amd import build
+        build.build()
+    return _capi.lib()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_python_binding_covers_the_header():
+    from radiance_meshes_amd import _capi
+    assert set(_declared()) == set(_capi.SIGNATURES)
+
+
+def test_argument_errors_without_device(lib):
+    from radiance_meshes_amd import _capi
+    cam = _capi.RmCamera()
+    cam.tile_width = cam.tile_height = 5
+    cam.image_height = cam.image_width = 20
+    cam.grid_height = cam.grid_width = 4
+    rc = lib.rm_composite_fwd(cam, None, 0, None, None, None, None, 7, None, None, None, None, None)
+    assert rc == 2 and b"tile" in lib.rm_last_error()
+    rc = lib.rm_mlp_fwd(None, 10, None, None, None, None, None, None, 40, 0, None, None)
+    assert rc == 1 and b"n_out" in lib.rm_last_error()
+    assert lib.rm_sort_pairs_workspace_size(1000) > 1000 * 12
+    assert lib.rm_scan_workspace_size(1 << 20) > 0
+    assert lib.rm_bin_workspace_size(1000, 5000, 64) > 5000 * 16
