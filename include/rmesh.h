@@ -145,6 +145,49 @@ int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_
                      const float* grad_img_dev, const float* grad_dist_dev,
                      float* grad_vertices_dev, float* grad_features_dev, rm_stream_t stream);
 
+/* ---- batched forward rendering (serving) -----------------------------------------------
+ * No counterpart in the reference (which renders one view per call, evaluate.py:50-63 /
+ * tile_benchmark.py:59-94). Many views of device-resident scenes, with mixed resolutions, render
+ * as one fixed launch sequence: rm_batch_plan packs a host-side plan (copy it to device memory
+ * before rendering); rm_batch_render runs projection, binning and compositing of every view with
+ * all sizes taken from the plan and pair_capacity, leaving {n_visible, n_pairs} in counts_dev
+ * (n_pairs > pair_capacity means pairs were dropped: render again with a larger capacity). No
+ * host synchronisation and no memset inside, so the call can be captured in a HIP graph.
+ * Outputs are the views' images concatenated ([sum H*W, 4], [sum H*W, 5], [sum H*W]), pixel-centre
+ * rays. All views share one tile size; all scenes one feature layout. The info passed to
+ * rm_batch_render sets the launch geometry: the plan's totals, or capacities at least as large
+ * (kernels read the plan's actual totals from device memory and leave the rest idle), so one
+ * captured graph per resolution bucket replays for any plan that fits it. */
+typedef struct rm_scene {
+    const int32_t* indices_dev;
+    const float* vertices_dev;
+    const float* features_dev;
+    int64_t n_tets;
+    int64_t n_verts;
+    int32_t n_features;
+    int32_t _pad;
+} rm_scene;
+
+typedef struct rm_batch_info {
+    int64_t n_views;
+    int64_t total_tets;
+    int64_t total_tiles;
+    int64_t total_pixels;
+    int64_t total_workgroups;
+    int64_t plan_bytes;
+    int32_t tile;
+    int32_t n_features;
+} rm_batch_info;
+
+int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_camera* cams,
+                  const int32_t* scene_of_view, int32_t n_views, void* plan_host, size_t plan_bytes,
+                  rm_batch_info* info);
+size_t rm_batch_workspace_size(const rm_batch_info* info, int64_t pair_capacity);
+int rm_batch_render(const void* plan_dev, const rm_batch_info* info, int64_t pair_capacity,
+                    float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
+                    int64_t* counts_dev, void* workspace_dev, size_t workspace_bytes,
+                    rm_stream_t stream);
+
 /* ---- hash-grid field ----------------------------------------------------------------------
  * Replaces HashEmbedderOptimized.forward (utils/hashgrid.py:74-124): multi-level grid over
  * [0,1]^3, per-level resolution res[l] and table offset offs[l] (entries, level-concatenated

@@ -27,6 +27,17 @@ class RmCamera(C.Structure):
                 ("camera_type", C.c_int32), ("_pad", C.c_int32)]
 
 
+class RmScene(C.Structure):
+    _fields_ = [("indices_dev", C.c_void_p), ("vertices_dev", C.c_void_p), ("features_dev", C.c_void_p),
+                ("n_tets", C.c_int64), ("n_verts", C.c_int64), ("n_features", C.c_int32), ("_pad", C.c_int32)]
+
+
+class RmBatchInfo(C.Structure):
+    _fields_ = [("n_views", C.c_int64), ("total_tets", C.c_int64), ("total_tiles", C.c_int64),
+                ("total_pixels", C.c_int64), ("total_workgroups", C.c_int64), ("plan_bytes", C.c_int64),
+                ("tile", C.c_int32), ("n_features", C.c_int32)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _I32 = C.c_int32
@@ -51,6 +62,10 @@ SIGNATURES = {
     "rm_set_intersect_precision": (C.c_int, [C.c_int]),
     "rm_composite_bwd": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P,
                                    _P, _P, _P, _P, _P]),
+    "rm_batch_plan": (C.c_int, [C.POINTER(RmScene), _I32, C.POINTER(RmCamera), _P, _I32, _P, _SZ,
+                                C.POINTER(RmBatchInfo)]),
+    "rm_batch_workspace_size": (_SZ, [C.POINTER(RmBatchInfo), _I64]),
+    "rm_batch_render": (C.c_int, [_P, C.POINTER(RmBatchInfo), _I64, _P, _P, _P, _P, _P, _SZ, _P]),
     "rm_hashgrid_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P]),
     "rm_hashgrid_bwd": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P]),
     "rm_mlp_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),

@@ -69,4 +69,40 @@ inline CamArgs make_cam_args(const rm_camera& c) {
     return a;
 }
 
+// One view of a batched render (device-resident plan record, packed by rm_batch_plan).
+struct ViewRec {
+    CamArgs cam;
+    const int32_t* indices;    // [n_tets, 4] of the view's scene
+    const float* verts;        // [V, 3]
+    const float* feats;        // [n_tets, F]
+    int64_t tet_offset;        // first global tet id of this view in the batch
+    int64_t n_tets;
+    int64_t tile_offset;       // first global tile id
+    int64_t pixel_offset;      // first pixel of the view in the batch image buffers
+    int64_t block_offset;      // first compositing workgroup of the view
+    int64_t n_blocks;
+};
+
+struct PlanHeader {
+    int32_t n_views, tile, n_features, _pad;
+    int64_t total_tets, total_tiles, total_pixels, total_blocks;
+};
+
+__host__ __device__ inline const ViewRec* plan_views(const void* plan) {
+    return reinterpret_cast<const ViewRec*>(static_cast<const char*>(plan) + sizeof(PlanHeader));
+}
+
+// Batched stages (defined in project.hip, sort.hip, composite.hip), all on one stream, sized by
+// the plan and a pair capacity: no host synchronisation inside.
+hipError_t batch_project(const void* plan_dev, const PlanHeader& h, int32_t* tiles, int4* rect, float* depth,
+                         int32_t* tet_view, hipStream_t st);
+size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap);
+hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
+                     const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
+                     int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st);
+int batch_workgroups_per_view(int tile, int gh, int gw);
+hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet,
+                           int64_t pair_cap, const int32_t* ranges, float* img, float* dist, int32_t* nc,
+                           hipStream_t st);
+
 }  // namespace rmesh

@@ -196,11 +196,11 @@ __device__ inline int2 load_range(const int2* __restrict__ ranges, bool tile_ok,
     return rg;
 }
 
+// Tile group and pixel of this lane for workgroup b of a view (b already XCD-remapped).
 template <int TS>
-__device__ inline void locate(const CamArgs& cam, int& g, int& r, int& tx, int& ty, int& px, int& py,
+__device__ inline void locate(const CamArgs& cam, int b, int& g, int& r, int& tx, int& ty, int& px, int& py,
                               bool& tile_ok, bool& inside) {
     using C = Cfg<TS>;
-    int b = remap_block(blockIdx.x, gridDim.x);
     int sgw = (cam.gw + C::GX - 1) / C::GX;
     int sx = b % sgw, sy = b / sgw;
     g = threadIdx.x / C::GROUP;
@@ -309,31 +309,34 @@ __device__ inline void read_grads(const Stage<WG, F, BWD, P64>& S, int slot, flo
 
 // ------------------------------------------------------------------------------- forward --
 
+// Forward of the tiles of workgroup b of one view. Tile ids are offset by tile_base, the sorted
+// list holds tet ids offset by tet_base, outputs start at pixel pix_base; jitter may be null
+// (pixel centres).
 template <int TS, int F, bool P64>
-__global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
-    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
-    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
-    const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
-    int* __restrict__ out_nc) {
+__device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, int& s_chunks, const CamArgs& cam,
+                                          int b, const int* __restrict__ sorted, int64_t n_sorted,
+                                          const int2* __restrict__ ranges, int64_t tile_base, int64_t tet_base,
+                                          const int4* __restrict__ indices, const float* __restrict__ verts,
+                                          const float* __restrict__ feats, const float2* __restrict__ jitter,
+                                          float4* __restrict__ out_img, float* __restrict__ out_dist,
+                                          int* __restrict__ out_nc, int64_t pix_base) {
     using C = Cfg<TS>;
     constexpr int NGRD = NG_<F>::NGRD;
     using T = typename Prec<P64>::R;
     using PL = typename Prec<P64>::PL;
-    __shared__ Stage<C::WG, F, false, P64> S;
-    __shared__ int s_chunks;
 
     int g, r, tx, ty, px, py;
     bool tile_ok, inside;
-    locate<TS>(cam, g, r, tx, ty, px, py, tile_ok, inside);
-    int2 rg = load_range(ranges, tile_ok, ty * cam.gw + tx, n_sorted);
+    locate<TS>(cam, b, g, r, tx, ty, px, py, tile_ok, inside);
+    int2 rg = load_range(ranges, tile_ok, (int)(tile_base + ty * cam.gw + tx), n_sorted);
     int n = rg.y - rg.x;
     if (threadIdx.x == 0) s_chunks = 0;
     __syncthreads();
     if (r == 0) atomicMax(&s_chunks, (n + C::GROUP - 1) / C::GROUP);
-    const int64_t pix = (int64_t)py * cam.W + px;
+    const int64_t pix = pix_base + (int64_t)py * cam.W + px;
     RayT<T> ray = {0, 0, 1};
     if (inside) {
-        float2 j = jitter[pix];
+        float2 j = jitter ? jitter[pix - pix_base] : make_float2(0.5f, 0.5f);
         ray = make_ray<T>(cam, px, py, j.x, j.y);
     }
     const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
@@ -347,7 +350,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     for (int c = 0; c < nchunks; ++c) {
         if (!__syncthreads_or(alive)) break;
         int k = c * C::GROUP + r;
-        if (k < n) stage_tet<C::WG, F, false, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+        if (k < n)
+            stage_tet<C::WG, F, false, P64>(S, threadIdx.x, (int)(sorted[rg.x + k] - tet_base), cam, indices, verts,
+                                            feats);
         __syncthreads();
         if (alive) {
             int cnt = n - c * C::GROUP;
@@ -403,6 +408,42 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     }
 }
 
+
+template <int TS, int F, bool P64>
+__global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
+    int* __restrict__ out_nc) {
+    __shared__ Stage<Cfg<TS>::WG, F, false, P64> S;
+    __shared__ int s_chunks;
+    fwd_tiles<TS, F, P64>(S, s_chunks, cam, remap_block(blockIdx.x, gridDim.x), sorted, n_sorted, ranges, 0, 0,
+                          indices, verts, feats, jitter, out_img, out_dist, out_nc, 0);
+}
+
+// Batched forward: the workgroups of all views of a plan in one launch (pixel-centre rays).
+template <int TS, int F, bool P64>
+__global__ __launch_bounds__(Cfg<TS>::WG) void batch_composite_fwd_kernel(
+    const void* __restrict__ plan, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    float4* __restrict__ out_img, float* __restrict__ out_dist, int* __restrict__ out_nc) {
+    __shared__ Stage<Cfg<TS>::WG, F, false, P64> S;
+    __shared__ int s_chunks;
+    const PlanHeader* h = static_cast<const PlanHeader*>(plan);
+    const ViewRec* views = plan_views(plan);
+    const int bg = remap_block(blockIdx.x, gridDim.x);
+    if (bg >= h->total_blocks) return;  // launch sized for a capacity
+    int lo = 0, hi = h->n_views - 1;  // last view with block_offset <= bg
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (views[mid].block_offset <= bg) lo = mid;
+        else hi = mid - 1;
+    }
+    const ViewRec& v = views[lo];
+    fwd_tiles<TS, F, P64>(S, s_chunks, v.cam, (int)(bg - v.block_offset), sorted, n_sorted, ranges, v.tile_offset,
+                          v.tet_offset, reinterpret_cast<const int4*>(v.indices), v.verts, v.feats, nullptr, out_img,
+                          out_dist, out_nc, v.pixel_offset);
+}
+
 // ------------------------------------------------------------------------------ backward --
 
 // DPP lane permutes within a row of 16 lanes (full-rate VALU, no LDS traffic).
@@ -451,7 +492,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
 
     int g, r, tx, ty, px, py;
     bool tile_ok, inside;
-    locate<TS>(cam, g, r, tx, ty, px, py, tile_ok, inside);
+    locate<TS>(cam, remap_block(blockIdx.x, gridDim.x), g, r, tx, ty, px, py, tile_ok, inside);
     int2 rg = load_range(ranges, tile_ok, ty * cam.gw + tx, n_sorted);
     const int64_t pix = (int64_t)py * cam.W + px;
     if (threadIdx.x < C::NG) s_nc[threadIdx.x] = 0;
@@ -741,7 +782,34 @@ int check_common(const rm_camera* cam, int32_t nf, const void* sorted, const voi
 // Ray/plane intersection precision: FP64 plane records (default) or FP32.
 int g_fp64_planes = 1;
 
+template <int TS, int F, bool P64>
+void launch_batch(const void* plan, const PlanHeader& h, const int32_t* sorted, int64_t cap, const int32_t* ranges,
+                  float* img, float* dist, int32_t* nc, hipStream_t st) {
+    hipLaunchKernelGGL((batch_composite_fwd_kernel<TS, F, P64>), dim3((unsigned)h.total_blocks), dim3(Cfg<TS>::WG),
+                       0, st, plan, sorted, cap, (const int2*)ranges, (float4*)img, dist, nc);
+}
+
 }  // namespace
+
+int batch_workgroups_per_view(int tile, int gh, int gw) {
+    int gx = tile == 4 ? 2 : 1, gy = gx;
+    return ((gw + gx - 1) / gx) * ((gh + gy - 1) / gy);
+}
+
+hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet, int64_t pair_cap,
+                           const int32_t* ranges, float* img, float* dist, int32_t* nc, hipStream_t st) {
+    if (h.total_blocks <= 0) return hipSuccess;
+#define RM_BATCH(TS, F) (g_fp64_planes ? launch_batch<TS, F, true>(plan_dev, h, sorted_tet, pair_cap, ranges, img, dist, nc, st) \
+                                       : launch_batch<TS, F, false>(plan_dev, h, sorted_tet, pair_cap, ranges, img, dist, nc, st))
+    if (h.n_features == 7) {
+        if (h.tile == 4) RM_BATCH(4, 7); else if (h.tile == 8) RM_BATCH(8, 7); else RM_BATCH(16, 7);
+    } else {
+        if (h.tile == 4) RM_BATCH(4, 13); else if (h.tile == 8) RM_BATCH(8, 13); else RM_BATCH(16, 13);
+    }
+#undef RM_BATCH
+    return hipGetLastError();
+}
+
 }  // namespace rmesh
 
 using namespace rmesh;

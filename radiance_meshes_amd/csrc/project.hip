@@ -178,53 +178,90 @@ __device__ inline F3 load_vert(const float* __restrict__ v, int i) {
     return {p[0], p[1], p[2]};
 }
 
+// One tet: tile count, rect, clamped depth and circumcenter (zeros when culled).
+__device__ inline void project_one(const CamArgs& cam, int4 q, const float* __restrict__ verts, int32_t& n,
+                                   int4& r, float& depth_out, F3& ctr_out) {
+    F3 v[4] = {load_vert(verts, q.x), load_vert(verts, q.y), load_vert(verts, q.z), load_vert(verts, q.w)};
+    n = 0;
+    r = {0, 0, 0, 0};
+    depth_out = 0.f;
+    ctr_out = {0.f, 0.f, 0.f};
+    if (is_planar(v)) return;
+    F3 vc[4] = {to_view(cam, v[0]), to_view(cam, v[1]), to_view(cam, v[2]), to_view(cam, v[3])};
+    F3 mn, mx;
+    find_extent(cam, vc, mn, mx);
+    if (mx.z <= 0.0f) return;
+    D3 P = {(double)cam.cam_pos[0], (double)cam.cam_pos[1], (double)cam.cam_pos[2]};
+    D3 ctr = {0, 0, 0};
+    float depth = (float)power_of_circumsphere(todouble(v[0]), todouble(v[1]), todouble(v[2]), todouble(v[3]), P, ctr);
+    if (depth < -1e19f) return;
+    float gw = (float)cam.gw, gh = (float)cam.gh;
+    int4 rr;
+    rr.x = (int32_t)floorf(fmaxf(0.0f, fminf(gw, mn.x / (float)cam.tw)));
+    rr.y = (int32_t)floorf(fmaxf(0.0f, fminf(gh, mn.y / (float)cam.th)));
+    rr.z = (int32_t)ceilf(fmaxf(0.0f, fminf(gw, mx.x / (float)cam.tw)));
+    rr.w = (int32_t)ceilf(fmaxf(0.0f, fminf(gh, mx.y / (float)cam.th)));
+    int32_t nt = (rr.z - rr.x) * (rr.w - rr.y);
+    if (nt == 0) return;
+    n = nt;
+    r = rr;
+    depth_out = fminf(fmaxf(depth, -1e20f), 1e20f);
+    ctr_out = {(float)ctr.x, (float)ctr.y, (float)ctr.z};
+}
+
 __global__ __launch_bounds__(256) void project_fwd_kernel(
     CamArgs cam, const int4* __restrict__ indices, const float* __restrict__ verts, int64_t T,
     int32_t* __restrict__ tiles, int4* __restrict__ rect, float* __restrict__ vs,
     float* __restrict__ cc) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
-    int4 q = indices[t];
-    F3 v[4] = {load_vert(verts, q.x), load_vert(verts, q.y), load_vert(verts, q.z),
-               load_vert(verts, q.w)};
-    int32_t n = 0;
-    int4 r = {0, 0, 0, 0};
-    float depth_out = 0.f;
-    F3 ctr_out = {0.f, 0.f, 0.f};
-    if (!is_planar(v)) {
-        F3 vc[4] = {to_view(cam, v[0]), to_view(cam, v[1]), to_view(cam, v[2]), to_view(cam, v[3])};
-        F3 mn, mx;
-        find_extent(cam, vc, mn, mx);
-        if (!(mx.z <= 0.0f)) {
-            D3 P = {(double)cam.cam_pos[0], (double)cam.cam_pos[1], (double)cam.cam_pos[2]};
-            D3 ctr = {0, 0, 0};
-            float depth = (float)power_of_circumsphere(todouble(v[0]), todouble(v[1]),
-                                                       todouble(v[2]), todouble(v[3]), P, ctr);
-            if (!(depth < -1e19f)) {
-                float gw = (float)cam.gw, gh = (float)cam.gh;
-                int4 rr;
-                rr.x = (int32_t)floorf(fmaxf(0.0f, fminf(gw, mn.x / (float)cam.tw)));
-                rr.y = (int32_t)floorf(fmaxf(0.0f, fminf(gh, mn.y / (float)cam.th)));
-                rr.z = (int32_t)ceilf(fmaxf(0.0f, fminf(gw, mx.x / (float)cam.tw)));
-                rr.w = (int32_t)ceilf(fmaxf(0.0f, fminf(gh, mx.y / (float)cam.th)));
-                int32_t nt = (rr.z - rr.x) * (rr.w - rr.y);
-                if (nt != 0) {
-                    n = nt;
-                    r = rr;
-                    depth_out = fminf(fmaxf(depth, -1e20f), 1e20f);
-                    ctr_out = {(float)ctr.x, (float)ctr.y, (float)ctr.z};
-                }
-            }
-        }
-    }
+    int32_t n;
+    int4 r;
+    float depth;
+    F3 ctr;
+    project_one(cam, indices[t], verts, n, r, depth, ctr);
     tiles[t] = n;
     rect[t] = r;
     vs[3 * t + 0] = 0.f;
     vs[3 * t + 1] = 0.f;
-    vs[3 * t + 2] = depth_out;
-    cc[3 * t + 0] = ctr_out.x;
-    cc[3 * t + 1] = ctr_out.y;
-    cc[3 * t + 2] = ctr_out.z;
+    vs[3 * t + 2] = depth;
+    cc[3 * t + 0] = ctr.x;
+    cc[3 * t + 1] = ctr.y;
+    cc[3 * t + 2] = ctr.z;
+}
+
+// Batched projection: lane g projects global tet g of the batch with its view's camera.
+__global__ __launch_bounds__(256) void batch_project_kernel(const void* __restrict__ plan, int64_t total,
+                                                            int32_t* __restrict__ tiles, int4* __restrict__ rect,
+                                                            float* __restrict__ depth, int32_t* __restrict__ tet_view) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const PlanHeader* h = static_cast<const PlanHeader*>(plan);
+    const ViewRec* views = plan_views(plan);
+    if (g >= h->total_tets) {  // launch sized for a capacity: slots past the plan's tets stay empty
+        tiles[g] = 0;
+        rect[g] = make_int4(0, 0, 0, 0);
+        depth[g] = 0.f;
+        tet_view[g] = 0;
+        return;
+    }
+    int lo = 0, hi = h->n_views - 1;  // last view with tet_offset <= g
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tet_offset <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const ViewRec& v = views[lo];
+    const int64_t t = g - v.tet_offset;
+    int32_t n;
+    int4 r;
+    float d;
+    F3 ctr;
+    project_one(v.cam, reinterpret_cast<const int4*>(v.indices)[t], v.verts, n, r, d, ctr);
+    tiles[g] = n;
+    rect[g] = r;
+    depth[g] = d;
+    tet_view[g] = lo;
 }
 
 // Adjoint of (clamped float(power), float3(center)) w.r.t. the four vertices, in FP64.
@@ -304,6 +341,15 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
 }
 
 }  // namespace
+
+hipError_t batch_project(const void* plan_dev, const PlanHeader& h, int32_t* tiles, int4* rect, float* depth,
+                         int32_t* tet_view, hipStream_t st) {
+    if (h.total_tets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_project_kernel, dim3((unsigned)ceil_div(h.total_tets, 256)), dim3(256), 0, st, plan_dev,
+                       h.total_tets, tiles, rect, depth, tet_view);
+    return hipGetLastError();
+}
+
 }  // namespace rmesh
 
 using namespace rmesh;

@@ -21,6 +21,14 @@ constexpr int kRsTile = kThreads * kRsItems;        // 2048
 
 __device__ inline int lane_id() { return threadIdx.x & 63; }
 
+// Element count of a launch sized for a capacity: the capacity itself, or a count the device
+// wrote earlier in the stream (graph-capturable launches never read sizes back to the host).
+__device__ inline int64_t eff_n(int64_t cap, const int64_t* n_dev) {
+    if (!n_dev) return cap;
+    int64_t v = *n_dev;
+    return v < cap ? v : cap;
+}
+
 template <typename T>
 __device__ inline T wave_incl_scan(T v) {
     int lane = lane_id();
@@ -68,8 +76,10 @@ struct LoadGather {  // tiles_touched[order[i]]
 };
 
 template <typename Load>
-__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, int64_t n, uint32_t* partial) {
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, int64_t cap, const int64_t* n_dev,
+                                                               uint32_t* partial) {
     __shared__ uint32_t lds[4];
+    const int64_t n = eff_n(cap, n_dev);
     int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t s = 0;
 #pragma unroll
@@ -107,9 +117,10 @@ __global__ __launch_bounds__(kThreads) void scan_partials_kernel(uint32_t* parti
 }
 
 template <typename Load, typename Out>
-__global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, int64_t n, const uint32_t* partial,
-                                                             Out* out, int inclusive) {
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, int64_t cap, const int64_t* n_dev,
+                                                             const uint32_t* partial, Out* out, int inclusive) {
     __shared__ uint32_t lds[4];
+    const int64_t n = eff_n(cap, n_dev);
     int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -132,14 +143,15 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 inline int64_t scan_blocks(int64_t n) { return ceil_div(n, kScanTile); }
 inline size_t scan_ws_bytes(int64_t n) { return align256(sizeof(uint32_t) * (size_t)(scan_blocks(n) + 1)); }
 
+// n is the element count, or the capacity when n_dev points at the device-side count.
 template <typename Load, typename Out>
 hipError_t device_scan(Load ld, int64_t n, Out* out, bool inclusive, uint32_t* partial,
-                       int64_t* total_out, hipStream_t st) {
+                       int64_t* total_out, hipStream_t st, const int64_t* n_dev = nullptr) {
     if (n <= 0) return hipSuccess;
     int64_t nb = scan_blocks(n);
-    hipLaunchKernelGGL(scan_reduce_kernel<Load>, dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n, partial);
+    hipLaunchKernelGGL(scan_reduce_kernel<Load>, dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n, n_dev, partial);
     hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, total_out);
-    hipLaunchKernelGGL((scan_down_kernel<Load, Out>), dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n,
+    hipLaunchKernelGGL((scan_down_kernel<Load, Out>), dim3((unsigned)nb), dim3(kThreads), 0, st, ld, n, n_dev,
                        (const uint32_t*)partial, out, inclusive ? 1 : 0);
     return hipGetLastError();
 }
@@ -147,9 +159,11 @@ hipError_t device_scan(Load ld, int64_t n, Out* out, bool inclusive, uint32_t* p
 // ---------------------------------------------------------------- radix sort ------------
 
 template <typename K>
-__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t n,
-                                                           int shift, int bits, uint32_t* counts, int64_t nb) {
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t cap,
+                                                           const int64_t* n_dev, int shift, int bits,
+                                                           uint32_t* counts, int64_t nb) {
     __shared__ uint32_t h[256];
+    const int64_t n = eff_n(cap, n_dev);
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
@@ -166,8 +180,9 @@ __global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__
 template <typename K>
 __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n, int shift, int bits, const uint32_t* __restrict__ offsets,
-    int64_t nb) {
+    uint32_t* __restrict__ vout, int64_t cap, const int64_t* n_dev, int shift, int bits,
+    const uint32_t* __restrict__ offsets, int64_t nb) {
+    const int64_t n = eff_n(cap, n_dev);
     __shared__ uint32_t s_cnt[256];
     __shared__ uint32_t s_wcnt[4][256];
     __shared__ uint32_t s_loc[256];
@@ -229,6 +244,7 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     }
     __syncthreads();
     int64_t cnt = n - base < kRsTile ? n - base : kRsTile;
+    if (cnt <= 0) return;
 #pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
         int j = r * kThreads + tid;
@@ -251,10 +267,10 @@ inline size_t rs_counts_bytes(int64_t n) { return align256(sizeof(uint32_t) * 25
 template <typename K>
 hipError_t radix_sort(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, K* ktmp, uint32_t* vtmp,
                       int64_t n, int begin_bit, int end_bit, uint32_t* counts, uint32_t* partial,
-                      hipStream_t st) {
+                      hipStream_t st, const int64_t* n_dev = nullptr) {
     if (n <= 0) return hipSuccess;
     int nbits = end_bit - begin_bit;
-    if (nbits <= 0) {
+    if (nbits <= 0 && !n_dev) {
         hipMemcpyAsync(kout, kin, sizeof(K) * n, hipMemcpyDeviceToDevice, st);
         hipMemcpyAsync(vout, vin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
         return hipGetLastError();
@@ -269,13 +285,13 @@ hipError_t radix_sort(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout
         bool to_out = ((passes - 1 - p) % 2) == 0;
         K* kd = to_out ? kout : ktmp;
         uint32_t* vd = to_out ? vout : vtmp;
-        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, n, shift, bits,
-                           counts, nb);
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, n, n_dev, shift,
+                           bits, counts, nb);
         hipError_t e = device_scan(LoadU32{counts}, (int64_t)(1 << bits) * nb, counts, false, partial,
                                    (int64_t*)nullptr, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, vs, kd, vd, n,
-                           shift, bits, (const uint32_t*)counts, nb);
+                           n_dev, shift, bits, (const uint32_t*)counts, nb);
         ks = kd;
         vs = vd;
         shift += bits;
@@ -307,13 +323,19 @@ __global__ __launch_bounds__(kThreads) void generate_keys_kernel(
         }
 }
 
+__global__ void zero_i64_kernel(int64_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 __global__ void zero_ranges_kernel(int2* ranges, int64_t G) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < G) ranges[i] = make_int2(0, 0);
 }
 
 template <typename TileOf>
-__global__ __launch_bounds__(kThreads) void ranges_kernel(TileOf tile_of, int64_t n, int32_t* ranges) {
+__global__ __launch_bounds__(kThreads) void ranges_kernel(TileOf tile_of, int64_t cap, const int64_t* n_dev,
+                                                          int32_t* ranges) {
+    const int64_t n = eff_n(cap, n_dev);
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t cur = tile_of(i);
@@ -363,7 +385,7 @@ __global__ __launch_bounds__(kThreads) void bin_count_kernel(const int32_t* __re
 
 // Order-preserving compaction of the visible tets into (sortable depth key, tet id).
 __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __restrict__ tiles,
-                                                           const float* __restrict__ vs, int64_t T,
+                                                           const float* __restrict__ vs, int dstride, int64_t T,
                                                            const uint32_t* __restrict__ blk_vis_off,
                                                            uint32_t* __restrict__ vkey, uint32_t* __restrict__ vidx) {
     __shared__ uint32_t lds[4];
@@ -380,7 +402,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __rest
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
         if (vis[k]) {
-            vkey[pos] = sortable_key(vs[3 * (base + k) + 2]);
+            vkey[pos] = sortable_key(vs[dstride * (base + k) + (dstride - 1)]);
             vidx[pos] = (uint32_t)(base + k);
             ++pos;
         }
@@ -404,15 +426,33 @@ __device__ inline int64_t last_le(const uint32_t* __restrict__ a, int64_t n, uin
     return lo;
 }
 
+struct TileMapSingle {  // tile id in a single view's grid
+    int32_t gw;
+    __device__ uint32_t operator()(uint32_t, uint32_t x, uint32_t y) const { return y * (uint32_t)gw + x; }
+};
+struct TileMapBatch {   // global tile id of the tet's view in a batch
+    const int32_t* tet_view;
+    const ViewRec* views;
+    __device__ uint32_t operator()(uint32_t tet, uint32_t x, uint32_t y) const {
+        const ViewRec& v = views[tet_view[tet]];
+        return (uint32_t)v.tile_offset + y * (uint32_t)v.cam.gw + x;
+    }
+};
+
+template <typename TileMap>
 __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ poff, const int4* __restrict__ rect,
-    int64_t nvis, int64_t P, int32_t gw, uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet) {
+    int64_t nvis_cap, const int64_t* nvis_dev, int64_t P_cap, const int64_t* P_dev, TileMap tile_of,
+    uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet) {
     __shared__ uint32_t s_off[kPairsPerBlock];
     __shared__ int4 s_rect[kPairsPerBlock];
     __shared__ uint32_t s_tet[kPairsPerBlock];
     __shared__ int64_t s_lo;
     __shared__ int s_n;
+    const int64_t nvis = eff_n(nvis_cap, nvis_dev);
+    const int64_t P = eff_n(P_cap, P_dev);
     const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
+    if (p0 >= P || nvis <= 0) return;
     const int64_t p1 = (p0 + kPairsPerBlock < P) ? p0 + kPairsPerBlock : P;
     if (threadIdx.x == 0) {
         int64_t lo = last_le(poff, nvis, (uint32_t)p0);
@@ -442,7 +482,7 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
         const uint32_t m = pp - s_off[a];
         const uint32_t wdt = (uint32_t)(r.z - r.x);
         const uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
-        ptile[p] = y * (uint32_t)gw + x;
+        ptile[p] = tile_of(s_tet[a], x, y);
         ptet[p] = s_tet[a];
     }
 }
@@ -488,6 +528,45 @@ inline int bit_length(int64_t v) {
 }
 
 }  // namespace
+
+size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) { return bin_layout(h.total_tets, pair_cap).total; }
+
+// Binning of a whole batch with every size read from device memory: grids are sized by the
+// batch's tet count and the pair capacity; counts_dev receives {n_visible, n_pairs} (n_pairs may
+// exceed pair_cap: pairs beyond it are dropped and the caller re-renders with a larger capacity).
+hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
+                     const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
+                     int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st) {
+    const int64_t T = h.total_tets, G = h.total_tiles;
+    BinLayout L = bin_layout(T, pair_cap);
+    char* w = static_cast<char*>(ws);
+    auto W = [&](size_t off) { return reinterpret_cast<uint32_t*>(w + off); };
+    const int64_t nb = scan_blocks(T);
+    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (int2*)ranges, G);
+    hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles, T, W(L.blk_vis),
+                       W(L.blk_pairs));
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, W(L.blk_vis), nb, counts_dev);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, W(L.blk_pairs), nb, counts_dev + 1);
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles, depth, 1, T,
+                       (const uint32_t*)W(L.blk_vis), W(L.vkey), W(L.vidx));
+    uint32_t* partial = W(L.partial);
+    uint32_t* rcounts = W(L.rcounts);
+    radix_sort<uint32_t>(W(L.vkey), W(L.vidx), W(L.vkey2), W(L.vidx2), W(L.vkey3), W(L.vidx3), T, 0, 32, rcounts,
+                         partial, st, counts_dev);
+    const uint32_t* order = W(L.vidx2);
+    device_scan(LoadGather{tiles, order}, T, W(L.poff), false, partial, (int64_t*)nullptr, st, counts_dev);
+    hipLaunchKernelGGL(emit_pairs_kernel<TileMapBatch>, dim3((unsigned)ceil_div(pair_cap, kPairsPerBlock)),
+                       dim3(kThreads), 0, st, order, (const uint32_t*)W(L.poff), rect, T,
+                       (const int64_t*)counts_dev, pair_cap, (const int64_t*)(counts_dev + 1),
+                       TileMapBatch{tet_view, plan_views(plan_dev)}, W(L.ptile), W(L.ptet));
+    radix_sort<uint32_t>(W(L.ptile), W(L.ptet), W(L.ptile2), (uint32_t*)sorted_tet, W(L.ptile3), W(L.ptet3),
+                         pair_cap, 0, bit_length(G), rcounts, partial, st, counts_dev + 1);
+    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)ceil_div(pair_cap, kThreads)), dim3(kThreads), 0,
+                       st, TileOfU32{W(L.ptile2)}, pair_cap, (const int64_t*)(counts_dev + 1), ranges);
+    return hipGetLastError();
+}
+
 }  // namespace rmesh
 
 using namespace rmesh;
@@ -565,7 +644,7 @@ extern "C" int rm_tile_ranges(const int64_t* sorted_keys_dev, int64_t n, int32_t
     if (n > 0) {
         RM_CHECK_ARG(sorted_keys_dev != nullptr, "rm_tile_ranges: null keys");
         hipLaunchKernelGGL(ranges_kernel<TileOfKey>, dim3((unsigned)ceil_div(n, kThreads)), dim3(kThreads), 0,
-                           st, TileOfKey{sorted_keys_dev}, n, ranges_dev);
+                           st, TileOfKey{sorted_keys_dev}, n, (const int64_t*)nullptr, ranges_dev);
     }
     RM_CHECK_LAUNCH("rm_tile_ranges");
     return RM_OK;
@@ -587,7 +666,7 @@ extern "C" int rm_bin_count(const int32_t* tiles_touched_dev, int64_t n_tets, in
     }
     hipStream_t st = as_stream(stream);
     if (n_tets == 0) {
-        hipMemsetAsync(counts_dev, 0, 2 * sizeof(int64_t), st);
+        hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, st, counts_dev, 2);
         RM_CHECK_LAUNCH("rm_bin_count");
         return RM_OK;
     }
@@ -630,7 +709,7 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     uint32_t* partial = WS(uint32_t, L.partial);
     uint32_t* rcounts = WS(uint32_t, L.rcounts);
     // 1. compact visible tets (blk_vis holds exclusive offsets after rm_bin_count)
-    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles_touched_dev, vs_dev,
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles_touched_dev, vs_dev, 3,
                        n_tets, (const uint32_t*)WS(uint32_t, L.blk_vis), WS(uint32_t, L.vkey),
                        WS(uint32_t, L.vidx));
     // 2. stable sort of the visible tets by depth key (32 bits)
@@ -641,9 +720,10 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     device_scan(LoadGather{tiles_touched_dev, order}, n_visible, WS(uint32_t, L.poff), false, partial,
                 (int64_t*)nullptr, st);
     // 4. expand pairs
-    hipLaunchKernelGGL(emit_pairs_kernel, dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)), dim3(kThreads), 0, st,
-                       order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev, n_visible, n_pairs,
-                       grid_width, WS(uint32_t, L.ptile), WS(uint32_t, L.ptet));
+    hipLaunchKernelGGL(emit_pairs_kernel<TileMapSingle>, dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)),
+                       dim3(kThreads), 0, st, order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev,
+                       n_visible, (const int64_t*)nullptr, n_pairs, (const int64_t*)nullptr,
+                       TileMapSingle{grid_width}, WS(uint32_t, L.ptile), WS(uint32_t, L.ptet));
     // 5. stable sort of pairs by tile id over bits [0, bit_length(G))
     int msb = bit_length(G);
     radix_sort<uint32_t>(WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), WS(uint32_t, L.ptile2),
@@ -651,7 +731,7 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
                          msb, rcounts, partial, st);
     // 6. ranges from the sorted tile ids
     hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)ceil_div(n_pairs, kThreads)), dim3(kThreads), 0,
-                       st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, ranges_dev);
+                       st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, (const int64_t*)nullptr, ranges_dev);
     RM_CHECK_LAUNCH("rm_bin_sort");
     return RM_OK;
 }

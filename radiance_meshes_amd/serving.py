@@ -1,0 +1,330 @@
+"""Render serving on device-resident meshes: continuous batching of mixed-resolution views.
+
+The reference renders one view per call (evaluate.py:50-63, tile_benchmark.py:59-94). Here:
+
+* scenes (mesh + per-tet features) are registered once and stay in HBM;
+* requests are grouped by resolution bucket; each bucket owns a fixed number of slots with
+  preallocated buffers, a pinned host plan, its own stream and (optionally) a captured HIP graph
+  of the whole batched forward (`rm_batch_render`: projection, binning and compositing of every
+  view of the batch in one launch sequence, no host synchronisation inside);
+* a request is admitted when its bucket has a free batch and the HBM budget covers the bucket's
+  reservation; requests that arrive while other batches run on the GPU are admitted as soon as a
+  bucket frees up, so batches of different buckets overlap on the device (one stream each);
+* `per_frame` serves the same trace one view at a time through the single-view path, the baseline.
+
+Latency is completion time minus arrival time; p50/p99 and frames/s are reported per mode.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import RmBatchInfo, RmCamera, RmScene, check, lib, ptr, stream_of
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: int
+    camera: object          # scenes.Camera
+    arrival: float
+    done: float = -1.0
+    image: torch.Tensor = None
+
+
+@dataclass
+class _Scene:
+    indices: torch.Tensor
+    vertices: torch.Tensor
+    features: torch.Tensor
+    rec: RmScene
+
+
+def _cam_struct(camera, tile, min_t):
+    return _capi.camera_from_tcam(camera.tcam(tile, min_t))
+
+
+class Bucket:
+    """One resolution: `slots` views per batch. Launch geometry and buffers are sized for `slots`
+    views of the largest registered scene, so any batch of this resolution fits one captured graph."""
+
+    def __init__(self, server, H, W, slots, pair_cap, use_graph):
+        self.server, self.H, self.W, self.slots = server, H, W, slots
+        self.tile = server.tile
+        dev = server.device
+        big = max(range(len(server.scenes)), key=lambda i: server.scenes[i].rec.n_tets)
+        cam = server.probe_camera(H, W)
+        self.info = server.plan_info([(big, cam)] * slots)
+        self.pair_cap = pair_cap
+        self.plan_bytes = int(self.info.plan_bytes)
+        self.plan_host = torch.empty(self.plan_bytes, dtype=torch.uint8, pin_memory=True)
+        self.plan_dev = torch.empty(self.plan_bytes, dtype=torch.uint8, device=dev)
+        wsb = lib().rm_batch_workspace_size(self.info, pair_cap)
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        npx = slots * H * W
+        self.img = torch.empty((npx, 4), device=dev)
+        self.dist = torch.empty((npx, 5), device=dev)
+        self.nc = torch.empty((npx,), device=dev, dtype=torch.int32)
+        self.counts = torch.zeros(2, device=dev, dtype=torch.int64)
+        self.counts_host = torch.zeros(2, dtype=torch.int64, pin_memory=True)
+        self.stream = torch.cuda.Stream(device=dev)
+        self.bytes = wsb + npx * 40 + 2 * self.plan_bytes
+        self.graph = None
+        self.use_graph = use_graph
+        self.busy = None   # (requests, images, event) of the batch in flight
+
+    def _render_call(self):
+        check(lib().rm_batch_render(ptr(self.plan_dev), self.info, self.pair_cap, ptr(self.img), ptr(self.dist),
+                                    ptr(self.nc), ptr(self.counts), ptr(self.ws), self.ws.numel(),
+                                    stream_of(self.server.device)), "rm_batch_render")
+
+    def launch(self, reqs, keep_aux=False):
+        """Enqueues one batch on the bucket's stream and returns at once. The images (and with
+        keep_aux the distortion images and contributor counts) are copied out on the same stream,
+        so the next batch of this bucket cannot overwrite them."""
+        info = self.server.plan_info([(r.scene, r.camera) for r in reqs], self.plan_host)
+        if info.total_tets > self.info.total_tets or len(reqs) > self.slots:
+            raise RuntimeError("batch exceeds the bucket's capacity")
+        hw = self.H * self.W
+        with torch.cuda.stream(self.stream):
+            self.plan_dev.copy_(self.plan_host, non_blocking=True)
+            if self.use_graph:
+                if self.graph is None:
+                    self._capture()
+                self.graph.replay()
+            else:
+                self._render_call()
+            self.counts_host.copy_(self.counts, non_blocking=True)
+            n = len(reqs) * hw
+            outs = (self.img[:n].clone(),) + ((self.dist[:n].clone(), self.nc[:n].clone()) if keep_aux else ())
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.busy = (reqs, outs, ev)
+        return ev
+
+    def _capture(self):
+        # run once eagerly (first-launch set-up stays out of the graph), then capture
+        self._render_call()
+        self.stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            self._render_call()
+        self.graph = g
+
+    def ready(self):
+        return self.busy is not None and self.busy[2].query()
+
+    def finish(self):
+        """[(request, ([H,W,4] image, [H,W,5] distortion, [H,W] n_contrib) or (image,)), ...] of the
+        completed batch. When the batch overflowed the pair capacity its images are incomplete: the
+        views are rendered again one at a time (image only)."""
+        reqs, outs, ev = self.busy
+        ev.synchronize()
+        self.busy = None
+        overflow = int(self.counts_host[1]) > self.pair_cap
+        hw = self.H * self.W
+        shapes = ((self.H, self.W, 4), (self.H, self.W, 5), (self.H, self.W))
+        res = []
+        for i, r in enumerate(reqs):
+            if overflow:
+                res.append((r, (self.server.render_single(r.scene, r.camera),)))
+            else:
+                res.append((r, tuple(o[i * hw:(i + 1) * hw].view(sh) for o, sh in zip(outs, shapes))))
+        return res, overflow
+
+
+class RenderServer:
+    def __init__(self, tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=64 << 30, max_slots=8,
+                 use_graphs=True, device="cuda"):
+        self.tile, self.n_features, self.min_t = tile, n_features, min_t
+        self.budget = hbm_budget_bytes
+        self.max_slots = max_slots
+        self.use_graphs = use_graphs
+        self.device = device
+        self.scenes: list[_Scene] = []
+        self.buckets: dict = {}
+        self.pair_per_px: dict = {}   # scene -> pairs per pixel measured at registration
+
+    # ------------------------------------------------------------------ scenes
+    def add_scene(self, vertices, indices, features, probe_camera=None):
+        v = vertices.to(self.device, torch.float32).contiguous()
+        i = indices.to(self.device, torch.int32).contiguous()
+        f = features.to(self.device, torch.float32).contiguous()
+        if f.shape[1] != self.n_features:
+            raise ValueError(f"features must have {self.n_features} channels")
+        rec = RmScene(indices_dev=i.data_ptr(), vertices_dev=v.data_ptr(), features_dev=f.data_ptr(),
+                      n_tets=i.shape[0], n_verts=v.shape[0], n_features=f.shape[1])
+        self.scenes.append(_Scene(i, v, f, rec))
+        sid = len(self.scenes) - 1
+        if probe_camera is not None:
+            P = self._pairs_of(sid, probe_camera)
+            self.pair_per_px[sid] = P / (probe_camera.width * probe_camera.height)
+        return sid
+
+    def _pairs_of(self, sid, camera):
+        from .rasterizer import RenderGrid, vertex_and_tile_shader
+        sc = self.scenes[sid]
+        rg = RenderGrid(camera.height, camera.width, self.tile, self.tile)
+        with torch.no_grad():
+            sidx, *_ = vertex_and_tile_shader(sc.indices, sc.vertices, self._tcam(camera), rg)
+        return sidx.numel()
+
+    def _tcam(self, camera):
+        return camera.tcam(self.tile, self.min_t)
+
+    # ------------------------------------------------------------------ single view (baseline)
+    def render_single(self, sid, camera):
+        """One view through the single-view path (host sync for the pair count, as the reference)."""
+        from .rasterizer import AlphaBlendTiledRender, RenderGrid, vertex_and_tile_shader
+        sc = self.scenes[sid]
+        rg = RenderGrid(camera.height, camera.width, self.tile, self.tile)
+        tcam = self._tcam(camera)
+        with torch.no_grad():
+            sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(sc.indices, sc.vertices, tcam, rg)
+            jit = torch.full((camera.height, camera.width, 2), 0.5, device=self.device)
+            img, dist, _ = AlphaBlendTiledRender.apply(sidx, ranges, sc.indices, sc.vertices, sc.features, rg,
+                                                       tcam, jit)
+        return img
+
+    # ------------------------------------------------------------------ buckets
+    def probe_camera(self, H, W):
+        from . import scenes
+        return scenes.orbit_camera(6.0, 0.0, 0.0, W, H, 50.0)
+
+    def plan_info(self, views, plan_host=None):
+        """rm_batch_plan over [(scene, camera), ...]; packs into `plan_host` (pinned) when given."""
+        n = len(views)
+        cams = (RmCamera * n)(*[_cam_struct(c, self.tile, self.min_t) for _, c in views])
+        sid = (C.c_int32 * n)(*[s for s, _ in views])
+        scs = (RmScene * len(self.scenes))(*[sc.rec for sc in self.scenes])
+        info = RmBatchInfo()
+        buf = None if plan_host is None else C.c_void_p(plan_host.data_ptr())
+        nbytes = 0 if plan_host is None else plan_host.numel()
+        check(lib().rm_batch_plan(scs, len(self.scenes), cams, sid, n, buf, nbytes, C.byref(info)),
+              "rm_batch_plan")
+        return info
+
+    def bucket(self, H, W):
+        """The bucket of resolution HxW, created on first use: as many slots (up to max_slots) as
+        the HBM budget left over by the other buckets allows."""
+        key = (H, W)
+        if key not in self.buckets:
+            ppx = max(self.pair_per_px.values()) if self.pair_per_px else 64.0
+            per_view_pairs = int(ppx * H * W * 1.5) + 4096
+            remaining = self.budget - sum(b.bytes for b in self.buckets.values())
+            big = max(range(len(self.scenes)), key=lambda i: self.scenes[i].rec.n_tets)
+            slots = self.max_slots
+            while slots > 1:
+                info = self.plan_info([(big, self.probe_camera(H, W))] * slots)
+                need = lib().rm_batch_workspace_size(info, per_view_pairs * slots) + slots * H * W * 40
+                if need <= remaining:
+                    break
+                slots //= 2
+            self.buckets[key] = Bucket(self, H, W, slots, per_view_pairs * slots, self.use_graphs)
+        return self.buckets[key]
+
+    def render_views(self, views, keep_aux=False):
+        """Synchronous batched render of [(scene, camera), ...] of one resolution (tests, warm-up):
+        a list of (image,) or, with keep_aux, (image, distortion, n_contrib) per view."""
+        cam0 = views[0][1]
+        b = self.bucket(cam0.height, cam0.width)
+        out = []
+        for k in range(0, len(views), b.slots):
+            chunk = [Request(i, s, c, 0.0) for i, (s, c) in enumerate(views[k:k + b.slots])]
+            b.launch(chunk, keep_aux)
+            res, _ = b.finish()
+            out += [o for _, o in res]
+        return out
+
+    # ------------------------------------------------------------------ trace replay
+    def serve(self, trace, mode="batched", max_batch=None, poll_s=2e-4):
+        """Replays `trace` (Requests with arrival times in seconds from now) in real time.
+        mode "batched": continuous batching (at most max_batch views per batch, default the
+        bucket's slots); "per_frame": one view at a time through the single-view path.
+        Returns stats: frames/s over the makespan, latency p50/p99 (ms), batches, overflows."""
+        torch.cuda.synchronize()
+        reqs = sorted(trace, key=lambda r: r.arrival)
+        pending = deque(reqs)
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0  # noqa: E731
+        done = 0
+        n_batches = 0
+        overflows = 0
+        sizes = []
+        if mode == "per_frame":
+            while pending:
+                r = pending.popleft()
+                wait = r.arrival - clock()
+                if wait > 0:
+                    time.sleep(wait)
+                r.image = self.render_single(r.scene, r.camera)
+                torch.cuda.current_stream().synchronize()
+                r.done = clock()
+                n_batches += 1
+                sizes.append(1)
+        else:
+            queue: dict = {}
+            in_use = 0
+            while done < len(reqs):
+                now = clock()
+                while pending and pending[0].arrival <= now:
+                    r = pending.popleft()
+                    queue.setdefault((r.camera.height, r.camera.width), deque()).append(r)
+                progressed = False
+                for b in self.buckets.values():
+                    if b.ready():
+                        res, ov = b.finish()
+                        t = clock()
+                        for r, o in res:
+                            r.done, r.image = t, o[0]
+                        done += len(res)
+                        in_use -= b.bytes
+                        overflows += int(ov)
+                        progressed = True
+                # admission: oldest waiting request first, if its bucket is free and the budget allows
+                order = sorted((q[0].arrival, key) for key, q in queue.items() if q)
+                for _, key in order:
+                    b = self.bucket(*key)
+                    if b.busy is not None or in_use + b.bytes > self.budget:
+                        continue
+                    q = queue[key]
+                    batch = [q.popleft() for _ in range(min(b.slots, len(q), max_batch or b.slots))]
+                    b.launch(batch)
+                    in_use += b.bytes
+                    n_batches += 1
+                    sizes.append(len(batch))
+                    progressed = True
+                if not progressed:
+                    nxt = pending[0].arrival - clock() if pending else poll_s
+                    time.sleep(max(0.0, min(poll_s, nxt)))
+        torch.cuda.synchronize()
+        lat = np.array([(r.done - r.arrival) * 1e3 for r in reqs])
+        makespan = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+        return {"mode": mode, "frames": len(reqs), "fps": len(reqs) / makespan,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "batches": n_batches, "mean_batch": float(np.mean(sizes)),
+                "overflows": overflows, "makespan_s": makespan}
+
+
+def poisson_trace(n_scenes, resolutions, rate_hz, duration_s, seed=0, radius=6.0):
+    """Requests with exponential inter-arrival times (rate_hz), a uniform scene, a random orbit pose
+    (azimuth U[0,360), elevation U[-20,40) degrees, 50 degree vertical FOV) and a uniform resolution."""
+    from . import scenes
+    rng = np.random.default_rng(seed)
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t > duration_s:
+            break
+        H, W = resolutions[rng.integers(len(resolutions))]
+        cam = scenes.orbit_camera(radius, rng.uniform(0, 360), rng.uniform(-20, 40), W, H, 50.0)
+        out.append(Request(rid, int(rng.integers(n_scenes)), cam, t))
+        rid += 1
+    return out

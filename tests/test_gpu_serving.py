@@ -1,0 +1,128 @@
+"""Batched serving path: every view rendered in a mixed batch (several scenes, two resolutions,
+captured-graph replays) is bit-identical to the same view rendered alone, through the batch path
+and through the single-view path; one batched view against the FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from radiance_meshes_amd import scenes
+from tests._util import scene
+
+RADIUS = (3.0, 3.0, 6.0)
+RES_A = (66, 90)     # not a multiple of the tile: ragged edge tiles
+RES_B = (48, 64)
+
+
+def _server(use_graphs, max_slots=4):
+    import torch
+    from radiance_meshes_amd.serving import RenderServer
+    srv = RenderServer(tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=4 << 30, max_slots=max_slots,
+                       use_graphs=use_graphs)
+    for kind, kw in (("plain", {}), ("ball", {}), ("gmm", dict(n_points=20000, width=160, height=120))):
+        sc = scene(kind, 4, 13, **kw)
+        srv.add_scene(torch.from_numpy(np.ascontiguousarray(sc.vertices, np.float32)),
+                      torch.from_numpy(np.ascontiguousarray(sc.indices, np.int32)),
+                      torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)), probe_camera=sc.camera)
+    return srv
+
+
+def _views(res, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        s = int(k % 3)
+        cam = scenes.orbit_camera(RADIUS[s], rng.uniform(0, 360), rng.uniform(-30, 40), res[1], res[0], 50.0)
+        out.append((s, cam))
+    return out
+
+
+def _single_full(srv, sid, cam):
+    """Single-view path with pixel-centre rays: (image, distortion, n_contrib)."""
+    import torch
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    from radiance_meshes_amd.rasterizer import RenderGrid, _cam, vertex_and_tile_shader
+    sc = srv.scenes[sid]
+    rg = RenderGrid(cam.height, cam.width, srv.tile, srv.tile)
+    tcam = cam.tcam(srv.tile, srv.min_t)
+    sidx, ranges, *_ = vertex_and_tile_shader(sc.indices, sc.vertices, tcam, rg)
+    H, W = cam.height, cam.width
+    jit = torch.full((H, W, 2), 0.5, device="cuda")
+    img = torch.empty((H, W, 4), device="cuda")
+    dist = torch.empty((H, W, 5), device="cuda")
+    nc = torch.empty((H, W), device="cuda", dtype=torch.int32)
+    check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(sc.indices),
+                                 ptr(sc.vertices), ptr(sc.features), 13, ptr(jit), ptr(img), ptr(dist), ptr(nc),
+                                 stream_of()), "rm_composite_fwd")
+    torch.cuda.synchronize()
+    return img, dist, nc
+
+
+def _assert_same(a, b, what):
+    import torch
+    for x, y, name in zip(a, b, ("image", "distortion", "n_contrib")):
+        assert x.shape == y.shape, (what, name)
+        if not torch.equal(x, y):
+            d = (x.double() - y.double()).abs().max().item()
+            raise AssertionError(f"{what}: {name} differs (max abs {d:.3e})")
+
+
+@pytest.mark.parametrize("use_graphs", [False, True])
+def test_mixed_batch_matches_single_views(use_graphs):
+    import torch
+    print(f"serving mixed batch use_graphs={use_graphs}", flush=True)
+    srv = _server(use_graphs)
+    views_a = _views(RES_A, 7, 1)     # two batches of the 4-slot bucket: 4 + 3 views
+    views_b = _views(RES_B, 3, 2)
+    ref = {id(c): _single_full(srv, s, c) for s, c in views_a + views_b}
+    for rep in range(3):
+        got_a = srv.render_views(views_a, keep_aux=True)
+        got_b = srv.render_views(views_b, keep_aux=True)
+        for (s, c), g in zip(views_a + views_b, got_a + got_b):
+            assert len(g) == 3, "pair capacity overflowed"
+            _assert_same(g, ref[id(c)], f"replay {rep} scene {s} {c.height}x{c.width}")
+        # eager work between replays: single-view renders and unrelated allocations
+        _single_full(srv, *views_b[0])
+        torch.randn(1 << 20, device="cuda").sum().item()
+    for s, c in views_a[:3] + views_b[:1]:
+        alone = srv.render_views([(s, c)], keep_aux=True)[0]
+        _assert_same(alone, ref[id(c)], f"alone scene {s}")
+    assert any(int(r[2].sum()) > 0 for r in ref.values())
+
+
+def test_batched_view_against_oracle(oracle_lib):
+    O = oracle_lib
+    srv = _server(False, max_slots=2)
+    views = _views(RES_A, 2, 5)
+    got = srv.render_views(views, keep_aux=True)
+    for (s, c), (img, dist, nc) in zip(views, got):
+        sc = srv.scenes[s]
+        V = sc.vertices.cpu().numpy()
+        I = sc.indices.cpu().numpy()
+        Fe = sc.features.cpu().numpy()
+        cam = O.make_camera(c, 4, 0.1)
+        t, r, vs, cc = O.project(cam, I, V)
+        keys, vals, ranges = O.bin_tiles(cam, t, r, vs)
+        jit = np.full((c.height, c.width, 2), 0.5, np.float32)
+        o_img, o_dist, o_nc, margin = O.composite_fwd(cam, vals, ranges, I, V, Fe, jit)
+        keep = margin >= 1e-4
+        err = np.abs(img.cpu().numpy().astype(np.float64) - o_img)[keep]
+        print(f"scene {s}: rgb max err {err[:, :3].max():.3e}, logT max err {err[:, 3].max():.3e}")
+        assert err[:, :3].max() <= 1e-5
+        assert (np.abs(img.cpu().numpy()[..., 3] - o_img[..., 3]) / np.maximum(1, np.abs(o_img[..., 3])))[keep].max() <= 1e-5
+        np.testing.assert_array_equal(nc.cpu().numpy()[keep], o_nc[keep])
+
+
+def test_serve_trace_completes_every_request():
+    from radiance_meshes_amd.serving import poisson_trace
+    srv = _server(True)
+    trace = poisson_trace(3, [RES_A, RES_B], rate_hz=400.0, duration_s=0.25, seed=3, radius=3.0)
+    for res in (RES_A, RES_B):
+        srv.render_views(_views(res, 1, 9))          # create and capture both buckets
+    print(f"trace of {len(trace)} requests", flush=True)
+    st = srv.serve(trace, mode="batched")
+    print(st)
+    assert st["frames"] == len(trace) and all(r.done >= r.arrival for r in trace)
+    for r in trace[:4]:
+        ref = _single_full(srv, r.scene, r.camera)[0]
+        assert np.array_equal(r.image.cpu().numpy(), ref.cpu().numpy())
