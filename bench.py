@@ -53,7 +53,28 @@ def parse():
     p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
     p.add_argument("--profile-steps", type=int, default=0,
                    help="run only this many steps after warmup, without the CPU baseline (for rocprof)")
+    p.add_argument("--serve-seconds", type=float, default=0.0,
+                   help="length of the Poisson request trace of the serving leg (default 0: leg off)")
+    p.add_argument("--serve-rate", type=float, default=200.0, help="request arrival rate (1/s)")
+    p.add_argument("--serve-points", default="250000,1000000,3000000", help="points of the served meshes")
+    p.add_argument("--serve-slots", type=int, default=8, help="views per batch of a resolution bucket")
+    p.add_argument("--serve-hbm-gib", type=float, default=160.0, help="HBM budget of the server (GiB)")
+    p.add_argument("--serve-only", action="store_true", help="skip the training-step leg")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a 256 MiB device-to-device copy 5 times (calibrates the HBM byte counters)")
     return p.parse_args()
+
+
+def _mesh(n_points, layout=True):
+    """GMM points (seed 0) -> Delaunay tets (-> Morton layout); runs in a worker process."""
+    from radiance_meshes_amd import scenes
+    t0 = time.time()
+    pts = scenes.gmm_points(n_points, 0)
+    tets = scenes.tetrahedralize(pts)
+    if layout:
+        from radiance_meshes_amd.layout import spatial_layout
+        pts, tets, _, _ = spatial_layout(pts, tets)
+    return n_points, pts.astype(np.float32), tets, time.time() - t0
 
 
 def build_workload(args):
@@ -91,6 +112,21 @@ def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats):
                                  ptr(feats), feats.shape[1], ptr(jitter), ptr(img), ptr(dist), ptr(nc),
                                  stream_of()), "rm_composite_fwd")
     return int(nc.sum().item())
+
+
+def measured_traffic(workload, tile, precision, span):
+    """HBM bytes per launch of `span`'s kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    recorded for this workload (profiles/traffic_*.json), or None when none was recorded."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "profiles", "traffic_gmm1M_1280x840.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("tile") != tile or t.get("precision") != precision:
+        return None
+    return t.get("traffic", {}).get(span)
 
 
 def run_config(args, pts, tets, cam, tile):
@@ -169,8 +205,10 @@ def run_config(args, pts, tets, cam, tile):
     dominant = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dominant]
     achieved = algo.get(dominant, 0) / (d["mean_ms"] * 1e-3) / 1e9
+    workload = f"gmm_{args.points}pts_{W}x{H}_fwd_bwd_train_step"
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(workload, tile, args.precision, dominant),
                 "mean_ms": round(d["mean_ms"], 4),
                 "algorithmic_bytes": int(algo.get(dominant, 0))}
     total_algo = sum(algo[k] * kern[k]["launches"] / steps for k in algo if k in kern)
@@ -179,12 +217,73 @@ def run_config(args, pts, tets, cam, tile):
     return {"value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
             "hbm_frac_all_kernels": round(total_algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "breakdown_ms": breakdown,
-            "config": {"workload": f"gmm_{args.points}pts_{W}x{H}_fwd_bwd_train_step", "points": int(V),
+            "config": {"workload": workload, "points": int(V),
                        "tets": int(T), "visible_tets": n_vis, "tile_pairs": P, "evaluations": n_eval,
                        "width": W, "height": H, "tile": tile,
                        "field": f"hashgrid L16 F2 2^19 base16 x1.38 + mlp 32-64-64-13 {args.precision}",
                        "intersect": args.intersect, "layout": "delaunay" if args.no_layout else "morton",
                        "parallelism": "single_gpu"}}
+
+
+def calibrate_copy():
+    """A plain 256 MiB device-to-device copy, 5 launches: under `rocprofv3 --pmc FETCH_SIZE` /
+    `WRITE_SIZE` it gives the counters' reading for 256 MiB read + 256 MiB written."""
+    import torch
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": src.numel() * 4, "launches": 5}), flush=True)
+
+
+def serve_leg(args, meshes):
+    """Batched serving against per-frame rendering on one Poisson trace (forward only, fp32).
+    meshes: [(n_points, pts, tets)], registered as device-resident scenes with seeded features."""
+    import torch
+    from radiance_meshes_amd import scenes
+    from radiance_meshes_amd.serving import RenderServer, Request, poisson_trace
+    srv = RenderServer(tile=args.tile, n_features=13, min_t=0.1,
+                       hbm_budget_bytes=int(args.serve_hbm_gib * (1 << 30)), max_slots=args.serve_slots,
+                       use_graphs=True)
+    tets_of = []
+    for n, pts, tets in meshes:
+        feats = scenes.tet_features(len(tets), 1, 13)
+        srv.add_scene(torch.from_numpy(pts), torch.from_numpy(tets), torch.from_numpy(feats))
+        tets_of.append(len(tets))
+    resolutions = [(256, 256), (512, 512), (768, 1024), (1080, 1920)]   # (H, W)
+    # warm-up outside the trace: every bucket created (graph captured), every kernel loaded
+    for H, W in resolutions:
+        views = [(s, scenes.orbit_camera(6.0, 10.0 * s, 10.0, W, H, 50.0)) for s in range(len(meshes))]
+        srv.render_views(views)
+        for s, c in views:
+            srv.render_single(s, c)
+    torch.cuda.synchronize()
+    trace = poisson_trace(len(meshes), resolutions, args.serve_rate, args.serve_seconds, seed=0, radius=6.0)
+    log(f"[bench] serving: {len(trace)} requests over {args.serve_seconds} s at {args.serve_rate}/s")
+    keep = lambda r: r.rid % 25 == 0   # noqa: E731  (results kept for the batched/per-frame comparison)
+    runs = {}
+    for mode in ("batched", "per_frame"):
+        reqs = [Request(r.rid, r.scene, r.camera, r.arrival) for r in trace]
+        runs[mode] = (srv.serve(reqs, mode=mode, keep=keep), reqs)
+        log(f"[bench] {mode}: {json.dumps(runs[mode][0])}")
+    # every kept frame: batched result against the per-frame result of the same request
+    max_err, n_cmp = 0.0, 0
+    for rb, rp in zip(runs["batched"][1], runs["per_frame"][1]):
+        if rb.image is not None and rp.image is not None:
+            max_err = max(max_err, float((rb.image - rp.image).abs().max().item()))
+            n_cmp += 1
+    sb, sp = runs["batched"][0], runs["per_frame"][0]
+    return {"fps_batched": round(sb["fps"], 2), "fps_per_frame": round(sp["fps"], 2),
+            "fps_ratio": round(sb["fps"] / sp["fps"], 3),
+            "p50_ms_batched": round(sb["p50_ms"], 2), "p99_ms_batched": round(sb["p99_ms"], 2),
+            "p50_ms_per_frame": round(sp["p50_ms"], 2), "p99_ms_per_frame": round(sp["p99_ms"], 2),
+            "mean_batch": round(sb["mean_batch"], 2), "overflows": sb["overflows"],
+            "max_abs_err_batched_vs_per_frame": max_err, "frames_compared": n_cmp,
+            "trace": {"requests": len(trace), "seconds": args.serve_seconds, "rate_hz": args.serve_rate,
+                      "resolutions": [f"{w}x{h}" for h, w in resolutions], "points": [m[0] for m in meshes],
+                      "tets": tets_of, "tile": args.tile,
+                      "slots": {f"{b.W}x{b.H}": b.slots for b in srv.buckets.values()}}}
 
 
 def main():
@@ -195,18 +294,47 @@ def main():
               "(multi-GPU image-space sharding is a design note, not built); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+
+    # the serving leg's other meshes build on host cores while the GPU runs the training leg
+    pool, pending = None, []
+    serve_points = [int(x) for x in args.serve_points.split(",") if x.strip()] if args.serve_seconds > 0 else []
+    if serve_points and not args.profile_steps:
+        import multiprocessing as mp
+        others = [n for n in serve_points if n != args.points]
+        if others:
+            pool = mp.get_context("fork").Pool(len(others))
+            pending = [pool.apply_async(_mesh, (n, not args.no_layout)) for n in others]
 
     pts, tets, cam = build_workload(args)
-    for extra in [int(t) for t in args.sweep_tiles.split(",") if t.strip()]:
-        r = run_config(args, pts, tets, cam, extra)
-        log(json.dumps({"sweep_tile": extra, **{k: r[k] for k in ("value", "ms_per_step", "breakdown_ms", "config")}}))
-    res = run_config(args, pts, tets, cam, args.tile)
+    res = None
+    if not args.serve_only:
+        for extra in [int(t) for t in args.sweep_tiles.split(",") if t.strip()]:
+            r = run_config(args, pts, tets, cam, extra)
+            log(json.dumps({"sweep_tile": extra, **{k: r[k] for k in ("value", "ms_per_step", "breakdown_ms", "config")}}))
+        res = run_config(args, pts, tets, cam, args.tile)
+    serving = None
+    if pending:
+        built = {args.points: (pts, tets)}
+        for job in pending:
+            n, p_, t_, secs = job.get()
+            log(f"[bench] serving mesh {n} points: {len(t_)} tets ({secs:.1f} s)")
+            built[n] = (p_, t_)
+        pool.close()
+        serving = serve_leg(args, [(n, *built[n]) for n in serve_points])
+    elif serve_points and not args.profile_steps:
+        serving = serve_leg(args, [(args.points, pts, tets)])
     cpu = None
     if not args.no_cpu_baseline and not args.profile_steps:
         cpu = cpu_baseline(args, pts, tets, cam)
+    if res is None:
+        print(json.dumps({"metric": "batched-served vs per-frame fps", "serving": serving}), flush=True)
+        return
 
     out = {
-        "metric": "fwd+bwd Mrays/s @1M verts",
+        "metric": "fwd+bwd Mrays/s @1M verts; batched-served vs per-frame fps + p99 ms @<=1e-4 err",
         "value": res["value"],
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -223,6 +351,7 @@ def main():
         "hbm_frac_all_kernels": res["hbm_frac_all_kernels"],
         "breakdown_ms": res["breakdown_ms"],
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -62,12 +62,9 @@ class Bucket:
         big = max(range(len(server.scenes)), key=lambda i: server.scenes[i].rec.n_tets)
         cam = server.probe_camera(H, W)
         self.info = server.plan_info([(big, cam)] * slots)
-        self.pair_cap = pair_cap
         self.plan_bytes = int(self.info.plan_bytes)
         self.plan_host = torch.empty(self.plan_bytes, dtype=torch.uint8, pin_memory=True)
         self.plan_dev = torch.empty(self.plan_bytes, dtype=torch.uint8, device=dev)
-        wsb = lib().rm_batch_workspace_size(self.info, pair_cap)
-        self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         npx = slots * H * W
         self.img = torch.empty((npx, 4), device=dev)
         self.dist = torch.empty((npx, 5), device=dev)
@@ -75,10 +72,20 @@ class Bucket:
         self.counts = torch.zeros(2, device=dev, dtype=torch.int64)
         self.counts_host = torch.zeros(2, dtype=torch.int64, pin_memory=True)
         self.stream = torch.cuda.Stream(device=dev)
-        self.bytes = wsb + npx * 40 + 2 * self.plan_bytes
         self.graph = None
         self.use_graph = use_graph
-        self.busy = None   # (requests, images, event) of the batch in flight
+        self.busy = None   # (requests, outputs, event, keep_aux) of the batch in flight
+        self.ws = None
+        self._alloc(pair_cap)
+
+    def _alloc(self, pair_cap):
+        """(Re)sizes the workspace for `pair_cap` (tile, tet) pairs; a captured graph is dropped."""
+        self.pair_cap = int(pair_cap)
+        self.ws = None
+        self.ws = torch.empty(lib().rm_batch_workspace_size(self.info, self.pair_cap), dtype=torch.uint8,
+                              device=self.server.device)
+        self.bytes = self.ws.numel() + self.img.numel() * 40 // 4 + 2 * self.plan_bytes
+        self.graph = None
 
     def _render_call(self):
         check(lib().rm_batch_render(ptr(self.plan_dev), self.info, self.pair_cap, ptr(self.img), ptr(self.dist),
@@ -106,7 +113,7 @@ class Bucket:
             outs = (self.img[:n].clone(),) + ((self.dist[:n].clone(), self.nc[:n].clone()) if keep_aux else ())
             ev = torch.cuda.Event()
             ev.record(self.stream)
-        self.busy = (reqs, outs, ev)
+        self.busy = (reqs, outs, ev, keep_aux)
         return ev
 
     def _capture(self):
@@ -122,38 +129,49 @@ class Bucket:
         return self.busy is not None and self.busy[2].query()
 
     def finish(self):
-        """[(request, ([H,W,4] image, [H,W,5] distortion, [H,W] n_contrib) or (image,)), ...] of the
-        completed batch. When the batch overflowed the pair capacity its images are incomplete: the
-        views are rendered again one at a time (image only)."""
-        reqs, outs, ev = self.busy
+        """([(request, ([H,W,4] image, [H,W,5] distortion, [H,W] n_contrib) or (image,)), ...],
+        overflowed) of the completed batch. A batch whose pairs exceeded the capacity is rendered
+        again after the workspace grows to fit it (or view by view through the single-view path
+        when the HBM budget does not allow the growth)."""
+        reqs, outs, ev, keep_aux = self.busy
         ev.synchronize()
         self.busy = None
-        overflow = int(self.counts_host[1]) > self.pair_cap
+        n_pairs = int(self.counts_host[1])
+        if n_pairs > self.pair_cap:
+            if self.server.on_overflow == "raise":
+                raise RuntimeError(f"batch of {len(reqs)} views needs {n_pairs} pairs, capacity {self.pair_cap}")
+            need = -(-n_pairs // len(reqs)) * self.slots * 5 // 4 + 4096
+            grown = need < (1 << 31) - 1 and self.server.can_grow(self, need)
+            if grown:
+                self._alloc(need)
+                self.launch(reqs, keep_aux)
+                res, _ = self.finish()
+            else:
+                res = [(r, (self.server.render_single(r.scene, r.camera),)) for r in reqs]
+            return res, True
         hw = self.H * self.W
         shapes = ((self.H, self.W, 4), (self.H, self.W, 5), (self.H, self.W))
-        res = []
-        for i, r in enumerate(reqs):
-            if overflow:
-                res.append((r, (self.server.render_single(r.scene, r.camera),)))
-            else:
-                res.append((r, tuple(o[i * hw:(i + 1) * hw].view(sh) for o, sh in zip(outs, shapes))))
-        return res, overflow
+        res = [(r, tuple(o[i * hw:(i + 1) * hw].view(sh) for o, sh in zip(outs, shapes))) for i, r in enumerate(reqs)]
+        return res, False
 
 
 class RenderServer:
     def __init__(self, tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=64 << 30, max_slots=8,
-                 use_graphs=True, device="cuda"):
+                 use_graphs=True, device="cuda", pair_margin=1.3, probe_radius=6.0, on_overflow="grow"):
         self.tile, self.n_features, self.min_t = tile, n_features, min_t
+        self.pair_margin, self.probe_radius = pair_margin, probe_radius
+        if on_overflow not in ("grow", "raise"):
+            raise ValueError("on_overflow must be 'grow' or 'raise'")
+        self.on_overflow = on_overflow
         self.budget = hbm_budget_bytes
         self.max_slots = max_slots
         self.use_graphs = use_graphs
         self.device = device
         self.scenes: list[_Scene] = []
         self.buckets: dict = {}
-        self.pair_per_px: dict = {}   # scene -> pairs per pixel measured at registration
 
     # ------------------------------------------------------------------ scenes
-    def add_scene(self, vertices, indices, features, probe_camera=None):
+    def add_scene(self, vertices, indices, features):
         v = vertices.to(self.device, torch.float32).contiguous()
         i = indices.to(self.device, torch.int32).contiguous()
         f = features.to(self.device, torch.float32).contiguous()
@@ -162,11 +180,7 @@ class RenderServer:
         rec = RmScene(indices_dev=i.data_ptr(), vertices_dev=v.data_ptr(), features_dev=f.data_ptr(),
                       n_tets=i.shape[0], n_verts=v.shape[0], n_features=f.shape[1])
         self.scenes.append(_Scene(i, v, f, rec))
-        sid = len(self.scenes) - 1
-        if probe_camera is not None:
-            P = self._pairs_of(sid, probe_camera)
-            self.pair_per_px[sid] = P / (probe_camera.width * probe_camera.height)
-        return sid
+        return len(self.scenes) - 1
 
     def _pairs_of(self, sid, camera):
         from .rasterizer import RenderGrid, vertex_and_tile_shader
@@ -196,7 +210,7 @@ class RenderServer:
     # ------------------------------------------------------------------ buckets
     def probe_camera(self, H, W):
         from . import scenes
-        return scenes.orbit_camera(6.0, 0.0, 0.0, W, H, 50.0)
+        return scenes.orbit_camera(self.probe_radius, 0.0, 0.0, W, H, 50.0)
 
     def plan_info(self, views, plan_host=None):
         """rm_batch_plan over [(scene, camera), ...]; packs into `plan_host` (pinned) when given."""
@@ -211,20 +225,28 @@ class RenderServer:
               "rm_batch_plan")
         return info
 
+    def can_grow(self, bucket, pair_cap):
+        """Whether `bucket` may grow its workspace to `pair_cap` pairs within the HBM budget."""
+        extra = lib().rm_batch_workspace_size(bucket.info, pair_cap) - bucket.ws.numel()
+        return sum(b.bytes for b in self.buckets.values()) + extra <= self.budget
+
     def bucket(self, H, W):
-        """The bucket of resolution HxW, created on first use: as many slots (up to max_slots) as
-        the HBM budget left over by the other buckets allows."""
+        """The bucket of resolution HxW, created on first use. Its pair capacity per view is
+        pair_margin times the largest pair count of any registered scene seen from a probe pose at this resolution (each
+        visible tet costs at least one pair, so small images of large meshes need far more pairs
+        per pixel than large ones); slots (up to max_slots) are what the HBM budget left over by
+        the other buckets allows."""
         key = (H, W)
         if key not in self.buckets:
-            ppx = max(self.pair_per_px.values()) if self.pair_per_px else 64.0
-            per_view_pairs = int(ppx * H * W * 1.5) + 4096
-            remaining = self.budget - sum(b.bytes for b in self.buckets.values())
+            probe = self.probe_camera(H, W)
+            per_view_pairs = int(self.pair_margin * max(self._pairs_of(s, probe) for s in range(len(self.scenes)))) + 4096
             big = max(range(len(self.scenes)), key=lambda i: self.scenes[i].rec.n_tets)
+            remaining = self.budget - sum(b.bytes for b in self.buckets.values())
             slots = self.max_slots
             while slots > 1:
-                info = self.plan_info([(big, self.probe_camera(H, W))] * slots)
+                info = self.plan_info([(big, probe)] * slots)
                 need = lib().rm_batch_workspace_size(info, per_view_pairs * slots) + slots * H * W * 40
-                if need <= remaining:
+                if need <= remaining and per_view_pairs * slots < (1 << 31) - 1:
                     break
                 slots //= 2
             self.buckets[key] = Bucket(self, H, W, slots, per_view_pairs * slots, self.use_graphs)
@@ -244,11 +266,14 @@ class RenderServer:
         return out
 
     # ------------------------------------------------------------------ trace replay
-    def serve(self, trace, mode="batched", max_batch=None, poll_s=2e-4):
+    def serve(self, trace, mode="batched", max_batch=None, keep=None, poll_s=2e-4):
         """Replays `trace` (Requests with arrival times in seconds from now) in real time.
         mode "batched": continuous batching (at most max_batch views per batch, default the
         bucket's slots); "per_frame": one view at a time through the single-view path.
-        Returns stats: frames/s over the makespan, latency p50/p99 (ms), batches, overflows."""
+        Results are stored in Request.image for the requests `keep(request)` accepts (all when
+        keep is None). Returns stats: frames/s over the makespan, latency p50/p99 (ms), batches,
+        overflows."""
+        keep = keep or (lambda r: True)
         torch.cuda.synchronize()
         reqs = sorted(trace, key=lambda r: r.arrival)
         pending = deque(reqs)
@@ -264,9 +289,11 @@ class RenderServer:
                 wait = r.arrival - clock()
                 if wait > 0:
                     time.sleep(wait)
-                r.image = self.render_single(r.scene, r.camera)
+                img = self.render_single(r.scene, r.camera)
                 torch.cuda.current_stream().synchronize()
                 r.done = clock()
+                if keep(r):
+                    r.image = img
                 n_batches += 1
                 sizes.append(1)
         else:
@@ -280,12 +307,15 @@ class RenderServer:
                 progressed = False
                 for b in self.buckets.values():
                     if b.ready():
+                        held = b.bytes
                         res, ov = b.finish()
                         t = clock()
                         for r, o in res:
-                            r.done, r.image = t, o[0]
+                            r.done = t
+                            if keep(r):
+                                r.image = o[0]
                         done += len(res)
-                        in_use -= b.bytes
+                        in_use -= held
                         overflows += int(ov)
                         progressed = True
                 # admission: oldest waiting request first, if its bucket is free and the budget allows

@@ -17,13 +17,15 @@ RES_B = (48, 64)
 def _server(use_graphs, max_slots=4):
     import torch
     from radiance_meshes_amd.serving import RenderServer
+    # capacity probe closer than any test view and a wide margin: these tests never overflow
     srv = RenderServer(tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=4 << 30, max_slots=max_slots,
-                       use_graphs=use_graphs)
+                       use_graphs=use_graphs, pair_margin=4.0, probe_radius=2.5,
+                       on_overflow="raise")
     for kind, kw in (("plain", {}), ("ball", {}), ("gmm", dict(n_points=20000, width=160, height=120))):
         sc = scene(kind, 4, 13, **kw)
         srv.add_scene(torch.from_numpy(np.ascontiguousarray(sc.vertices, np.float32)),
                       torch.from_numpy(np.ascontiguousarray(sc.indices, np.int32)),
-                      torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)), probe_camera=sc.camera)
+                      torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)))
     return srv
 
 
@@ -86,6 +88,7 @@ def test_mixed_batch_matches_single_views(use_graphs):
         torch.randn(1 << 20, device="cuda").sum().item()
     for s, c in views_a[:3] + views_b[:1]:
         alone = srv.render_views([(s, c)], keep_aux=True)[0]
+        assert len(alone) == 3, "pair capacity overflowed"
         _assert_same(alone, ref[id(c)], f"alone scene {s}")
     assert any(int(r[2].sum()) > 0 for r in ref.values())
 
@@ -95,6 +98,7 @@ def test_batched_view_against_oracle(oracle_lib):
     srv = _server(False, max_slots=2)
     views = _views(RES_A, 2, 5)
     got = srv.render_views(views, keep_aux=True)
+    assert all(len(g) == 3 for g in got), "pair capacity overflowed"
     for (s, c), (img, dist, nc) in zip(views, got):
         sc = srv.scenes[s]
         V = sc.vertices.cpu().numpy()
@@ -122,6 +126,7 @@ def test_serve_trace_completes_every_request():
     print(f"trace of {len(trace)} requests", flush=True)
     st = srv.serve(trace, mode="batched")
     print(st)
+    assert st["overflows"] == 0
     assert st["frames"] == len(trace) and all(r.done >= r.arrival for r in trace)
     for r in trace[:4]:
         ref = _single_full(srv, r.scene, r.camera)[0]
