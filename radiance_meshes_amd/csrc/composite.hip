@@ -472,6 +472,66 @@ __device__ inline void group_sum(float (&v)[N], int width) {
     }
 }
 
+// Per-tet gradients from the reduced per-slot sums acc = [dsig, dbase[3], Sq[NCH], Rg[3*NCH],
+// u[4], r[12]]: outputs 0..11 are the four vertices' gradients, 12.. the feature gradients.
+// Colour gradients enter through the base colour folded at v0 (cv0 = cam - v0); each face's
+// plane record n = +-cross(b - a, c - a), e = n . (cam - a) passes its adjoint to a, b, c.
+// The face normals are recomputed from the staged vertices in the plane precision (the same
+// cross product plane_rec formed, flipped by the recorded sign), so the plane records in LDS are
+// not read here.
+template <int WG, int F, bool P64, int NV, int NOUT>
+__device__ inline void bwd_outputs(const Stage<WG, F, true, P64>& S, int slot, const float (&acc)[NV], float3 cp,
+                                   float (&outv)[NOUT]) {
+    using T = typename Prec<P64>::R;
+    constexpr int NGRD = NG_<F>::NGRD;
+    constexpr int NCH = F == 7 ? 1 : 3;
+    const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
+    const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x), f3(q2.y, q2.z, q2.w)};
+    const int signs = S.meta[1][slot];
+    float gr[NGRD];
+    read_grads<WG, F, true, P64>(S, slot, gr);
+    float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
+    const float3 cv0 = cp - V4[0];
+    outv[12] = acc[0];
+    outv[13] = acc[1]; outv[14] = acc[2]; outv[15] = acc[3];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        float Sq = acc[4 + ch];
+        float3 grow = f3(gr[3 * ch], gr[(3 * ch + 1) % NGRD], gr[(3 * ch + 2) % NGRD]);
+        gv[0] = gv[0] - Sq * grow;
+        outv[16 + 3 * ch] = Sq * cv0.x + acc[4 + NCH + 3 * ch];
+        outv[17 + 3 * ch] = Sq * cv0.y + acc[5 + NCH + 3 * ch];
+        outv[18 + 3 * ch] = Sq * cv0.z + acc[6 + NCH + 3 * ch];
+    }
+    const int PU = 4 + NCH + 3 * NCH;
+    const int fa_[4] = {0, 0, 0, 1}, fb_[4] = {1, 1, 2, 2}, fc_[4] = {2, 3, 3, 3};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float u = acc[PU + i];
+        const float3 rr = f3(acc[PU + 4 + 3 * i], acc[PU + 5 + 3 * i], acc[PU + 6 + 3 * i]);
+        const int a = fa_[i], b = fb_[i], cc = fc_[i];
+        const float sgn = ((signs >> i) & 1) ? -1.f : 1.f;
+        const V3<T> va = {(T)V4[a].x, (T)V4[a].y, (T)V4[a].z}, vb = {(T)V4[b].x, (T)V4[b].y, (T)V4[b].z};
+        const V3<T> vc = {(T)V4[cc].x, (T)V4[cc].y, (T)V4[cc].z};
+        const V3<T> n = v3cross(v3sub(vb, va), v3sub(vc, va));
+        const float3 nn = sgn * f3((float)n.x, (float)n.y, (float)n.z);
+        const float3 gn = (-u) * (cp - V4[a]) - rr;
+        gv[a] = gv[a] + u * nn;
+        const float3 gcr = sgn * gn;
+        const float3 e1 = V4[b] - V4[a], e2 = V4[cc] - V4[a];
+        const float3 ge1 = cross3(e2, gcr), ge2 = cross3(gcr, e1);
+        gv[b] = gv[b] + ge1;
+        gv[cc] = gv[cc] + ge2;
+        gv[a] = gv[a] - (ge1 + ge2);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        outv[3 * v] = gv[v].x;
+        outv[3 * v + 1] = gv[v].y;
+        outv[3 * v + 2] = gv[v].z;
+    }
+}
+
 template <int TS, int F, bool P64>
 __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
@@ -489,6 +549,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     using PL = typename Prec<P64>::PL;
     __shared__ Stage<C::WG, F, true, P64> S;
     __shared__ int s_nc[C::NG];
+    // deferred per-lane epilogue: the reduction group is the tile group, the workgroup one wave,
+    // and the slots' gradients fit over the plane records (FP64 records; FP32 ones are too small)
+    constexpr bool kDefer = C::RED == C::GROUP && C::WG == 64 && sizeof(float) * C::WG * (NOUT + 1) <= sizeof(S.pl);
 
     int g, r, tx, ty, px, py;
     bool tile_ok, inside;
@@ -530,6 +593,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     const int lane = threadIdx.x & 63;
     const int red_base = lane & ~(C::RED - 1);
     const uint64_t red_mask = (C::RED == 64) ? ~0ull : (((1ull << C::RED) - 1ull) << red_base);
+    float kept[NV];   // reduced sums of the slot with in-group index r (deferred epilogue)
+    bool kept_on = false;
 
     for (int c = nchunks_wg - 1; c >= 0; --c) {
         __syncthreads();
@@ -537,8 +602,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         if (c < nchunks_g && k < nmax_g)
             stage_tet<C::WG, F, true, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
         __syncthreads();
-        if (c >= nchunks_g) continue;               // uniform per tile group
-        int cnt = nmax_g - c * C::GROUP;
+        // tile groups past their last chunk run no slots (cnt = 0) but still reach the barriers below
+        int cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
         cnt = cnt < C::GROUP ? cnt : C::GROUP;
         const int base = g * C::GROUP;
         for (int jj = cnt - 1; jj >= 0; --jj) {
@@ -659,52 +724,19 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             uint64_t any = __ballot(active) & red_mask;
             if (any == 0) continue;
             group_sum<NV>(acc, C::RED);
-            // per-tet epilogue (uniform across the reduction group)
-            const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
-            const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x),
-                                  f3(q2.y, q2.z, q2.w)};
-            const int tet = S.meta[0][slot], signs = S.meta[1][slot];
-            float gr[NGRD];
-            read_grads<C::WG, F, true, P64>(S, slot, gr);
-            float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
-            const float3 cv0 = cp - V4[0];
+            if constexpr (kDefer) {
+                // the lane whose in-group index is this slot keeps the sums; the chunk's epilogues
+                // then run once per lane instead of once per slot in every lane of the group
+                if (r == jj) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) kept[i] = acc[i];
+                    kept_on = true;
+                }
+                continue;
+            }
             float outv[NOUT];
-            outv[12] = acc[0];
-            outv[13] = acc[1]; outv[14] = acc[2]; outv[15] = acc[3];
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-                float Sq = acc[4 + ch];
-                float3 grow = f3(gr[3 * ch], gr[(3 * ch + 1) % NGRD], gr[(3 * ch + 2) % NGRD]);
-                gv[0] = gv[0] - Sq * grow;
-                outv[16 + 3 * ch] = Sq * cv0.x + acc[4 + NCH + 3 * ch];
-                outv[17 + 3 * ch] = Sq * cv0.y + acc[5 + NCH + 3 * ch];
-                outv[18 + 3 * ch] = Sq * cv0.z + acc[6 + NCH + 3 * ch];
-            }
-            const int PU = 4 + NCH + 3 * NCH;
-            const int fa_[4] = {0, 0, 0, 1}, fb_[4] = {1, 1, 2, 2}, fc_[4] = {2, 3, 3, 3};
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float u = acc[PU + i];
-                const float3 rr = f3(acc[PU + 4 + 3 * i], acc[PU + 5 + 3 * i], acc[PU + 6 + 3 * i]);
-                const PL pl = S.pl[i][slot];
-                const float3 nn = f3((float)pl.x, (float)pl.y, (float)pl.z);
-                const int a = fa_[i], b = fb_[i], cc = fc_[i];
-                const float3 gn = (-u) * (cp - V4[a]) - rr;
-                gv[a] = gv[a] + u * nn;
-                const float sgn = ((signs >> i) & 1) ? -1.f : 1.f;
-                const float3 gcr = sgn * gn;
-                const float3 e1 = V4[b] - V4[a], e2 = V4[cc] - V4[a];
-                const float3 ge1 = cross3(e2, gcr), ge2 = cross3(gcr, e1);
-                gv[b] = gv[b] + ge1;
-                gv[cc] = gv[cc] + ge2;
-                gv[a] = gv[a] - (ge1 + ge2);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                outv[3 * v] = gv[v].x;
-                outv[3 * v + 1] = gv[v].y;
-                outv[3 * v + 2] = gv[v].z;
-            }
+            bwd_outputs<C::WG, F, P64, NV, NOUT>(S, slot, acc, cp, outv);
+            const int tet = S.meta[0][slot];
             // lane l of the reduction group adds outputs l, l + RED, ... (NOUT may exceed RED)
 #pragma unroll
             for (int l0 = 0; l0 < NOUT; l0 += C::RED) {
@@ -721,6 +753,45 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                             unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
                         } else {
                             unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (kDefer) {
+            // each lane turns its kept slot's sums into the tet's gradients; the gradients then go
+            // through LDS (over the plane/colour records, dead by then) so that the atomics are
+            // issued tet by tet with consecutive lanes on consecutive outputs, as in the path above
+            // the gradients overlay the plane records only, which bwd_outputs does not read
+            float* mine = reinterpret_cast<float*>(&S.pl[0][0]) + (base + r) * (NOUT + 1);
+            if (kept_on) {
+                float outv[NOUT];
+                bwd_outputs<C::WG, F, P64, NV, NOUT>(S, base + r, kept, cp, outv);
+#pragma unroll
+                for (int i = 0; i < NOUT; ++i) mine[i] = outv[i];
+            }
+            mine[NOUT] = kept_on ? 1.f : 0.f;
+            kept_on = false;
+            __syncthreads();
+#pragma unroll 1
+            for (int jj = 0; jj < cnt; ++jj) {
+                const float* out = reinterpret_cast<const float*>(&S.pl[0][0]) + (base + jj) * (NOUT + 1);
+                if (out[NOUT] == 0.f) continue;          // uniform per tile group
+                const int tet = S.meta[0][base + jj];
+#pragma unroll
+                for (int l0 = 0; l0 < NOUT; l0 += C::GROUP) {
+                    const int l = r + l0;
+                    if (l < NOUT) {
+                        const float val = out[l];
+                        if (val != 0.f) {
+                            if (l < 12) {
+                                const int4 qi = indices[tet];
+                                const int vsel = l / 3;
+                                const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
+                                unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
+                            } else {
+                                unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                            }
                         }
                     }
                 }

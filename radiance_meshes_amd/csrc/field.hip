@@ -30,21 +30,33 @@ __device__ inline uint32_t corner_index(const GridArgs& g, int l, int cx, int cy
     return ((uint32_t)cx * 1u ^ (uint32_t)cy * 2654435761u ^ (uint32_t)cz * 805459861u) & g.mask;
 }
 
-__device__ inline void cell(const float* __restrict__ x, int64_t n, const GridArgs& g, int l, int& bx, int& by,
-                            int& bz, float& fx, float& fy, float& fz) {
+// Cell corner and fractions of a query point already clamped to [0, 1].
+__device__ inline void cell_of(const float v[3], const GridArgs& g, int l, int& bx, int& by, int& bz, float& fx,
+                               float& fy, float& fz) {
 #pragma clang fp contract(off)
     // x * r and + 0.5 round separately, as the reference's float32 torch ops do
     const float r = (float)g.res[l];
     float p[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float v = fminf(fmaxf(x[3 * n + k], 0.0f), 1.0f);
-        const float xr = v * r;
+        const float xr = v[k] * r;
         p[k] = xr + 0.5f;
     }
     float f0 = floorf(p[0]), f1 = floorf(p[1]), f2 = floorf(p[2]);
     bx = (int)f0; by = (int)f1; bz = (int)f2;
     fx = p[0] - f0; fy = p[1] - f1; fz = p[2] - f2;
+}
+
+__device__ inline void load_point(const float* __restrict__ x, int64_t n, float v[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = fminf(fmaxf(x[3 * n + k], 0.0f), 1.0f);
+}
+
+__device__ inline void cell(const float* __restrict__ x, int64_t n, const GridArgs& g, int l, int& bx, int& by,
+                            int& bz, float& fx, float& fy, float& fz) {
+    float v[3];
+    load_point(x, n, v);
+    cell_of(v, g, l, bx, by, bz, fx, fy, fz);
 }
 
 template <int F>
@@ -139,6 +151,91 @@ __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restri
     }
 }
 
+// Backward, chunk-major with workgroup aggregation (n_feat 1 or 2): a workgroup owns 256
+// consecutive query points, one per lane (a Morton-ordered mesh makes them a compact region), and
+// walks every level. Per level its 8 x 256 corner updates are summed in an LDS open-addressing
+// table keyed by table slot and then flushed with one global atomic per distinct (slot, feature).
+// Each grad_out row is read once (a level-major pass re-reads every point's row once per level),
+// and the global atomics drop to the distinct slots a chunk touches (about a third of the
+// per-lane count on a 6.7M-point Morton-ordered query set).
+constexpr int kAggPoints = 256;
+constexpr int kAggLog2Slots = 12;
+constexpr int kAggSlots = 1 << kAggLog2Slots;  // > 8 corners x 256 points: an insert always finds room
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;   // table slots are < 2^31
+
+template <int F>
+__global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const float* __restrict__ x, int64_t N,
+                                                                       const float* __restrict__ gout, GridArgs g,
+                                                                       float* __restrict__ gtable) {
+    static_assert(F == 1 || F == 2, "LDS table sized for n_feat <= 2");
+    __shared__ uint32_t s_key[kAggSlots];
+    __shared__ float s_val[kAggSlots * F];
+    const int tid = threadIdx.x;
+    // dispatch slots go round-robin over the 8 XCDs: give each XCD a contiguous band of chunks so
+    // the coarse-level slots that neighbouring chunks share stay in one L2
+    const int nb = (int)gridDim.x, per = nb >> 3, bi = (int)blockIdx.x;
+    const int b = bi < (per << 3) ? (bi & 7) * per + (bi >> 3) : bi;
+    const int64_t n = (int64_t)b * kAggPoints + tid;
+    const bool valid = n < N;
+    float xv[3] = {0.f, 0.f, 0.f};
+    if (valid) load_point(x, n, xv);
+    for (int p = tid; p < kAggSlots; p += kAggPoints) s_key[p] = kEmptySlot;
+    for (int p = tid; p < kAggSlots * F; p += kAggPoints) s_val[p] = 0.f;
+    __syncthreads();
+    for (int l = 0; l < g.L; ++l) {
+        if (valid) {
+            float go[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) go[f] = gout[(n * g.L + l) * F + f];
+            int bx, by, bz;
+            float fx, fy, fz;
+            cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+                const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+                float v[F];
+                bool nz = false;
+#pragma unroll
+                for (int f = 0; f < F; ++f) {
+                    v[f] = w * go[f];
+                    nz = nz || v[f] != 0.f;
+                }
+                if (!nz) continue;
+                const uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
+                uint32_t p = (h * 2654435761u) >> (32 - kAggLog2Slots);
+                bool placed = false;
+                for (int probe = 0; probe < kAggSlots && !placed; ++probe) {
+                    const uint32_t k = atomicCAS(&s_key[p], kEmptySlot, h);
+                    if (k == kEmptySlot || k == h) {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) atomicAdd(&s_val[p * F + f], v[f]);
+                        placed = true;
+                    }
+                    p = (p + 1) & (kAggSlots - 1);
+                }
+                if (!placed)  // unreachable while the table holds more slots than a level can insert
+#pragma unroll
+                    for (int f = 0; f < F; ++f) unsafeAtomicAdd(gtable + (g.offs[l] + (int64_t)h) * F + f, v[f]);
+            }
+        }
+        __syncthreads();
+        float* tb = gtable + g.offs[l] * F;
+        for (int p = tid; p < kAggSlots; p += kAggPoints) {
+            const uint32_t k = s_key[p];
+            if (k == kEmptySlot) continue;
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const float v = s_val[p * F + f];
+                if (v != 0.f) unsafeAtomicAdd(tb + (int64_t)k * F + f, v);
+                s_val[p * F + f] = 0.f;
+            }
+            s_key[p] = kEmptySlot;
+        }
+        __syncthreads();
+    }
+}
+
 int make_grid_args(const int32_t* res, const int64_t* offs, int32_t L, int32_t F, int32_t log2_size, GridArgs& g,
                    const char* who) {
     std::string w(who);
@@ -199,6 +296,17 @@ extern "C" int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_
     if (e) return e;
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && grad_out_dev && grad_table_dev, "rm_hashgrid_bwd: null buffer");
+    if (n_feat <= 2) {
+        dim3 grid((unsigned)ceil_div(n, kAggPoints)), block(kAggPoints);
+        if (n_feat == 1)
+            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<1>, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
+                               grad_table_dev);
+        else
+            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<2>, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
+                               grad_table_dev);
+        RM_CHECK_LAUNCH("hashgrid_bwd_agg_kernel");
+        return RM_OK;
+    }
     dim3 grid((unsigned)ceil_div(n, 256), (unsigned)n_levels), block(256);
     RM_FEAT_DISPATCH(n_feat, hashgrid_bwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
                      grad_table_dev);
