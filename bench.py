@@ -153,8 +153,7 @@ def run_config(args, pts, tets, cam, tile):
     def step():
         opt.zero_grad(set_to_none=True)
         sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(indices, vertices, tcam, rg)
-        feats = torch.zeros((T, 13), device=dev)
-        feats[mask] = field(vertices, indices, mask)
+        feats = field.features(vertices, indices, mask)
         img, dist, _ = AlphaBlendTiledRender.apply(sidx, ranges, indices, vertices, feats, rg, tcam, jitter)
         loss = ((img[..., :3] - target) ** 2).mean() + 0.01 * distortion_loss(dist).mean()
         loss.backward()

@@ -213,6 +213,23 @@ int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const float* b1, 
                const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1, float* gw2,
                float* gb2, float* gw3, float* gb3, rm_stream_t stream);
 
+/* ---- field glue -----------------------------------------------------------------------------
+ * Replaces the PyTorch steps around the field for a list of visible tets (tet_ids_dev, n ids):
+ * rm_field_points writes the query point of each listed tet, its centroid
+ * (models/ingp_color.py:376, vertices[indices].mean(dim=1)) divided by scale and contracted into
+ * [0,1]^3 (utils/contraction.py:6-20 position part: |z| <= 1 kept, else (2|z| - 1)/|z|^2 z; then
+ * (z/2 + 1)/2), [n, 3].
+ * rm_field_scatter_fwd writes the activated head outputs raw [n, n_features] into the listed tets'
+ * rows of features [T, n_features] (models/ingp_color.py:200-203: density exp(raw) with raw clamped
+ * at log(1e20), base colour raw + 0.5, colour-gradient rows as they are); other rows are untouched.
+ * rm_field_scatter_bwd writes grad_raw [n, n_features] from grad_features [T, n_features]. */
+int rm_field_points(const int32_t* indices_dev, const float* vertices_dev, const int32_t* tet_ids_dev, int64_t n,
+                    float scale, float* points_dev, rm_stream_t stream);
+int rm_field_scatter_fwd(const float* raw_dev, const int32_t* tet_ids_dev, int64_t n, int32_t n_features,
+                         float* features_dev, rm_stream_t stream);
+int rm_field_scatter_bwd(const float* raw_dev, const int32_t* tet_ids_dev, int64_t n, int32_t n_features,
+                         const float* grad_features_dev, float* grad_raw_dev, rm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,9 @@ SIGNATURES = {
     "rm_mlp_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "rm_mlp_bwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P,
                              _P, _P]),
+    "rm_field_points": (C.c_int, [_P, _P, _P, _I64, C.c_float, _P, _P]),
+    "rm_field_scatter_fwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P]),
+    "rm_field_scatter_bwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P]),
 }
 
 _lib = None

@@ -136,6 +136,43 @@ def activate(raw, n_features=13):
     return torch.cat([dens, rgb, raw[:, 4:7]], 1)
 
 
+def field_points(vertices, indices, tet_ids, scale):
+    """Query points of the listed tets: centroid / scale contracted into [0,1]^3 (rm_field_points;
+    the same arithmetic as contract(vertices[indices[tet_ids]].mean(1), 0, scale)). No gradient."""
+    vertices = vertices.detach().contiguous().float()
+    out = torch.empty((tet_ids.shape[0], 3), device=vertices.device, dtype=torch.float32)
+    with timing.span("field_glue"):
+        check(lib().rm_field_points(ptr(indices.contiguous()), ptr(vertices), ptr(tet_ids), tet_ids.shape[0],
+                                    float(scale), ptr(out), stream_of(vertices.device)), "rm_field_points")
+    return out
+
+
+class _FieldScatter(torch.autograd.Function):
+    """activate(raw) written into rows tet_ids of a zero [n_tets, F] feature tensor
+    (rm_field_scatter_fwd) and its adjoint (rm_field_scatter_bwd)."""
+
+    @staticmethod
+    def forward(ctx, raw, tet_ids, n_tets):
+        raw = raw.contiguous()
+        F = raw.shape[1]
+        feats = torch.zeros((n_tets, F), device=raw.device, dtype=torch.float32)
+        with timing.span("field_glue"):
+            check(lib().rm_field_scatter_fwd(ptr(raw), ptr(tet_ids), raw.shape[0], F, ptr(feats),
+                                             stream_of(raw.device)), "rm_field_scatter_fwd")
+        ctx.save_for_backward(raw, tet_ids)
+        return feats
+
+    @staticmethod
+    def backward(ctx, gfeats):
+        raw, tet_ids = ctx.saved_tensors
+        graw = torch.empty_like(raw)
+        with timing.span("field_glue"):
+            check(lib().rm_field_scatter_bwd(ptr(raw), ptr(tet_ids), raw.shape[0], raw.shape[1],
+                                             ptr(gfeats.contiguous()), ptr(graw), stream_of(raw.device)),
+                  "rm_field_scatter_bwd")
+        return graw, None, None
+
+
 class Field(torch.nn.Module):
     """Tet-centroid field: centroid -> contraction -> hash grid -> MLP -> features."""
 
@@ -151,3 +188,11 @@ class Field(torch.nn.Module):
         centroid = vertices[idx.long()].mean(dim=1)
         h = self.grid(contract(centroid, 0.0, self.scale).detach())
         return activate(self.mlp(h), self.n_features)
+
+    def features(self, vertices, indices, mask):
+        """Per-tet features [T, n_features], zero rows outside mask: the same values as
+        forward() scattered into a [T, F] tensor, with the glue fused into three kernels."""
+        tet_ids = torch.nonzero(mask).reshape(-1).to(torch.int32)
+        x = field_points(vertices, indices, tet_ids, self.scale)
+        raw = self.mlp(self.grid(x))
+        return _FieldScatter.apply(raw, tet_ids, indices.shape[0])

@@ -94,3 +94,76 @@ def test_mlp_backward_matches_fp64(precision, tol):
         rel = (a.cpu().double() - b).norm() / b.norm()
         print(f"grad {name}: rel L2 err vs FP64 = {rel:.3e}")
         assert rel <= tol, name
+
+
+def test_hashgrid_backward_clustered_points(oracle_lib):
+    """Morton-sorted points in a few tight clusters: 256-point chunks share most corner slots at
+    every level, so the workgroup aggregation merges many updates per slot before the atomics."""
+    import torch
+    from radiance_meshes_amd.field import _HashEncode
+    from radiance_meshes_amd.layout import morton_codes
+    O = oracle_lib
+    _, table, res, offs, L, F, log2 = _grid(seed=3, n=1)
+    rng = np.random.default_rng(4)
+    centres = rng.uniform(0.2, 0.8, (6, 3))
+    x = (centres[rng.integers(0, 6, 30000)] + rng.normal(0, 0.01, (30000, 3))).astype(np.float32)
+    x = x[np.argsort(morton_codes(x, np.zeros(3), np.ones(3)), kind="stable")]
+    g = rng.standard_normal((len(x), L * F))
+    print(f"clustered hashgrid bwd n={len(x)}", flush=True)
+    tt = torch.from_numpy(table).cuda().requires_grad_(True)
+    out = _HashEncode.apply(torch.from_numpy(x).cuda(), tt, res, offs, L, F, log2)
+    (out * torch.from_numpy(g).float().cuda()).sum().backward()
+    ref = O.hashgrid_bwd(x, g, res, offs, F, log2, table.shape[0])
+    got = tt.grad.cpu().numpy()
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"rel L2 err vs FP64 adjoint = {rel:.3e}, max abs = {np.abs(got - ref).max():.3e}")
+    assert rel <= 1e-5
+
+
+def _glue_case(nf=13, seed=5):
+    import torch
+    from tests._util import device_inputs, scene
+    sc = scene("ball", 16, nf)
+    verts, idx, feats, _ = device_inputs(sc)
+    rng = np.random.default_rng(seed)
+    mask = torch.from_numpy(rng.uniform(size=idx.shape[0]) < 0.6).cuda()
+    tet_ids = torch.nonzero(mask).reshape(-1).to(torch.int32)
+    return verts, idx, mask, tet_ids
+
+
+def test_field_points_match_torch_contraction():
+    import torch
+    from radiance_meshes_amd.field import contract, field_points
+    verts, idx, mask, tet_ids = _glue_case()
+    for scale in (0.3, 2.0):   # 0.3 puts most centroids outside the unit ball (contracted branch)
+        got = field_points(verts, idx, tet_ids, scale)
+        ref = contract(verts[idx[mask].long()].mean(dim=1), 0.0, scale)
+        err = (got - ref).abs().max().item()
+        print(f"scale {scale}: max abs diff vs torch = {err:.3e}")
+        assert err <= 1e-6   # centroid summation order differs from torch's mean by an ulp
+        assert float(got.min()) >= 0.0 and float(got.max()) <= 1.0
+
+
+@pytest.mark.parametrize("nf", [7, 13])
+def test_field_scatter_matches_torch(nf):
+    import torch
+    from radiance_meshes_amd.field import _FieldScatter, activate
+    verts, idx, mask, tet_ids = _glue_case(nf)
+    T = idx.shape[0]
+    rng = np.random.default_rng(6)
+    raw_np = rng.standard_normal((tet_ids.shape[0], nf)).astype(np.float32) * 3
+    raw_np[::97, 0] = 60.0          # density argument above the clamp
+    raw = torch.from_numpy(raw_np).cuda().requires_grad_(True)
+    out = _FieldScatter.apply(raw, tet_ids, T)
+    raw_ref = raw.detach().clone().requires_grad_(True)
+    ref = torch.zeros((T, nf), device="cuda")
+    ref = ref.index_put((tet_ids.long(),), activate(raw_ref, nf))
+    rel = ((out - ref).abs() / ref.abs().clamp_min(1.0)).max().item()
+    print(f"nf={nf}: max rel diff forward = {rel:.3e}")
+    assert rel <= 1e-6
+    w = torch.from_numpy(rng.standard_normal((T, nf)).astype(np.float32)).cuda()
+    (out * w).sum().backward()
+    (ref * w).sum().backward()
+    g_rel = ((raw.grad - raw_ref.grad).abs() / raw_ref.grad.abs().clamp_min(1.0)).max().item()
+    print(f"nf={nf}: max rel diff backward = {g_rel:.3e}")
+    assert g_rel <= 1e-6
