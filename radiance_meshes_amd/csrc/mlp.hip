@@ -1,19 +1,28 @@
 // Field MLP 32 -> 64 -> 64 -> n_out (n_out <= 16) with SELU hidden activations, on MFMA.
 //
 // Reference head: Linear -> SELU -> Linear -> SELU -> Linear (models/ingp_color.py:141-151),
-// weights row-major [out, in] as torch.nn.Linear. Each wave owns tiles of 16 query points; every
-// layer is a chain of 16x16 MFMA tiles with FP32 accumulation:
+// weights row-major [out, in] as torch.nn.Linear. Every product is a chain of 16x16 MFMA tiles
+// with FP32 accumulation:
 //   precision 0: v_mfma_f32_16x16x4_f32 (FP32 operands),
 //   precision 1: v_mfma_f32_16x16x16_bf16 (BF16 operands, rounded to nearest even).
-// Operand fragment layouts (64 lanes, q = lane / 16, c = lane % 16):
+// Operand fragments (64 lanes, q = lane / 16, c = lane % 16):
 //   A[i][k]: i = c, k = 4*step + q (f32) or k = 16*step + 4*q + e, e in 0..3 (bf16)
 //   B[k][j]: j = c, k as for A
 //   D[i][j]: j = c, i = 4*q + r for accumulator register r in 0..3
-// Weights and per-wave activation tiles live in LDS (rows padded by one float against bank
-// conflicts). The backward recomputes the forward per tile (as the reference's checkpointed
-// field), forms dX, and accumulates weight gradients gW = dH^T A as MFMA tiles with the 16
-// points of a tile as the contraction dimension, in registers across all tiles of the wave;
-// each wave adds its totals to global memory with one float atomic per element at the end.
+//
+// Layers are computed transposed, out^T[unit][point] = W[unit][k] x act^T[k][point]: the weights
+// are the A operand (read from LDS), the 16 points of a tile are the columns. A layer's result then
+// sits in the lanes as D[unit 16*ot + 4q + r][point c], and feeding it to the next layer only needs
+// the contraction order of k chosen to match: at step s the lane supplies unit
+// 16*(s/4) + 4q + (s%4) (f32) or 16*s + 4q + e (bf16), which is exactly its own register r (or e)
+// of block s/4 (or s). Activations never leave registers; the input rows come from global memory
+// in that layout (two float4 per lane) and dX leaves the same way.
+// The backward recomputes the forward per tile (as the reference's checkpointed field), chains
+// dY -> dH2 -> dH1 -> dX through the transposed weights, and forms the weight gradients
+// gW = dH act^T as MFMA tiles contracted over the tile's points; those need the point index on
+// the lanes' k, so dH and act blocks are transposed through a 16x17 LDS block per 16 units.
+// Weight-gradient tiles accumulate in registers across all tiles of a wave, are summed across the
+// workgroup in LDS and added to global memory with one float atomic per element.
 #include "common.h"
 
 namespace rmesh {
@@ -23,9 +32,11 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 32, kHid = 64, kOutMax = 16;
-constexpr int kWaves = 2, kThreads = 64 * kWaves;
+constexpr int kWaves = 4, kThreads = 64 * kWaves;
 constexpr int SW1 = kIn + 1, SW2 = kHid + 1, SW3 = kHid + 1;   // LDS row strides (floats)
-constexpr int SX = kIn + 1, SA = kHid + 1, SY = kOutMax + 1;
+constexpr int kTS = 17, kTBlock = 16 * kTS;                    // transpose block (16 x 17 floats)
+constexpr int kTBlocks = 8;                                    // transpose blocks per wave
+constexpr int kNGrad = kOutMax * kHid + kHid * kHid + kHid * kIn + kHid + kHid + kOutMax;
 
 constexpr float kSeluAlpha = 1.6732632423543772f;
 constexpr float kSeluScale = 1.0507009873554805f;
@@ -45,58 +56,72 @@ __device__ inline short to_bf16(float f) {  // round to nearest even
     return (short)(u >> 16);
 }
 
-// Fragment loader: element (row, col) of an LDS matrix with row stride s, scaled access pattern
-// chosen by the caller through a functor get(k, c) returning operand[k-index][c].
-template <int P>
-struct MM;
-
-template <>
-struct MM<0> {  // FP32 operands, K step 4
-    static constexpr int KS = 4;
-    template <typename GA, typename GB>
-    __device__ static f4 run(int K, GA ga, GB gb, f4 c) {
-        const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
-        for (int s = 0; s < K / 4; ++s) {
-            int k = 4 * s + q;
-            c = __builtin_amdgcn_mfma_f32_16x16x4f32(ga(j, k), gb(k, j), c, 0, 0, 0);
-        }
-        return c;
+__device__ inline f4 mfma_step(float a, float b, f4 d) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0); }
+__device__ inline f4 mfma_step(const float (&a)[4], const float (&b)[4], f4 d) {
+    s4 av, bv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        av[e] = to_bf16(a[e]);
+        bv[e] = to_bf16(b[e]);
     }
-};
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, d, 0, 0, 0);
+}
 
-template <>
-struct MM<1> {  // BF16 operands, K step 16
-    static constexpr int KS = 16;
-    template <typename GA, typename GB>
-    __device__ static f4 run(int K, GA ga, GB gb, f4 c) {
-        const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
-        for (int s = 0; s < K / 16; ++s) {
-            s4 a, b;
+// One output block of a transposed layer: D[unit 16*ot + 4q + r][point c] = sum_k A(16*ot + c, k)
+// act[k][point c], with act in the chain layout (lane (q, c) holds unit 16*kt + 4q + r of point c
+// in act[kt][r]) and wa(i, k) the weight of output row i (relative to the block) and input k.
+template <int P, int KT, typename WA>
+__device__ inline f4 chain(const f4 (&act)[KT], WA wa) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+        if constexpr (P == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d = mfma_step(wa(c, 16 * kt + 4 * q + r), act[kt][r], d);
+        } else {
+            float a[4], b[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int k = 16 * s + 4 * q + e;
-                a[e] = to_bf16(ga(j, k));
-                b[e] = to_bf16(gb(k, j));
+                a[e] = wa(c, 16 * kt + 4 * q + e);
+                b[e] = act[kt][e];
             }
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+            d = mfma_step(a, b, d);
         }
-        return c;
     }
-};
+    return d;
+}
+
+// D[unit 16*ot + 4q + r][unit' 16*kt + c] += sum over the tile's 16 points of a[.][p] b[.][p],
+// with a and b as point fragments: lane (q, c) holds unit c of its block at points 4q + s.
+template <int P>
+__device__ inline f4 outer(const float (&a)[4], const float (&b)[4], f4 d) {
+    if constexpr (P == 0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) d = mfma_step(a[s], b[s], d);
+    } else {
+        d = mfma_step(a, b, d);
+    }
+    return d;
+}
+
+// Chain layout -> point fragment of one 16-unit block through LDS block tb.
+__device__ inline void put_block(float* tb, f4 v) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tb[(4 * q + r) * kTS + c] = v[r];
+}
+__device__ inline void get_block(const float* tb, float (&out)[4]) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) out[s] = tb[c * kTS + 4 * q + s];
+}
 
 struct Smem {
     float w1[kHid * SW1];
     float w2[kHid * SW2];
     float w3[kOutMax * SW3];
     float b1[kHid], b2[kHid], b3[kOutMax];
-};
-
-struct WaveBuf {
-    float x[16 * SX];
-    float a1[16 * SA];
-    float a2[16 * SA];     // second hidden activations; reused for dH1 in the backward
-    float dh2[16 * SA];
-    float dy[16 * SY];
 };
 
 __device__ void load_weights(Smem& S, const float* w1, const float* b1, const float* w2, const float* b2,
@@ -114,55 +139,46 @@ __device__ void load_weights(Smem& S, const float* w1, const float* b1, const fl
     for (int i = threadIdx.x; i < kOutMax; i += blockDim.x) S.b3[i] = i < n_out ? b3[i] : 0.f;
 }
 
-// Stage the tile's inputs x[16, 32] (rows beyond n are zero).
-__device__ inline void load_x(WaveBuf& B, const float* __restrict__ x, int64_t row0, int64_t n) {
-    const int lane = threadIdx.x & 63;
-    for (int i = lane; i < 16 * kIn; i += 64) {
-        int r = i / kIn, k = i % kIn;
-        int64_t g = row0 + r;
-        B.x[r * SX + k] = g < n ? x[g * kIn + k] : 0.f;
+// Input rows of the tile in the chain layout: xin[kt][r] = x[point c][16 kt + 4q + r].
+__device__ inline void load_x_chain(const float* __restrict__ x, int64_t p, int64_t n, f4 (&xin)[2]) {
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+        if (p < n) {
+            const float4 v = *reinterpret_cast<const float4*>(x + p * kIn + 16 * kt + 4 * q);
+            xin[kt] = f4{v.x, v.y, v.z, v.w};
+        } else {
+            xin[kt] = f4{0.f, 0.f, 0.f, 0.f};
+        }
     }
 }
 
-// Forward of one 16-point tile. Leaves pre-activations h1, h2 (D layout) in registers and the
-// activations in B.a1 / B.a2; returns the output tile (D layout).
+// Forward of one tile: pre-activations h1, h2 (chain layout, biases added) and the output block.
 template <int P>
-__device__ inline f4 tile_forward(const Smem& S, WaveBuf& B, f4 h1[4], f4 h2[4]) {
-    const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
+__device__ inline f4 tile_forward(const Smem& S, const f4 (&xin)[2], f4 (&h1)[4], f4 (&h2)[4]) {
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+    f4 a1[4], a2[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        f4 c = {0.f, 0.f, 0.f, 0.f};
-        c = MM<P>::run(kIn, [&](int i, int k) { return B.x[i * SX + k]; },
-                       [&](int k, int jj) { return S.w1[(16 * nt + jj) * SW1 + k]; }, c);
-        float bias = S.b1[16 * nt + j];
+    for (int ot = 0; ot < 4; ++ot) {
+        f4 d = chain<P, 2>(xin, [&](int i, int k) { return S.w1[(16 * ot + i) * SW1 + k]; });
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            c[r] += bias;
-            B.a1[(4 * q + r) * SA + 16 * nt + j] = selu(c[r]);
+            h1[ot][r] = d[r] + S.b1[16 * ot + 4 * q + r];
+            a1[ot][r] = selu(h1[ot][r]);
         }
-        h1[nt] = c;
     }
-    wave_sync();
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        f4 c = {0.f, 0.f, 0.f, 0.f};
-        c = MM<P>::run(kHid, [&](int i, int k) { return B.a1[i * SA + k]; },
-                       [&](int k, int jj) { return S.w2[(16 * nt + jj) * SW2 + k]; }, c);
-        float bias = S.b2[16 * nt + j];
+    for (int ot = 0; ot < 4; ++ot) {
+        f4 d = chain<P, 4>(a1, [&](int i, int k) { return S.w2[(16 * ot + i) * SW2 + k]; });
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            c[r] += bias;
-            B.a2[(4 * q + r) * SA + 16 * nt + j] = selu(c[r]);
+            h2[ot][r] = d[r] + S.b2[16 * ot + 4 * q + r];
+            a2[ot][r] = selu(h2[ot][r]);
         }
-        h2[nt] = c;
     }
-    wave_sync();
-    f4 y = {0.f, 0.f, 0.f, 0.f};
-    y = MM<P>::run(kHid, [&](int i, int k) { return B.a2[i * SA + k]; },
-                   [&](int k, int jj) { return S.w3[jj * SW3 + k]; }, y);
-    float bias = S.b3[j];
+    f4 y = chain<P, 4>(a2, [&](int i, int k) { return S.w3[i * SW3 + k]; });
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] += bias;
+    for (int r = 0; r < 4; ++r) y[r] += S.b3[4 * q + r];
     return y;
 }
 
@@ -172,25 +188,23 @@ __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(const float* __restri
                                                            const float* b2, const float* w3, const float* b3,
                                                            int n_out, float* __restrict__ y) {
     __shared__ Smem S;
-    __shared__ WaveBuf WB[kWaves];
     load_weights(S, w1, b1, w2, b2, w3, b3, n_out);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
-    WaveBuf& B = WB[wave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
     const int64_t ntiles = (n + 15) / 16;
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
-        load_x(B, x, t * 16, n);
-        wave_sync();
-        f4 h1[4], h2[4];
-        f4 out = tile_forward<P>(S, B, h1, h2);
-        if (j < n_out) {
+        asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
+        const int64_t p = t * 16 + c;
+        f4 xin[2], h1[4], h2[4];
+        load_x_chain(x, p, n, xin);
+        const f4 out = tile_forward<P>(S, xin, h1, h2);
+        if (p < n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int64_t row = t * 16 + 4 * q + r;
-                if (row < n) y[row * n_out + j] = out[r];
+                const int o = 4 * q + r;
+                if (o < n_out) y[p * n_out + o] = out[r];
             }
         }
-        wave_sync();
     }
 }
 
@@ -200,17 +214,21 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
     const float* w3, const float* b3, int n_out, const float* __restrict__ gy, float* __restrict__ gx,
     float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3) {
     __shared__ Smem S;
-    __shared__ WaveBuf WB[kWaves];
+    __shared__ float TB[kWaves * kTBlocks * kTBlock];
+    static_assert(kWaves * kTBlocks * kTBlock >= kNGrad, "gradient sums reuse the transpose blocks");
     load_weights(S, w1, b1, w2, b2, w3, b3, n_out);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
-    WaveBuf& B = WB[wave];
-    // weight-gradient accumulators (D layout): gW3 [16 x 64], gW2 [64 x 64], gW1 [64 x 32]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+    float* tb = TB + wave * kTBlocks * kTBlock;
+    // weight-gradient tiles (D layout): gW3 [16 x 64], gW2 [64 x 64], gW1 [64 x 32]; bias partials
+    // per lane for the units it holds in the chain layout
     f4 g3[4], g2[4][4], g1[4][2];
-    float gb1p[4] = {0, 0, 0, 0}, gb2p[4] = {0, 0, 0, 0}, gb3p = 0.f;
+    f4 gb1p[4], gb2p[4], gb3p = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         g3[a] = f4{0.f, 0.f, 0.f, 0.f};
+        gb1p[a] = f4{0.f, 0.f, 0.f, 0.f};
+        gb2p[a] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int b = 0; b < 4; ++b) g2[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -218,117 +236,153 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
     }
     const int64_t ntiles = (n + 15) / 16;
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
-        load_x(B, x, t * 16, n);
-        for (int i = lane; i < 16 * kOutMax; i += 64) {
-            int r = i / kOutMax, o = i % kOutMax;
-            int64_t g = t * 16 + r;
-            B.dy[r * SY + o] = (g < n && o < n_out) ? gy[g * n_out + o] : 0.f;
-        }
-        wave_sync();
-        f4 h1[4], h2[4];
-        tile_forward<P>(S, B, h1, h2);
-        // dH2 = (dY W3) * selu'(H2)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f4 c = {0.f, 0.f, 0.f, 0.f};
-            c = MM<P>::run(kOutMax, [&](int i, int k) { return B.dy[i * SY + k]; },
-                           [&](int k, int jj) { return S.w3[k * SW3 + 16 * nt + jj]; }, c);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float d = c[r] * dselu(h2[nt][r]);
-                B.dh2[(4 * q + r) * SA + 16 * nt + j] = d;
-                gb2p[nt] += d;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) gb3p += B.dy[(4 * q + r) * SY + j];
-        wave_sync();
-        // gW3 += dY^T A2 (contract over the tile's points)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-            g3[nt] = MM<P>::run(16, [&](int i, int k) { return B.dy[k * SY + i]; },
-                                [&](int k, int jj) { return B.a2[k * SA + 16 * nt + jj]; }, g3[nt]);
-        wave_sync();
-        // dH1 = (dH2 W2) * selu'(H1), stored over A2
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f4 c = {0.f, 0.f, 0.f, 0.f};
-            c = MM<P>::run(kHid, [&](int i, int k) { return B.dh2[i * SA + k]; },
-                           [&](int k, int jj) { return S.w2[k * SW2 + 16 * nt + jj]; }, c);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float d = c[r] * dselu(h1[nt][r]);
-                B.a2[(4 * q + r) * SA + 16 * nt + j] = d;
-                gb1p[nt] += d;
-            }
-        }
-        // gW2 += dH2^T A1
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                g2[mt][nt] = MM<P>::run(16, [&](int i, int k) { return B.dh2[k * SA + 16 * mt + i]; },
-                                        [&](int k, int jj) { return B.a1[k * SA + 16 * nt + jj]; }, g2[mt][nt]);
-        wave_sync();
-        // dX = dH1 W1
-        if (gx) {
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                f4 c = {0.f, 0.f, 0.f, 0.f};
-                c = MM<P>::run(kHid, [&](int i, int k) { return B.a2[i * SA + k]; },
-                               [&](int k, int jj) { return S.w1[k * SW1 + 16 * nt + jj]; }, c);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int64_t row = t * 16 + 4 * q + r;
-                    if (row < n) gx[row * kIn + 16 * nt + j] = c[r];
-                }
-            }
-        }
-        // gW1 += dH1^T X
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
-                g1[mt][nt] = MM<P>::run(16, [&](int i, int k) { return B.a2[k * SA + 16 * mt + i]; },
-                                        [&](int k, int jj) { return B.x[k * SX + 16 * nt + jj]; }, g1[mt][nt]);
-        wave_sync();
-    }
-    // flush: D layout element (4q + r, j) of tile (mt, nt)
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
+        asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
+        const int64_t p = t * 16 + c;
+        f4 xin[2], h1[4], h2[4];
+        load_x_chain(x, p, n, xin);
+        tile_forward<P>(S, xin, h1, h2);
+        // dY in the chain layout (unit 4q + r of point c) and as a point fragment (unit c, points 4q + s)
+        f4 dyc[1];
+        float dyp[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            int o = 4 * q + r;
-            if (o < n_out && g3[nt][r] != 0.f) unsafeAtomicAdd(gw3 + o * kHid + 16 * nt + j, g3[nt][r]);
+            const int o = 4 * q + r;
+            dyc[0][r] = (p < n && o < n_out) ? gy[p * n_out + o] : 0.f;
+            const int64_t pp = t * 16 + 4 * q + r;
+            dyp[r] = (pp < n && c < n_out) ? gy[pp * n_out + c] : 0.f;
         }
+        gb3p += dyc[0];
+        // gW3 += dY A2^T
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+        for (int kt = 0; kt < 4; ++kt) {
+            f4 a2;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
+            for (int r = 0; r < 4; ++r) a2[r] = selu(h2[kt][r]);
+            put_block(tb + kt * kTBlock, a2);
+        }
+        wave_sync();
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                unsafeAtomicAdd(gw2 + (16 * mt + 4 * q + r) * kHid + 16 * nt + j, g2[mt][nt][r]);
+        for (int kt = 0; kt < 4; ++kt) {
+            float a2p[4];
+            get_block(tb + kt * kTBlock, a2p);
+            g3[kt] = outer<P>(dyp, a2p, g3[kt]);
+        }
+        // dH2 = (W3^T dY) * selu'(H2)
+        f4 dh2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+        for (int ot = 0; ot < 4; ++ot) {
+            f4 d = chain<P, 1>(dyc, [&](int i, int k) { return S.w3[k * SW3 + 16 * ot + i]; });
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
+            for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * dselu(h2[ot][r]);
+            gb2p[ot] += dh2[ot];
+        }
+        wave_sync();
+        // gW2 += dH2 A1^T
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                unsafeAtomicAdd(gw1 + (16 * mt + 4 * q + r) * kIn + 16 * nt + j, g1[mt][nt][r]);
-    // biases: lanes j, j+16, j+32, j+48 hold partial sums of column j
+        for (int b = 0; b < 4; ++b) {
+            f4 a1;
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        float v1 = gb1p[nt], v2 = gb2p[nt];
-        v1 += __shfl_xor(v1, 16, 64); v1 += __shfl_xor(v1, 32, 64);
-        v2 += __shfl_xor(v2, 16, 64); v2 += __shfl_xor(v2, 32, 64);
-        if (q == 0) {
-            unsafeAtomicAdd(gb1 + 16 * nt + j, v1);
-            unsafeAtomicAdd(gb2 + 16 * nt + j, v2);
+            for (int r = 0; r < 4; ++r) a1[r] = selu(h1[b][r]);
+            put_block(tb + b * kTBlock, dh2[b]);
+            put_block(tb + (4 + b) * kTBlock, a1);
+        }
+        wave_sync();
+        {
+            float dh2p[4][4], a1p[4][4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                get_block(tb + b * kTBlock, dh2p[b]);
+                get_block(tb + (4 + b) * kTBlock, a1p[b]);
+            }
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt) g2[ot][kt] = outer<P>(dh2p[ot], a1p[kt], g2[ot][kt]);
+        }
+        // dH1 = (W2^T dH2) * selu'(H1)
+        f4 dh1[4];
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) {
+            f4 d = chain<P, 4>(dh2, [&](int i, int k) { return S.w2[k * SW2 + 16 * ot + i]; });
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * dselu(h1[ot][r]);
+            gb1p[ot] += dh1[ot];
+        }
+        // dX = W1^T dH1
+        if (gx) {
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) {
+                f4 d = chain<P, 4>(dh1, [&](int i, int k) { return S.w1[k * SW1 + 16 * ot + i]; });
+                if (p < n) *reinterpret_cast<float4*>(gx + p * kIn + 16 * ot + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
+            }
+        }
+        wave_sync();
+        // gW1 += dH1 X^T
+#pragma unroll
+        for (int b = 0; b < 4; ++b) put_block(tb + b * kTBlock, dh1[b]);
+        wave_sync();
+        {
+            float xp[2][4];
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int64_t pp = t * 16 + 4 * q + s;
+                    xp[kt][s] = pp < n ? x[pp * kIn + 16 * kt + c] : 0.f;
+                }
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot) {
+                float dh1p[4];
+                get_block(tb + ot * kTBlock, dh1p);
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt) g1[ot][kt] = outer<P>(dh1p, xp[kt], g1[ot][kt]);
+            }
+        }
+        wave_sync();
+    }
+    // sum the waves' gradients in LDS (over the transpose blocks), then one global atomic per element
+    __syncthreads();
+    float* G3 = TB;                       // [16][64]
+    float* G2 = G3 + kOutMax * kHid;      // [64][64]
+    float* G1 = G2 + kHid * kHid;         // [64][32]
+    float* B1 = G1 + kHid * kIn;
+    float* B2 = B1 + kHid;
+    float* B3 = B2 + kHid;
+    for (int i = threadIdx.x; i < kNGrad; i += blockDim.x) TB[i] = 0.f;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int o = 4 * q + r;
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) atomicAdd(&G3[o * kHid + 16 * kt + c], g3[kt][r]);
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) {
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) atomicAdd(&G2[(16 * ot + o) * kHid + 16 * kt + c], g2[ot][kt][r]);
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt) atomicAdd(&G1[(16 * ot + o) * kIn + 16 * kt + c], g1[ot][kt][r]);
+            atomicAdd(&B1[16 * ot + o], gb1p[ot][r]);
+            atomicAdd(&B2[16 * ot + o], gb2p[ot][r]);
+        }
+        atomicAdd(&B3[o], gb3p[r]);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNGrad; i += blockDim.x) {
+        const float v = TB[i];
+        if (v == 0.f) continue;
+        if (i < kOutMax * kHid) {
+            if (i / kHid < n_out) unsafeAtomicAdd(gw3 + i, v);
+        } else if (i < kOutMax * kHid + kHid * kHid) {
+            unsafeAtomicAdd(gw2 + (i - kOutMax * kHid), v);
+        } else if (i < kOutMax * kHid + kHid * kHid + kHid * kIn) {
+            unsafeAtomicAdd(gw1 + (i - kOutMax * kHid - kHid * kHid), v);
+        } else {
+            const int b = i - (kOutMax * kHid + kHid * kHid + kHid * kIn);
+            if (b < kHid) unsafeAtomicAdd(gb1 + b, v);
+            else if (b < 2 * kHid) unsafeAtomicAdd(gb2 + (b - kHid), v);
+            else if (b - 2 * kHid < n_out) unsafeAtomicAdd(gb3 + (b - 2 * kHid), v);
         }
     }
-    gb3p += __shfl_xor(gb3p, 16, 64);
-    gb3p += __shfl_xor(gb3p, 32, 64);
-    if (q == 0 && j < n_out) unsafeAtomicAdd(gb3 + j, gb3p);
 }
 
 inline unsigned grid_for(int64_t n, int cap) {
@@ -350,7 +404,8 @@ extern "C" int rm_mlp_fwd(const float* x_dev, int64_t n, const float* w1, const 
     RM_CHECK_ARG(precision == 0 || precision == 1, "rm_mlp_fwd: precision must be 0 (fp32) or 1 (bf16)");
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && w1 && b1 && w2 && b2 && w3 && b3 && y_dev, "rm_mlp_fwd: null buffer");
-    dim3 grid(grid_for(n, 4096)), block(kThreads);
+    RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0, "rm_mlp_fwd: x must be 16-byte aligned");
+    dim3 grid(grid_for(n, 1280)), block(kThreads);
     if (precision == 0)
         hipLaunchKernelGGL(mlp_fwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
                            n_out, y_dev);
@@ -371,7 +426,9 @@ extern "C" int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const 
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && w1 && b1 && w2 && b2 && w3 && b3 && grad_y_dev && gw1 && gb1 && gw2 && gb2 && gw3 && gb3,
                  "rm_mlp_bwd: null buffer");
-    dim3 grid(grid_for(n, 1024)), block(kThreads);
+    RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0 && ((uintptr_t)grad_x_dev & 15) == 0,
+                 "rm_mlp_bwd: x and grad_x must be 16-byte aligned");
+    dim3 grid(grid_for(n, 512)), block(kThreads);
     if (precision == 0)
         hipLaunchKernelGGL(mlp_bwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
                            n_out, grad_y_dev, grad_x_dev, gw1, gb1, gw2, gb2, gw3, gb3);
