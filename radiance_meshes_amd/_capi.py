@@ -68,6 +68,8 @@ SIGNATURES = {
     "rm_batch_render": (C.c_int, [_P, C.POINTER(RmBatchInfo), _I64, _P, _P, _P, _P, _P, _SZ, _P]),
     "rm_hashgrid_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P]),
     "rm_hashgrid_bwd": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P]),
+    "rm_hashgrid_bwd_workspace_size": (_SZ, [_I64, _P, _P, _I32, _I32, _I32]),
+    "rm_hashgrid_bwd_ws": (C.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P, _SZ, _P]),
     "rm_mlp_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "rm_mlp_bwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P,
                              _P, _P]),

@@ -166,7 +166,7 @@ constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;   // table slots are < 2^31
 template <int F>
 __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const float* __restrict__ x, int64_t N,
                                                                        const float* __restrict__ gout, GridArgs g,
-                                                                       float* __restrict__ gtable) {
+                                                                       int l_end, float* __restrict__ gtable) {
     static_assert(F == 1 || F == 2, "LDS table sized for n_feat <= 2");
     __shared__ uint32_t s_key[kAggSlots];
     __shared__ float s_val[kAggSlots * F];
@@ -182,42 +182,63 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
     for (int p = tid; p < kAggSlots; p += kAggPoints) s_key[p] = kEmptySlot;
     for (int p = tid; p < kAggSlots * F; p += kAggPoints) s_val[p] = 0.f;
     __syncthreads();
-    for (int l = 0; l < g.L; ++l) {
-        if (valid) {
-            float go[F];
+    const int lane = tid & 63;
+    for (int l = 0; l < l_end; ++l) {
+        float go[F];
+        int bx = 0, by = 0, bz = 0;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
-            for (int f = 0; f < F; ++f) go[f] = gout[(n * g.L + l) * F + f];
-            int bx, by, bz;
-            float fx, fy, fz;
-            cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+        for (int f = 0; f < F; ++f) go[f] = valid ? gout[(n * g.L + l) * F + f] : 0.f;
+        if (valid) cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
-                const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-                float v[F];
-                bool nz = false;
+        for (int c = 0; c < 8; ++c) {
+            const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+            const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+            const uint32_t h = valid ? corner_index(g, l, bx + ox, by + oy, bz + oz) : kEmptySlot;
+            float v[F];
 #pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    v[f] = w * go[f];
-                    nz = nz || v[f] != 0.f;
-                }
-                if (!nz) continue;
-                const uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
-                uint32_t p = (h * 2654435761u) >> (32 - kAggLog2Slots);
-                bool placed = false;
-                for (int probe = 0; probe < kAggSlots && !placed; ++probe) {
-                    const uint32_t k = atomicCAS(&s_key[p], kEmptySlot, h);
-                    if (k == kEmptySlot || k == h) {
+            for (int f = 0; f < F; ++f) v[f] = w * go[f];
+            // consecutive lanes are neighbouring points: at coarse levels runs of lanes share a
+            // slot, so a wave with at most 32 runs first sums each run (segmented scan) and only
+            // the run's last lane inserts
+            const uint32_t hprev = __shfl_up(h, 1, 64);
+            const bool head = lane == 0 || h != hprev;
+            const uint64_t heads = __ballot(head);
+            bool tail = true;
+            if (__popcll(heads) <= 32) {
+                bool seg = head;
 #pragma unroll
-                        for (int f = 0; f < F; ++f) atomicAdd(&s_val[p * F + f], v[f]);
-                        placed = true;
+                for (int o = 1; o < 64; o <<= 1) {
+                    float u[F];
+#pragma unroll
+                    for (int f = 0; f < F; ++f) u[f] = __shfl_up(v[f], o, 64);
+                    const bool su = __shfl_up((int)seg, o, 64);
+                    if (lane >= o && !seg) {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) v[f] += u[f];
                     }
-                    p = (p + 1) & (kAggSlots - 1);
+                    if (lane >= o) seg = seg || su;
                 }
-                if (!placed)  // unreachable while the table holds more slots than a level can insert
-#pragma unroll
-                    for (int f = 0; f < F; ++f) unsafeAtomicAdd(gtable + (g.offs[l] + (int64_t)h) * F + f, v[f]);
+                tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
             }
+            bool nz = false;
+#pragma unroll
+            for (int f = 0; f < F; ++f) nz = nz || v[f] != 0.f;
+            if (!valid || !tail || !nz) continue;
+            uint32_t p = (h * 2654435761u) >> (32 - kAggLog2Slots);
+            bool placed = false;
+            for (int probe = 0; probe < kAggSlots && !placed; ++probe) {
+                const uint32_t k = atomicCAS(&s_key[p], kEmptySlot, h);
+                if (k == kEmptySlot || k == h) {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) atomicAdd(&s_val[p * F + f], v[f]);
+                    placed = true;
+                }
+                p = (p + 1) & (kAggSlots - 1);
+            }
+            if (!placed)  // unreachable while the table holds more slots than a level can insert
+#pragma unroll
+                for (int f = 0; f < F; ++f) unsafeAtomicAdd(gtable + (g.offs[l] + (int64_t)h) * F + f, v[f]);
         }
         __syncthreads();
         float* tb = gtable + g.offs[l] * F;
@@ -234,6 +255,181 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
         }
         __syncthreads();
     }
+}
+
+// Backward of the fine hashed levels by slot ownership (n_feat 2). Past a few hundred cells per
+// axis nearly every corner update of a 256-point chunk hits its own table slot, so the
+// aggregation above issues one global atomic per update. Instead the updates of levels
+// [lp, L) are sorted by bucket (2048 consecutive slots of one level) with a counting sort, and one
+// workgroup per bucket sums its updates in LDS and adds them to the table with plain stores:
+//  1. part_count: per 4096-point chunk, the number of updates per bucket (LDS histogram),
+//  2. part_offsets: exclusive offsets per (chunk, bucket), bucket-major,
+//  3. part_scatter: the same walk as (1), each update stored at its bucket's next position as
+//     (slot within the bucket, value pair),
+//  4. part_accumulate: per bucket, the sum of its updates added to the table.
+constexpr int kPartPoints = 4096;              // query points per chunk (16 per lane)
+constexpr int kPartLog2Slots = 11;
+constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket
+constexpr int kPartMaxBuckets = 8192;
+
+__global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
+                                                         int lp, int nb_level, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_hist[kPartMaxBuckets];
+    const int nb = (g.L - lp) * nb_level;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s_hist[b] = 0;
+    __syncthreads();
+    for (int i = 0; i < kPartPoints / 256; ++i) {
+        const int64_t n = (int64_t)blockIdx.x * kPartPoints + i * 256 + threadIdx.x;
+        if (n >= N) break;
+        float xv[3];
+        load_point(x, n, xv);
+        for (int l = lp; l < g.L; ++l) {
+            int bx, by, bz;
+            float fx, fy, fz;
+            cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t h = corner_index(g, l, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
+                atomicAdd(&s_hist[(l - lp) * nb_level + (int)(h >> kPartLog2Slots)], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)blockIdx.x * nb + b] = s_hist[b];
+}
+
+// hist[chunk][bucket] -> exclusive offset of (chunk, bucket) in bucket-major order; bucket_start
+// gets each bucket's first position and bucket_start[nb] the total.
+__global__ __launch_bounds__(256) void part_column_kernel(uint32_t* __restrict__ hist, int64_t n_chunks, int nb,
+                                                          uint32_t* __restrict__ bucket_total) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    uint32_t run = 0;
+    for (int64_t k = 0; k < n_chunks; ++k) {
+        const uint32_t v = hist[k * nb + b];
+        hist[k * nb + b] = run;
+        run += v;
+    }
+    bucket_total[b] = run;
+}
+
+__global__ __launch_bounds__(256) void part_bucket_scan_kernel(const uint32_t* __restrict__ bucket_total, int nb,
+                                                               uint32_t* __restrict__ bucket_start) {
+    __shared__ uint32_t s_part[256];
+    // each lane owns a contiguous run of ceil(nb / 256) buckets
+    const int per = (nb + 255) / 256;
+    const int b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int b = b0; b < b0 + per && b < nb; ++b) sum += bucket_total[b];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int t = 0; t < 256; ++t) {
+            const uint32_t v = s_part[t];
+            s_part[t] = run;
+            run += v;
+        }
+        bucket_start[nb] = run;
+    }
+    __syncthreads();
+    uint32_t run = s_part[threadIdx.x];
+    for (int b = b0; b < b0 + per && b < nb; ++b) {
+        bucket_start[b] = run;
+        run += bucket_total[b];
+    }
+}
+
+__global__ __launch_bounds__(256) void part_scatter_kernel(const float* __restrict__ x, int64_t N,
+                                                           const float* __restrict__ gout, GridArgs g, int lp,
+                                                           int nb_level, const uint32_t* __restrict__ hist,
+                                                           const uint32_t* __restrict__ bucket_start,
+                                                           uint16_t* __restrict__ rslot, float2* __restrict__ rval) {
+    __shared__ uint32_t s_cur[kPartMaxBuckets];
+    const int nb = (g.L - lp) * nb_level;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        s_cur[b] = bucket_start[b] + hist[(int64_t)blockIdx.x * nb + b];
+    __syncthreads();
+    for (int i = 0; i < kPartPoints / 256; ++i) {
+        const int64_t n = (int64_t)blockIdx.x * kPartPoints + i * 256 + threadIdx.x;
+        if (n >= N) break;
+        float xv[3];
+        load_point(x, n, xv);
+        for (int l = lp; l < g.L; ++l) {
+            const float2 go = *reinterpret_cast<const float2*>(gout + (n * g.L + l) * 2);
+            int bx, by, bz;
+            float fx, fy, fz;
+            cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+                const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+                const uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
+                const uint32_t pos = atomicAdd(&s_cur[(l - lp) * nb_level + (int)(h >> kPartLog2Slots)], 1u);
+                rslot[pos] = (uint16_t)(h & (kPartSlots - 1));
+                rval[pos] = make_float2(w * go.x, w * go.y);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void part_accumulate_kernel(const uint32_t* __restrict__ bucket_start,
+                                                              const uint16_t* __restrict__ rslot,
+                                                              const float2* __restrict__ rval, GridArgs g, int lp,
+                                                              int nb_level, float* __restrict__ gtable) {
+    __shared__ float s_acc[kPartSlots * 2];
+    for (int s = threadIdx.x; s < kPartSlots * 2; s += blockDim.x) s_acc[s] = 0.f;
+    __syncthreads();
+    const int b = blockIdx.x;
+    const uint32_t lo = bucket_start[b], hi = bucket_start[b + 1];
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+        const int s = rslot[p];
+        const float2 v = rval[p];
+        atomicAdd(&s_acc[2 * s], v.x);
+        atomicAdd(&s_acc[2 * s + 1], v.y);
+    }
+    __syncthreads();
+    const int l = lp + b / nb_level;
+    float* tb = gtable + (g.offs[l] + (int64_t)(b % nb_level) * kPartSlots) * 2;
+    for (int s = threadIdx.x; s < kPartSlots * 2; s += blockDim.x) {
+        const float v = s_acc[s];
+        if (v != 0.f) tb[s] += v;   // this workgroup owns these slots
+    }
+}
+
+// First level of the partitioned suffix, or L when the partitioned backward does not apply:
+// n_feat 2, hashed levels with res >= 256 (a suffix, as resolutions grow with the level), whole
+// buckets per level, and few enough buckets for the LDS histograms.
+int partition_first_level(const GridArgs& g, int log2_size) {
+    if (g.F != 2 || log2_size < kPartLog2Slots) return g.L;
+    int lp = g.L;
+    while (lp > 0 && !g.dense[lp - 1] && g.res[lp - 1] >= 256) --lp;
+    const int64_t nb = (int64_t)(g.L - lp) << (log2_size - kPartLog2Slots);
+    if (lp == g.L || nb > kPartMaxBuckets) return g.L;
+    return lp;
+}
+
+struct PartLayout {
+    size_t hist, total, start, rslot, rval, bytes;
+    int64_t n_chunks, n_records;
+    int nb_level, nb;
+};
+
+PartLayout part_layout(const GridArgs& g, int log2_size, int lp, int64_t n) {
+    PartLayout P;
+    auto a256 = [](size_t v) { return (v + 255) & ~size_t(255); };
+    P.nb_level = 1 << (log2_size - kPartLog2Slots);
+    P.nb = (g.L - lp) * P.nb_level;
+    P.n_chunks = (n + kPartPoints - 1) / kPartPoints;
+    P.n_records = n * 8 * (int64_t)(g.L - lp);
+    size_t o = 0;
+    P.hist = o; o += a256(sizeof(uint32_t) * (size_t)P.n_chunks * P.nb);
+    P.total = o; o += a256(sizeof(uint32_t) * (size_t)P.nb);
+    P.start = o; o += a256(sizeof(uint32_t) * (size_t)(P.nb + 1));
+    P.rslot = o; o += a256(sizeof(uint16_t) * (size_t)P.n_records);
+    P.rval = o; o += a256(sizeof(float2) * (size_t)P.n_records);
+    P.bytes = o;
+    return P;
 }
 
 int make_grid_args(const int32_t* res, const int64_t* offs, int32_t L, int32_t F, int32_t log2_size, GridArgs& g,
@@ -258,11 +454,6 @@ int make_grid_args(const int32_t* res, const int64_t* offs, int32_t L, int32_t F
     return RM_OK;
 }
 
-}  // namespace
-}  // namespace rmesh
-
-using namespace rmesh;
-
 #define RM_FEAT_DISPATCH(F_, KERNEL, ...)                                                     \
     switch (F_) {                                                                             \
         case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                            \
@@ -270,6 +461,29 @@ using namespace rmesh;
         case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                            \
         default: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                           \
     }
+
+// Backward of levels [0, l_end): workgroup aggregation for n_feat <= 2, else level-major.
+int hashgrid_bwd_levels(const float* x, int64_t n, const float* gout, const GridArgs& g, int l_end, float* gtable,
+                        hipStream_t st) {
+    if (g.F <= 2) {
+        dim3 grid((unsigned)ceil_div(n, kAggPoints)), block(kAggPoints);
+        if (g.F == 1)
+            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<1>, grid, block, 0, st, x, n, gout, g, l_end, gtable);
+        else
+            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<2>, grid, block, 0, st, x, n, gout, g, l_end, gtable);
+        RM_CHECK_LAUNCH("hashgrid_bwd_agg_kernel");
+        return RM_OK;
+    }
+    dim3 grid((unsigned)ceil_div(n, 256), (unsigned)l_end), block(256);  // level = blockIdx.y < l_end
+    RM_FEAT_DISPATCH(g.F, hashgrid_bwd_kernel, grid, block, 0, st, x, n, gout, g, gtable);
+    RM_CHECK_LAUNCH("hashgrid_bwd_kernel");
+    return RM_OK;
+}
+
+}  // namespace
+}  // namespace rmesh
+
+using namespace rmesh;
 
 extern "C" int rm_hashgrid_fwd(const float* x_dev, int64_t n, const float* table_dev, const int32_t* res,
                                const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
@@ -296,21 +510,61 @@ extern "C" int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_
     if (e) return e;
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && grad_out_dev && grad_table_dev, "rm_hashgrid_bwd: null buffer");
-    if (n_feat <= 2) {
-        dim3 grid((unsigned)ceil_div(n, kAggPoints)), block(kAggPoints);
-        if (n_feat == 1)
-            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<1>, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
-                               grad_table_dev);
-        else
-            hipLaunchKernelGGL(hashgrid_bwd_agg_kernel<2>, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
-                               grad_table_dev);
-        RM_CHECK_LAUNCH("hashgrid_bwd_agg_kernel");
-        return RM_OK;
+    return hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, n_levels, grad_table_dev, as_stream(stream));
+}
+
+extern "C" size_t rm_hashgrid_bwd_workspace_size(int64_t n, const int32_t* res, const int64_t* offs, int32_t n_levels,
+                                                 int32_t n_feat, int32_t log2_size) {
+    GridArgs g;
+    if (n <= 0 || make_grid_args(res, offs, n_levels, n_feat, log2_size, g, "rm_hashgrid_bwd_workspace_size"))
+        return 0;
+    const int lp = partition_first_level(g, log2_size);
+    if (lp == g.L) return 0;
+    const PartLayout P = part_layout(g, log2_size, lp, n);
+    return P.n_records < (1ll << 31) ? P.bytes : 0;
+}
+
+extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* grad_out_dev, const int32_t* res,
+                                  const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
+                                  float* grad_table_dev, void* workspace_dev, size_t workspace_bytes,
+                                  rm_stream_t stream) {
+    RM_CHECK_ARG(n >= 0, "rm_hashgrid_bwd_ws: negative n");
+    GridArgs g;
+    int e = make_grid_args(res, offs, n_levels, n_feat, log2_size, g, "rm_hashgrid_bwd_ws");
+    if (e) return e;
+    if (n == 0) return RM_OK;
+    RM_CHECK_ARG(x_dev && grad_out_dev && grad_table_dev, "rm_hashgrid_bwd_ws: null buffer");
+    const size_t need = rm_hashgrid_bwd_workspace_size(n, res, offs, n_levels, n_feat, log2_size);
+    hipStream_t st = as_stream(stream);
+    if (need == 0) return hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, n_levels, grad_table_dev, st);
+    if (!workspace_dev || workspace_bytes < need) {
+        set_error("rm_hashgrid_bwd_ws: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
     }
-    dim3 grid((unsigned)ceil_div(n, 256), (unsigned)n_levels), block(256);
-    RM_FEAT_DISPATCH(n_feat, hashgrid_bwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, grad_out_dev, g,
-                     grad_table_dev);
-    RM_CHECK_LAUNCH("hashgrid_bwd_kernel");
+    RM_CHECK_ARG(((uintptr_t)workspace_dev & 255) == 0 && ((uintptr_t)grad_out_dev & 7) == 0,
+                 "rm_hashgrid_bwd_ws: misaligned workspace or grad_out");
+    const int lp = partition_first_level(g, log2_size);
+    const PartLayout P = part_layout(g, log2_size, lp, n);
+    char* w = static_cast<char*>(workspace_dev);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w + P.hist);
+    uint32_t* total = reinterpret_cast<uint32_t*>(w + P.total);
+    uint32_t* start = reinterpret_cast<uint32_t*>(w + P.start);
+    uint16_t* rslot = reinterpret_cast<uint16_t*>(w + P.rslot);
+    float2* rval = reinterpret_cast<float2*>(w + P.rval);
+    // coarse levels: workgroup aggregation
+    if (lp > 0) {
+        int r = hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, lp, grad_table_dev, st);
+        if (r) return r;
+    }
+    hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, g, lp, P.nb_level, hist);
+    hipLaunchKernelGGL(part_column_kernel, dim3((unsigned)ceil_div(P.nb, 256)), dim3(256), 0, st, hist, P.n_chunks, P.nb,
+                       total);
+    hipLaunchKernelGGL(part_bucket_scan_kernel, dim3(1), dim3(256), 0, st, (const uint32_t*)total, P.nb, start);
+    hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, grad_out_dev, g, lp,
+                       P.nb_level, (const uint32_t*)hist, (const uint32_t*)start, rslot, rval);
+    hipLaunchKernelGGL(part_accumulate_kernel, dim3((unsigned)P.nb), dim3(256), 0, st, (const uint32_t*)start,
+                       (const uint16_t*)rslot, (const float2*)rval, g, lp, P.nb_level, grad_table_dev);
+    RM_CHECK_LAUNCH("rm_hashgrid_bwd_ws");
     return RM_OK;
 }
 

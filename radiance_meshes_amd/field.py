@@ -46,10 +46,14 @@ class _HashEncode(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         res, offs, n_levels, n_feat, log2_size, tshape = ctx.meta
         gtable = torch.zeros(tshape, device=x.device, dtype=torch.float32)
+        L = lib()
+        n = x.shape[0]
+        wsb = L.rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, n_levels, n_feat, log2_size)
+        ws = torch.empty((max(wsb, 1),), device=x.device, dtype=torch.uint8)
         with timing.span("hashgrid_bwd"):
-            check(lib().rm_hashgrid_bwd(ptr(x), x.shape[0], ptr(gout.contiguous()), res.ctypes.data, offs.ctypes.data,
-                                    n_levels, n_feat, log2_size, ptr(gtable), stream_of(x.device)),
-              "rm_hashgrid_bwd")
+            check(L.rm_hashgrid_bwd_ws(ptr(x), n, ptr(gout.contiguous()), res.ctypes.data, offs.ctypes.data,
+                                       n_levels, n_feat, log2_size, ptr(gtable), ptr(ws), wsb, stream_of(x.device)),
+                  "rm_hashgrid_bwd_ws")
         return None, gtable, None, None, None, None, None
 
 

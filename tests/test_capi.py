@@ -50,3 +50,17 @@ def test_argument_errors_without_device(lib):
     assert lib.rm_sort_pairs_workspace_size(1000) > 1000 * 12
     assert lib.rm_scan_workspace_size(1 << 20) > 0
     assert lib.rm_bin_workspace_size(1000, 5000, 64) > 5000 * 16
+
+
+def test_hashgrid_workspace_rule(lib):
+    """Slot-ownership backward applies to the hashed levels with res >= 256 (n_feat 2 only)."""
+    from radiance_meshes_amd.field import level_table
+    res, offs = level_table(16, 16, 1.38, 19)          # levels 9..15 (res 290..2005) qualify
+    n = 1_000_000
+    ws = lib.rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, 16, 2, 19)
+    n_part = sum(1 for r in res if int(r) ** 3 > 2 ** 19 and r >= 256)
+    assert n_part == 7
+    assert ws >= n * 8 * n_part * 10                  # (slot, value pair) per update
+    assert lib.rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, 16, 4, 19) == 0
+    res8, offs8 = level_table(8, 16, 1.38, 19)         # finest resolution 106: no level qualifies
+    assert lib.rm_hashgrid_bwd_workspace_size(n, res8.ctypes.data, offs8.ctypes.data, 8, 2, 19) == 0

@@ -167,3 +167,29 @@ def test_field_scatter_matches_torch(nf):
     g_rel = ((raw.grad - raw_ref.grad).abs() / raw_ref.grad.abs().clamp_min(1.0)).max().item()
     print(f"nf={nf}: max rel diff backward = {g_rel:.3e}")
     assert g_rel <= 1e-6
+
+
+def test_hashgrid_backward_workspace_path_matches_atomic_path():
+    """The slot-ownership backward of the fine hashed levels and the atomic backward of every level
+    give the same table gradient (up to float summation order) on the benchmark's level table."""
+    import torch
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    x, table, res, offs, L, F, log2 = _grid(seed=8, n=50000)
+    g = np.random.default_rng(9).standard_normal((len(x), L * F)).astype(np.float32)
+    xd, gd = torch.from_numpy(x).cuda(), torch.from_numpy(g).cuda()
+    n = len(x)
+    wsb = lib().rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, L, F, log2)
+    print(f"workspace {wsb} bytes, partitioned levels: {[int(r) for r in res if int(r) ** 3 > 2 ** log2 and r >= 256]}",
+          flush=True)
+    assert wsb > 0
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    a = torch.zeros(table.shape, device="cuda")
+    b = torch.zeros(table.shape, device="cuda")
+    check(lib().rm_hashgrid_bwd_ws(ptr(xd), n, ptr(gd), res.ctypes.data, offs.ctypes.data, L, F, log2, ptr(a), ptr(ws),
+                                   wsb, stream_of()), "rm_hashgrid_bwd_ws")
+    check(lib().rm_hashgrid_bwd(ptr(xd), n, ptr(gd), res.ctypes.data, offs.ctypes.data, L, F, log2, ptr(b),
+                                stream_of()), "rm_hashgrid_bwd")
+    torch.cuda.synchronize()
+    rel = ((a - b).norm() / b.norm()).item()
+    print(f"rel L2 difference = {rel:.3e}")
+    assert rel <= 1e-6
