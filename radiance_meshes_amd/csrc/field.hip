@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
 // Backward of the fine hashed levels by slot ownership (n_feat 2). Past a few hundred cells per
 // axis nearly every corner update of a 256-point chunk hits its own table slot, so the
 // aggregation above issues one global atomic per update. Instead the updates of levels
-// [lp, L) are sorted by bucket (2048 consecutive slots of one level) with a counting sort, and one
+// [lp, L) are sorted by bucket (8192 consecutive slots of one level) with a counting sort, and one
 // workgroup per bucket sums its updates in LDS and adds them to the table with plain stores:
 //  1. part_count: per 4096-point chunk, the number of updates per bucket (LDS histogram),
 //  2. part_offsets: exclusive offsets per (chunk, bucket), bucket-major,
@@ -268,8 +268,8 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
 //     (slot within the bucket, value pair),
 //  4. part_accumulate: per bucket, the sum of its updates added to the table.
 constexpr int kPartPoints = 4096;              // query points per chunk (16 per lane)
-constexpr int kPartLog2Slots = 11;
-constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket
+constexpr int kPartLog2Slots = 13;
+constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket (64 KB of LDS sums)
 constexpr int kPartMaxBuckets = 8192;
 
 __global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
@@ -295,22 +295,38 @@ __global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict
         }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)blockIdx.x * nb + b] = s_hist[b];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)b * gridDim.x + blockIdx.x] = s_hist[b];
 }
 
-// hist[chunk][bucket] -> exclusive offset of (chunk, bucket) in bucket-major order; bucket_start
-// gets each bucket's first position and bucket_start[nb] the total.
-__global__ __launch_bounds__(256) void part_column_kernel(uint32_t* __restrict__ hist, int64_t n_chunks, int nb,
+// hist[bucket][chunk] -> exclusive offset of each chunk's updates within the bucket (one workgroup
+// per bucket); bucket_total gets each bucket's update count.
+__global__ __launch_bounds__(256) void part_column_kernel(uint32_t* __restrict__ hist, int64_t n_chunks,
                                                           uint32_t* __restrict__ bucket_total) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    uint32_t run = 0;
-    for (int64_t k = 0; k < n_chunks; ++k) {
-        const uint32_t v = hist[k * nb + b];
-        hist[k * nb + b] = run;
-        run += v;
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_carry;
+    uint32_t* h = hist + (int64_t)blockIdx.x * n_chunks;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < n_chunks; c0 += 256) {
+        const int64_t k = c0 + threadIdx.x;
+        const uint32_t v = k < n_chunks ? h[k] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) s_wave[w] = incl;
+        __syncthreads();
+        uint32_t pre = s_carry;
+        for (int i = 0; i < w; ++i) pre += s_wave[i];
+        if (k < n_chunks) h[k] = pre + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 255) s_carry = pre + incl;
+        __syncthreads();
     }
-    bucket_total[b] = run;
+    if (threadIdx.x == 0) bucket_total[blockIdx.x] = s_carry;
 }
 
 __global__ __launch_bounds__(256) void part_bucket_scan_kernel(const uint32_t* __restrict__ bucket_total, int nb,
@@ -348,7 +364,7 @@ __global__ __launch_bounds__(256) void part_scatter_kernel(const float* __restri
     __shared__ uint32_t s_cur[kPartMaxBuckets];
     const int nb = (g.L - lp) * nb_level;
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        s_cur[b] = bucket_start[b] + hist[(int64_t)blockIdx.x * nb + b];
+        s_cur[b] = bucket_start[b] + hist[(int64_t)b * gridDim.x + blockIdx.x];
     __syncthreads();
     for (int i = 0; i < kPartPoints / 256; ++i) {
         const int64_t n = (int64_t)blockIdx.x * kPartPoints + i * 256 + threadIdx.x;
@@ -377,23 +393,58 @@ __global__ __launch_bounds__(256) void part_accumulate_kernel(const uint32_t* __
                                                               const uint16_t* __restrict__ rslot,
                                                               const float2* __restrict__ rval, GridArgs g, int lp,
                                                               int nb_level, float* __restrict__ gtable) {
-    __shared__ float s_acc[kPartSlots * 2];
-    for (int s = threadIdx.x; s < kPartSlots * 2; s += blockDim.x) s_acc[s] = 0.f;
+    // the two features in separate arrays: a wave's random slots then spread over all 64 banks
+    __shared__ float s_ax[kPartSlots], s_ay[kPartSlots];
+    for (int s = threadIdx.x; s < kPartSlots; s += blockDim.x) {
+        s_ax[s] = 0.f;
+        s_ay[s] = 0.f;
+    }
     __syncthreads();
     const int b = blockIdx.x;
     const uint32_t lo = bucket_start[b], hi = bucket_start[b + 1];
-    for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+    // records [lo, a0) and [a1, hi) one per lane; [a0, a1) eight per lane with 16-byte loads (lane
+    // t of a round reads records 8t..8t+7 of it, so a wave reads one contiguous stretch)
+    uint32_t a0 = (lo + 7u) & ~7u;
+    a0 = a0 < hi ? a0 : hi;
+    const uint32_t a1 = a0 + ((hi - a0) & ~7u);
+    for (uint32_t p = lo + threadIdx.x; p < a0; p += blockDim.x) {
         const int s = rslot[p];
         const float2 v = rval[p];
-        atomicAdd(&s_acc[2 * s], v.x);
-        atomicAdd(&s_acc[2 * s + 1], v.y);
+        atomicAdd(&s_ax[s], v.x);
+        atomicAdd(&s_ay[s], v.y);
+    }
+    for (uint32_t p = a0 + 8u * threadIdx.x; p < a1; p += 8u * blockDim.x) {
+        const uint4 sl = *reinterpret_cast<const uint4*>(rslot + p);
+        float4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = *reinterpret_cast<const float4*>(rval + p + 2 * k);
+        const uint32_t w4[4] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s0 = (int)(w4[k] & 0xFFFFu), s1 = (int)(w4[k] >> 16);
+            atomicAdd(&s_ax[s0], v[k].x);
+            atomicAdd(&s_ay[s0], v[k].y);
+            atomicAdd(&s_ax[s1], v[k].z);
+            atomicAdd(&s_ay[s1], v[k].w);
+        }
+    }
+    for (uint32_t p = a1 + threadIdx.x; p < hi; p += blockDim.x) {
+        const int s = rslot[p];
+        const float2 v = rval[p];
+        atomicAdd(&s_ax[s], v.x);
+        atomicAdd(&s_ay[s], v.y);
     }
     __syncthreads();
     const int l = lp + b / nb_level;
     float* tb = gtable + (g.offs[l] + (int64_t)(b % nb_level) * kPartSlots) * 2;
-    for (int s = threadIdx.x; s < kPartSlots * 2; s += blockDim.x) {
-        const float v = s_acc[s];
-        if (v != 0.f) tb[s] += v;   // this workgroup owns these slots
+    for (int s = threadIdx.x; s < kPartSlots; s += blockDim.x) {   // this workgroup owns these slots
+        const float vx = s_ax[s], vy = s_ay[s];
+        if (vx != 0.f || vy != 0.f) {
+            float2 t = reinterpret_cast<float2*>(tb)[s];
+            t.x += vx;
+            t.y += vy;
+            reinterpret_cast<float2*>(tb)[s] = t;
+        }
     }
 }
 
@@ -557,8 +608,7 @@ extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* gr
         if (r) return r;
     }
     hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, g, lp, P.nb_level, hist);
-    hipLaunchKernelGGL(part_column_kernel, dim3((unsigned)ceil_div(P.nb, 256)), dim3(256), 0, st, hist, P.n_chunks, P.nb,
-                       total);
+    hipLaunchKernelGGL(part_column_kernel, dim3((unsigned)P.nb), dim3(256), 0, st, hist, P.n_chunks, total);
     hipLaunchKernelGGL(part_bucket_scan_kernel, dim3(1), dim3(256), 0, st, (const uint32_t*)total, P.nb, start);
     hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, grad_out_dev, g, lp,
                        P.nb_level, (const uint32_t*)hist, (const uint32_t*)start, rslot, rval);
