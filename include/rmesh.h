@@ -145,6 +145,23 @@ int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_
                      const float* grad_img_dev, const float* grad_dist_dev,
                      float* grad_vertices_dev, float* grad_features_dev, rm_stream_t stream);
 
+/* The same backward as rm_composite_bwd (same reference interface) in two passes: each (tile, tet)
+ * adds its reduced plane-record adjoint sums (24 or 32 floats) to the tet's 128-byte row of the
+ * workspace, then one thread per tet converts its row to vertex and feature gradients. Float
+ * atomics on this GPU are serviced per 64-byte segment, so a row costs 2 segments per (tile, tet)
+ * where the direct path's scattered vertex and feature rows cost about 6. n_tets is the number
+ * of rows of indices_dev / features_dev; the workspace (rm_composite_bwd_workspace_size(n_tets)
+ * bytes, 128-byte aligned) needs no initial contents. Gradients are added into
+ * grad_vertices_dev / grad_features_dev as by rm_composite_bwd. */
+size_t rm_composite_bwd_workspace_size(int64_t n_tets);
+int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                        const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                        const float* vertices_dev, const float* features_dev, int32_t n_features,
+                        const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                        const int32_t* ncontrib_dev, const float* grad_img_dev, const float* grad_dist_dev,
+                        float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
+                        size_t workspace_bytes, rm_stream_t stream);
+
 /* ---- batched forward rendering (serving) -----------------------------------------------
  * No counterpart in the reference (which renders one view per call, evaluate.py:50-63 /
  * tile_benchmark.py:59-94). Many views of device-resident scenes, with mixed resolutions, render

@@ -142,19 +142,18 @@ __device__ inline Hit intersect(const PL pl[4], RayT<T> r, float mtf) {
     T se = (T)-1e30, sx = (T)1e30, de = 1, dx = 1;
     int ie = -1, ix = -1;
     bool miss = false;
+    // branch-free: a plane nearly parallel to the ray (|den| < 1e-10) only decides a miss; the
+    // others are entering (den < 0) or exiting candidates (s of a parallel plane is not used)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        T den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
-        if (fabs(den) < (T)1e-10) {
-            if (pl[i].w + mt * den > (T)0) miss = true;
-        } else {
-            T s = -pl[i].w * rcp_(den);
-            if (den < (T)0) {
-                if (s > se) { se = s; de = den; ie = i; }
-            } else {
-                if (s < sx) { sx = s; dx = den; ix = i; }
-            }
-        }
+        const T den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
+        const bool par = fabs(den) < (T)1e-10;
+        miss = miss || (par && pl[i].w + mt * den > (T)0);
+        const T s = -pl[i].w * rcp_(den);
+        const bool ent = !par && den < (T)0 && s > se;
+        const bool ext = !par && !(den < (T)0) && s < sx;
+        se = ent ? s : se; de = ent ? den : de; ie = ent ? i : ie;
+        sx = ext ? s : sx; dx = ext ? den : dx; ix = ext ? i : ix;
     }
     h.hit = !miss && !(se > sx) && (sx > mt);
     T sec = se > mt ? se : mt;
@@ -173,12 +172,16 @@ __device__ inline Hit intersect(const PL pl[4], RayT<T> r, float mtf) {
 // phi(x) = (1 - e^-x) / x with the reference's small-x branch (interp_version.slang:21-28).
 __device__ inline float phi_of(float x, float em1) { return x < 1e-6f ? 1.0f - 0.5f * x : -em1 / x; }
 
-// phi'(x): series below 0.05 (truncation < 1e-9 relative), closed form above.
+// phi'(x): series below 0.05 (truncation < 1e-9 relative), closed form above (quotient through
+// the hardware reciprocal, 1 ulp).
 __device__ inline float dphi_of(float x, float em1, float alpha) {
-    if (x < 1e-6f) return -0.5f;
-    if (x < 0.05f) return -0.5f + x * (1.f / 3.f + x * (-1.f / 8.f + x * (1.f / 30.f - x * (1.f / 144.f))));
-    return (x * alpha + em1) / (x * x);
+    const float series = -0.5f + x * (1.f / 3.f + x * (-1.f / 8.f + x * (1.f / 30.f - x * (1.f / 144.f))));
+    const float closed = (x * alpha + em1) * rcp_(x * x);
+    return x < 1e-6f ? -0.5f : (x < 0.05f ? series : closed);
 }
+
+// phi(x) for the backward, quotient through the hardware reciprocal.
+__device__ inline float phi_of_rcp(float x, float em1) { return x < 1e-6f ? 1.0f - 0.5f * x : -em1 * rcp_(x); }
 
 // XCD-aware workgroup order: consecutive dispatch slots go round-robin over the 8 XCDs, so
 // slot b is remapped to make each XCD own a contiguous band of tiles (shared tets stay in its L2).
@@ -254,7 +257,7 @@ __device__ inline int plane_rec(float3 af, float3 bf, float3 cf, float3 of, floa
 template <int WG, int F, bool BWD, bool P64>
 __device__ inline void stage_tet(Stage<WG, F, BWD, P64>& S, int slot, int tet, const CamArgs& cam,
                                  const int4* __restrict__ indices, const float* __restrict__ verts,
-                                 const float* __restrict__ feats) {
+                                 const float* __restrict__ feats, bool stage_vtx = true) {
     using T = typename Prec<P64>::R;
     constexpr int NGRD = NG_<F>::NGRD;
     int4 q = indices[tet];
@@ -286,9 +289,11 @@ __device__ inline void stage_tet(Stage<WG, F, BWD, P64>& S, int slot, int tet, c
                                     4 * k + 2 < NGRD ? g[(4 * k + 2) % NGRD] : 0.f,
                                     4 * k + 3 < NGRD ? g[(4 * k + 3) % NGRD] : 0.f);
     if constexpr (BWD) {
-        S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
-        S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
-        S.vtx[2][slot] = make_float4(v2.z, v3.x, v3.y, v3.z);
+        if (stage_vtx) {
+            S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
+            S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
+            S.vtx[2][slot] = make_float4(v2.z, v3.x, v3.y, v3.z);
+        }
         S.meta[0][slot] = tet;
         S.meta[1][slot] = signs;
     }
@@ -476,20 +481,16 @@ __device__ inline void group_sum(float (&v)[N], int width) {
 // u[4], r[12]]: outputs 0..11 are the four vertices' gradients, 12.. the feature gradients.
 // Colour gradients enter through the base colour folded at v0 (cv0 = cam - v0); each face's
 // plane record n = +-cross(b - a, c - a), e = n . (cam - a) passes its adjoint to a, b, c.
-// The face normals are recomputed from the staged vertices in the plane precision (the same
-// cross product plane_rec formed, flipped by the recorded sign), so the plane records in LDS are
-// not read here.
-template <int WG, int F, bool P64, int NV, int NOUT>
-__device__ inline void bwd_outputs(const Stage<WG, F, true, P64>& S, int slot, const float (&acc)[NV], float3 cp,
-                                   float (&outv)[NOUT]) {
+// The face normals are recomputed from the vertices in the plane precision (the same cross
+// product plane_rec formed, flipped by the recorded sign), so no plane record is read here.
+// The map from acc to the outputs is linear with coefficients fixed by the tet and the camera,
+// so sums of acc over tiles may be converted once per tet.
+template <int F, bool P64, int NV, int NOUT>
+__device__ inline void tet_grads(const float3 (&V4)[4], int signs, const float (&gr)[NG_<F>::NGRD],
+                                 const float (&acc)[NV], float3 cp, float (&outv)[NOUT]) {
     using T = typename Prec<P64>::R;
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;
-    const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
-    const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x), f3(q2.y, q2.z, q2.w)};
-    const int signs = S.meta[1][slot];
-    float gr[NGRD];
-    read_grads<WG, F, true, P64>(S, slot, gr);
     float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
     const float3 cv0 = cp - V4[0];
     outv[12] = acc[0];
@@ -532,13 +533,69 @@ __device__ inline void bwd_outputs(const Stage<WG, F, true, P64>& S, int slot, c
     }
 }
 
-template <int TS, int F, bool P64>
+template <int WG, int F, bool P64, int NV, int NOUT>
+__device__ inline void bwd_outputs(const Stage<WG, F, true, P64>& S, int slot, const float (&acc)[NV], float3 cp,
+                                   float (&outv)[NOUT]) {
+    const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
+    const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x), f3(q2.y, q2.z, q2.w)};
+    float gr[NG_<F>::NGRD];
+    read_grads<WG, F, true, P64>(S, slot, gr);
+    tet_grads<F, P64, NV, NOUT>(V4, S.meta[1][slot], gr, acc, cp, outv);
+}
+
+// Per-tet sums of the plane-record adjoints, one 128-byte row per tet (at most 32 floats: the
+// 16-float halves are whole 64-byte segments, the unit in which float atomics are serviced).
+// Row position pos holds sum j = row_sum_index(pos): the order in which group16_transpose_sum
+// leaves the sums in the lanes, so that each half-row is added by one instruction.
+constexpr int kRowFloats = 32;
+
+__host__ __device__ constexpr int row_sum_index(int pos) {
+    return 16 * ((pos >> 3) & 1) + 8 * ((pos >> 2) & 1) + 4 * ((pos >> 4) & 1) + (pos & 3);
+}
+
+// Sum over a 16-lane group (one DPP row) of 32 values per lane, transposed: afterwards lane r
+// holds the group sums of values row_sum_index(r) (o0) and row_sum_index(16 + r) (o1). Two
+// halving steps exchange half of the values with a partner lane, r ^ 8 (row_ror:8) and then
+// r ^ 7 (row_half_mirror), and add only the half the lane keeps; the quad's 4 lanes are then
+// summed with two quad permutes: 24 exchange adds with 48 selects, 16 quad adds and 6 final
+// selects in place of the 4 x 32 adds of summing every value in every lane.
+__device__ inline void group16_transpose_sum(const float (&a)[32], int r, float& o0, float& o1) {
+    const bool h8 = (r & 8) != 0;
+    float k[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const float keep = h8 ? a[16 + i] : a[i];
+        const float send = h8 ? a[i] : a[16 + i];
+        k[i] = keep + dpp<0x128>(send);   // row_ror:8
+    }
+    const bool h4 = (r & 4) != 0;
+    float l[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float keep = h4 ? k[8 + i] : k[i];
+        const float send = h4 ? k[i] : k[8 + i];
+        l[i] = keep + dpp<0x141>(send);   // row_half_mirror
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        l[i] += dpp<0xB1>(l[i]);          // quad_perm [1,0,3,2]
+        l[i] += dpp<0x4E>(l[i]);          // quad_perm [2,3,0,1]
+    }
+    const int p = r & 3;
+    o0 = p == 0 ? l[0] : p == 1 ? l[1] : p == 2 ? l[2] : l[3];
+    o1 = p == 0 ? l[4] : p == 1 ? l[5] : p == 2 ? l[6] : l[7];
+}
+
+// ROWS: instead of converting each (tile, tet) sum to vertex and feature gradients, add the sums
+// themselves to the tet's row of `rows` (kRowFloats floats per tet); composite_tet_grads_kernel
+// converts the rows once per tet. Otherwise gradients go straight to gverts / gfeats.
+template <int TS, int F, bool P64, bool ROWS>
 __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
     const int* __restrict__ ncontrib, const float4* __restrict__ gimg, const float* __restrict__ gdist,
-    float* __restrict__ gverts, float* __restrict__ gfeats) {
+    float* __restrict__ gverts, float* __restrict__ gfeats, float* __restrict__ rows) {
     using C = Cfg<TS>;
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;        // independent colour-gradient rows
@@ -549,9 +606,12 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     using PL = typename Prec<P64>::PL;
     __shared__ Stage<C::WG, F, true, P64> S;
     __shared__ int s_nc[C::NG];
-    // deferred per-lane epilogue: the reduction group is the tile group, the workgroup one wave,
-    // and the slots' gradients fit over the plane records (FP64 records; FP32 ones are too small)
-    constexpr bool kDefer = C::RED == C::GROUP && C::WG == 64 && sizeof(float) * C::WG * (NOUT + 1) <= sizeof(S.pl);
+    static_assert(NV <= kRowFloats, "row too small");
+    // deferred per-lane epilogue of the direct path: the reduction group is the tile group, the
+    // workgroup one wave, and the slots' gradients fit over the plane records (FP64 records;
+    // FP32 ones are too small)
+    constexpr bool kDefer = !ROWS && C::RED == C::GROUP && C::WG == 64 &&
+                            sizeof(float) * C::WG * (NOUT + 1) <= sizeof(S.pl);
 
     int g, r, tx, ty, px, py;
     bool tile_ok, inside;
@@ -600,7 +660,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         __syncthreads();
         int k = c * C::GROUP + r;
         if (c < nchunks_g && k < nmax_g)
-            stage_tet<C::WG, F, true, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+            stage_tet<C::WG, F, true, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats, !ROWS);
         __syncthreads();
         // tile groups past their last chunk run no slots (cnt = 0) but still reach the barriers below
         int cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
@@ -626,7 +686,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     read_grads<C::WG, F, true, P64>(S, slot, gr);
                     float em1 = expm1f(-x);
                     float alpha = 1.0f + em1;
-                    float ph = phi_of(x, em1);
+                    float ph = phi_of_rcp(x, em1);
                     float w0 = ph - alpha, w1 = 1.0f - ph;
                     float gd[3];
                     gd[0] = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
@@ -703,8 +763,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     }
                     // plane adjoints: s_i = -e_i / den_i
                     const int PU = 4 + NCH + 3 * NCH;
-                    float ue = (h.se_raw > mt) ? dse / h.den_e : 0.f;
-                    float ux = dsx / h.den_x;
+                    float ue = (h.se_raw > mt) ? dse * rcp_(h.den_e) : 0.f;
+                    float ux = dsx * rcp_(h.den_x);
                     float re = ue * h.se_raw, rx = ux * h.sx;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
@@ -723,6 +783,18 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             }
             uint64_t any = __ballot(active) & red_mask;
             if (any == 0) continue;
+            if constexpr (ROWS && C::RED == 16) {
+                // transposed group sum: lane r ends with row positions r and 16 + r
+                float a32[kRowFloats];
+#pragma unroll
+                for (int i = 0; i < kRowFloats; ++i) a32[i] = i < NV ? acc[i] : 0.f;
+                float o0, o1;
+                group16_transpose_sum(a32, r, o0, o1);
+                float* row = rows + (int64_t)S.meta[0][slot] * kRowFloats;
+                if (o0 != 0.f) unsafeAtomicAdd(row + r, o0);
+                if (o1 != 0.f) unsafeAtomicAdd(row + 16 + r, o1);
+                continue;
+            }
             group_sum<NV>(acc, C::RED);
             if constexpr (kDefer) {
                 // the lane whose in-group index is this slot keeps the sums; the chunk's epilogues
@@ -731,6 +803,22 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
 #pragma unroll
                     for (int i = 0; i < NV; ++i) kept[i] = acc[i];
                     kept_on = true;
+                }
+                continue;
+            }
+            if constexpr (ROWS) {
+                // lane l of the reduction group adds row position l (the sum row_sum_index(l))
+                const int tet = S.meta[0][slot];
+#pragma unroll
+                for (int l0 = 0; l0 < kRowFloats; l0 += C::RED) {
+                    const int l = lane - red_base + l0;
+                    if (l < kRowFloats) {
+                        const int j = row_sum_index(l);
+                        float val = 0.f;
+#pragma unroll
+                        for (int i = 0; i < NV; ++i) val = (j == i) ? acc[i] : val;
+                        if (val != 0.f) unsafeAtomicAdd(rows + (int64_t)tet * kRowFloats + l, val);
+                    }
                 }
                 continue;
             }
@@ -758,7 +846,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                 }
             }
         }
-        if constexpr (kDefer) {
+        if constexpr (kDefer && !ROWS) {
             // each lane turns its kept slot's sums into the tet's gradients; the gradients then go
             // through LDS (over the plane/colour records, dead by then) so that the atomics are
             // issued tet by tet with consecutive lanes on consecutive outputs, as in the path above
@@ -811,15 +899,85 @@ int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
     return 0;
 }
 
-template <int TS, int F, bool P64>
+// Zeroes n4 float4s (a kernel rather than a memset, so that the sequence may be graph-captured).
+__global__ __launch_bounds__(256) void zero_f4_kernel(float4* __restrict__ p, int64_t n4) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Vertex and feature gradients of each tet from its row of plane-record adjoint sums (the
+// layout of acc in composite_bwd_kernel), one thread per tet; tets whose row is all zero are
+// skipped. Feature gradients are added in place (one writer per row), vertex gradients with
+// float atomics.
+template <int F, bool P64>
+__global__ __launch_bounds__(256) void composite_tet_grads_kernel(
+    const float4* __restrict__ rows, int64_t n_tets, const int4* __restrict__ indices, const float* __restrict__ verts,
+    const float* __restrict__ feats, float3 cp, float* __restrict__ gverts, float* __restrict__ gfeats) {
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    constexpr int NGRD = NG_<F>::NGRD;
+    constexpr int NCH = F == 7 ? 1 : 3;
+    constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;
+    constexpr int NOUT = 12 + F;
+    static_assert(NV % 4 == 0 && NV <= kRowFloats, "row layout");
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tets) return;
+    float acc[NV];
+    bool any = false;
+    float row[kRowFloats];
+#pragma unroll
+    for (int k = 0; k < kRowFloats / 4; ++k) {
+        const float4 v = rows[t * (kRowFloats / 4) + k];
+        row[4 * k] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+        any = any || v.x != 0.f || v.y != 0.f || v.z != 0.f || v.w != 0.f;
+    }
+#pragma unroll
+    for (int pos = 0; pos < kRowFloats; ++pos)
+        if (row_sum_index(pos) < NV) acc[row_sum_index(pos)] = row[pos];
+    if (!any) return;
+    const int4 q = indices[t];
+    const float3 V4[4] = {load_v(verts, q.x), load_v(verts, q.y), load_v(verts, q.z), load_v(verts, q.w)};
+    PL unused;
+    int signs = 0;
+    signs |= plane_rec<T>(V4[0], V4[1], V4[2], V4[3], cp, unused) << 0;
+    signs |= plane_rec<T>(V4[0], V4[1], V4[3], V4[2], cp, unused) << 1;
+    signs |= plane_rec<T>(V4[0], V4[2], V4[3], V4[1], cp, unused) << 2;
+    signs |= plane_rec<T>(V4[1], V4[2], V4[3], V4[0], cp, unused) << 3;
+    float* gf = gfeats + (int64_t)F * t;
+    const float* f = feats + (int64_t)F * t;
+    float gr[NGRD];
+#pragma unroll
+    for (int k = 0; k < NGRD; ++k) gr[k] = f[4 + k];
+    float outv[NOUT];
+    tet_grads<F, P64, NV, NOUT>(V4, signs, gr, acc, cp, outv);
+#pragma unroll
+    for (int k = 0; k < F; ++k) gf[k] += outv[12 + k];
+    const int vid[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (outv[3 * v + k] != 0.f) unsafeAtomicAdd(gverts + 3 * (int64_t)vid[v] + k, outv[3 * v + k]);
+}
+
+template <int TS, int F, bool P64, bool ROWS>
 int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, const float* img, const float* dist, const int* nc,
-               const float* gimg, const float* gdist, float* gverts, float* gfeats, hipStream_t st) {
+               const float* gimg, const float* gdist, float* gverts, float* gfeats, float* rows, int64_t n_tets,
+               hipStream_t st) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
-    hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+    if (ROWS && n_tets > 0) {
+        const int64_t n4 = n_tets * (kRowFloats / 4);
+        hipLaunchKernelGGL(zero_f4_kernel, dim3((unsigned)ceil_div(n4, 256)), dim3(256), 0, st, (float4*)rows, n4);
+    }
+    hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
                        n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
-                       (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats);
+                       (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats, rows);
+    if (ROWS && n_tets > 0)
+        hipLaunchKernelGGL((composite_tet_grads_kernel<F, P64>), dim3((unsigned)ceil_div(n_tets, 256)), dim3(256), 0,
+                           st, (const float4*)rows, n_tets, (const int4*)indices, verts, feats,
+                           make_float3(a.cam_pos[0], a.cam_pos[1], a.cam_pos[2]), gverts, gfeats);
     return 0;
 }
 
@@ -939,9 +1097,56 @@ extern "C" int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_
     CamArgs a = make_cam_args(*cam);
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
-#define RM_BWD(TS, F, P) launch_bwd<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
-                                        features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, grad_img_dev, \
-                                        grad_dist_dev, grad_vertices_dev, grad_features_dev, st)
+#define RM_BWD(TS, F, P) launch_bwd<TS, F, P, false>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, \
+                                        vertices_dev, features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, \
+                                        grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev, nullptr, \
+                                        0, st)
+#define RM_BWD_P(TS, F) (g_fp64_planes ? RM_BWD(TS, F, true) : RM_BWD(TS, F, false))
+    if (n_features == 7) {
+        if (ts == 4) RM_BWD_P(4, 7); else if (ts == 8) RM_BWD_P(8, 7); else RM_BWD_P(16, 7);
+    } else {
+        if (ts == 4) RM_BWD_P(4, 13); else if (ts == 8) RM_BWD_P(8, 13); else RM_BWD_P(16, 13);
+    }
+#undef RM_BWD_P
+#undef RM_BWD
+    RM_CHECK_LAUNCH("composite_bwd_kernel");
+    return RM_OK;
+}
+
+extern "C" size_t rm_composite_bwd_workspace_size(int64_t n_tets) {
+    return n_tets > 0 ? (size_t)n_tets * kRowFloats * sizeof(float) : 0;
+}
+
+extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                   const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                                   const float* vertices_dev, const float* features_dev, int32_t n_features,
+                                   const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                                   const int32_t* ncontrib_dev, const float* grad_img_dev,
+                                   const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
+                                   void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n_sorted >= 0, "rm_composite_bwd_ws: negative n_sorted");
+    RM_CHECK_ARG(n_tets >= 0, "rm_composite_bwd_ws: negative n_tets");
+    if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
+    int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
+                         jitter_dev, "rm_composite_bwd_ws");
+    if (e) return e;
+    RM_CHECK_ARG(img_dev && dist_dev && ncontrib_dev && grad_img_dev && grad_dist_dev && grad_vertices_dev &&
+                     grad_features_dev,
+                 "rm_composite_bwd_ws: null buffer");
+    RM_CHECK_ARG(((uintptr_t)img_dev & 15) == 0 && ((uintptr_t)grad_img_dev & 15) == 0,
+                 "rm_composite_bwd_ws: img and grad_img must be 16-byte aligned");
+    RM_CHECK_ARG(workspace_bytes >= rm_composite_bwd_workspace_size(n_tets),
+                 "rm_composite_bwd_ws: workspace smaller than rm_composite_bwd_workspace_size(n_tets)");
+    RM_CHECK_ARG(n_tets == 0 || (workspace_dev && ((uintptr_t)workspace_dev & 127) == 0),
+                 "rm_composite_bwd_ws: workspace must be 128-byte aligned");
+    CamArgs a = make_cam_args(*cam);
+    hipStream_t st = as_stream(stream);
+    const int ts = cam->tile_width;
+    float* rows = static_cast<float*>(workspace_dev);
+#define RM_BWD(TS, F, P) launch_bwd<TS, F, P, true>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, \
+                                        vertices_dev, features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, \
+                                        grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev, rows, \
+                                        n_tets, st)
 #define RM_BWD_P(TS, F) (g_fp64_planes ? RM_BWD(TS, F, true) : RM_BWD(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_BWD_P(4, 7); else if (ts == 8) RM_BWD_P(8, 7); else RM_BWD_P(16, 7);

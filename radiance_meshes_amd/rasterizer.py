@@ -275,12 +275,17 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         grad_distortion_img = grad_distortion_img.contiguous().float()
         vertices_grad = torch.zeros_like(vertices)
         tet_density_grad = torch.zeros_like(tet_density)
+        L = lib()
+        n_tets = indices.shape[0]
+        wsb = L.rm_composite_bwd_workspace_size(n_tets)
+        ws = torch.empty((max(wsb, 1),), device=vertices.device, dtype=torch.uint8)
         with timing.span("composite_bwd"):
-            check(lib().rm_composite_bwd(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices),
-                                     ptr(vertices), ptr(tet_density), tet_density.shape[1], ptr(ray_jitter),
-                                     ptr(output_img), ptr(distortion_img), ptr(n_contributors),
-                                     ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),
-                                     ptr(tet_density_grad), stream_of(vertices.device)), "rm_composite_bwd")
+            check(L.rm_composite_bwd_ws(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges),
+                                        ptr(indices), n_tets, ptr(vertices), ptr(tet_density), tet_density.shape[1],
+                                        ptr(ray_jitter), ptr(output_img), ptr(distortion_img), ptr(n_contributors),
+                                        ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),
+                                        ptr(tet_density_grad), ptr(ws), wsb, stream_of(vertices.device)),
+                  "rm_composite_bwd_ws")
         return None, None, None, vertices_grad, tet_density_grad, None, None, None
 
 

@@ -166,3 +166,61 @@ def test_composite_forward_fp32_planes(oracle_lib, kind, tile, nf):
     print(f"fp32 planes: rgb max-abs {err.max():.3e}, 99.9th pct {np.percentile(err, 99.9):.3e}")
     assert np.percentile(err, 99.9) <= 5e-5
     assert err.max() <= 2e-3
+
+
+@pytest.mark.parametrize("kind,tile,nf", CASES)
+@pytest.mark.parametrize("fp64", [True, False])
+def test_composite_backward_rows_match_direct(kind, tile, nf, fp64):
+    """rm_composite_bwd_ws (per-tet rows of summed plane adjoints, converted once per tet) against
+    rm_composite_bwd (conversion per tile and tet): the same gradients up to FP32 summation order."""
+    import torch
+    from radiance_meshes_amd.rasterizer import _cam, set_intersect_precision, vertex_and_tile_shader
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    sc = scene(kind, tile, nf)
+    print(f"case {kind} tile={tile} nf={nf} fp64={fp64}", flush=True)
+    verts, idx, feats, _ = device_inputs(sc)
+    jit = torch.from_numpy(jitter(sc, 5)).cuda()
+    tcam, rg = tcam_of(sc), grid_of(sc)
+    H, W = sc.camera.height, sc.camera.width
+    sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(idx, verts, tcam, rg)
+    cam = _cam(tcam, rg)
+    L = lib()
+    st = stream_of(verts.device)
+    img = torch.empty((H, W, 4), device="cuda")
+    dist = torch.empty((H, W, 5), device="cuda")
+    nc = torch.empty((H, W), device="cuda", dtype=torch.int32)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    gimg = torch.randn((H, W, 4), device="cuda", generator=gen)
+    gdist = torch.randn((H, W, 5), device="cuda", generator=gen) * 0.1
+    set_intersect_precision(fp64)
+    try:
+        check(L.rm_composite_fwd(cam, ptr(sidx), sidx.numel(), ptr(ranges), ptr(idx), ptr(verts), ptr(feats), nf,
+                                 ptr(jit), ptr(img), ptr(dist), ptr(nc), st), "rm_composite_fwd")
+        out = []
+        for ws_path in (False, True):
+            gv = torch.zeros_like(verts)
+            gf = torch.zeros_like(feats)
+            if ws_path:
+                T = idx.shape[0]
+                wsb = L.rm_composite_bwd_workspace_size(T)
+                ws = torch.full((wsb,), 7, device="cuda", dtype=torch.uint8)   # stale contents
+                check(L.rm_composite_bwd_ws(cam, ptr(sidx), sidx.numel(), ptr(ranges), ptr(idx), T, ptr(verts),
+                                            ptr(feats), nf, ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(gimg),
+                                            ptr(gdist), ptr(gv), ptr(gf), ptr(ws), wsb, st), "rm_composite_bwd_ws")
+            else:
+                check(L.rm_composite_bwd(cam, ptr(sidx), sidx.numel(), ptr(ranges), ptr(idx), ptr(verts), ptr(feats),
+                                         nf, ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(gimg), ptr(gdist), ptr(gv),
+                                         ptr(gf), st), "rm_composite_bwd")
+            torch.cuda.synchronize()
+            out.append((gv.cpu().double().numpy(), gf.cpu().double().numpy()))
+    finally:
+        set_intersect_precision(True)
+    for name, a, b in (("vertex", out[1][0], out[0][0]), ("feature", out[1][1], out[0][1])):
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"{name}: rows vs direct rel L2 = {rel:.3e} (|direct| = {np.linalg.norm(b):.3e})")
+        assert np.linalg.norm(b) > 0
+        assert rel <= 2e-5, name
+    # tets no tile list names keep exactly zero gradients
+    listed = np.zeros(idx.shape[0], bool)
+    listed[sidx.cpu().numpy()] = True
+    assert not out[1][1][~listed].any()
