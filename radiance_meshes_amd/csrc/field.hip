@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
                                                                        int l_end, float* __restrict__ gtable) {
     static_assert(F == 1 || F == 2, "LDS table sized for n_feat <= 2");
     __shared__ uint32_t s_key[kAggSlots];
-    __shared__ float s_val[kAggSlots * F];
+    __shared__ double s_val[kAggSlots * F];   // FP64: LDS FP64 adds run ~4x the rate of FP32 ones
     const int tid = threadIdx.x;
     // dispatch slots go round-robin over the 8 XCDs: give each XCD a contiguous band of chunks so
     // the coarse-level slots that neighbouring chunks share stay in one L2
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
     float xv[3] = {0.f, 0.f, 0.f};
     if (valid) load_point(x, n, xv);
     for (int p = tid; p < kAggSlots; p += kAggPoints) s_key[p] = kEmptySlot;
-    for (int p = tid; p < kAggSlots * F; p += kAggPoints) s_val[p] = 0.f;
+    for (int p = tid; p < kAggSlots * F; p += kAggPoints) s_val[p] = 0.0;
     __syncthreads();
     const int lane = tid & 63;
     for (int l = 0; l < l_end; ++l) {
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
                 const uint32_t k = atomicCAS(&s_key[p], kEmptySlot, h);
                 if (k == kEmptySlot || k == h) {
 #pragma unroll
-                    for (int f = 0; f < F; ++f) atomicAdd(&s_val[p * F + f], v[f]);
+                    for (int f = 0; f < F; ++f) atomicAdd(&s_val[p * F + f], (double)v[f]);
                     placed = true;
                 }
                 p = (p + 1) & (kAggSlots - 1);
@@ -247,9 +247,9 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
             if (k == kEmptySlot) continue;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const float v = s_val[p * F + f];
+                const float v = (float)s_val[p * F + f];
                 if (v != 0.f) unsafeAtomicAdd(tb + (int64_t)k * F + f, v);
-                s_val[p * F + f] = 0.f;
+                s_val[p * F + f] = 0.0;
             }
             s_key[p] = kEmptySlot;
         }
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
 // Backward of the fine hashed levels by slot ownership (n_feat 2). Past a few hundred cells per
 // axis nearly every corner update of a 256-point chunk hits its own table slot, so the
 // aggregation above issues one global atomic per update. Instead the updates of levels
-// [lp, L) are sorted by bucket (8192 consecutive slots of one level) with a counting sort, and one
+// [lp, L) are sorted by bucket (4096 consecutive slots of one level) with a counting sort, and one
 // workgroup per bucket sums its updates in LDS and adds them to the table with plain stores:
 //  1. part_count: per 4096-point chunk, the number of updates per bucket (LDS histogram),
 //  2. part_offsets: exclusive offsets per (chunk, bucket), bucket-major,
@@ -268,9 +268,10 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
 //     (slot within the bucket, value pair),
 //  4. part_accumulate: per bucket, the sum of its updates added to the table.
 constexpr int kPartPoints = 4096;              // query points per chunk (16 per lane)
-constexpr int kPartLog2Slots = 13;
-constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket (64 KB of LDS sums)
+constexpr int kPartLog2Slots = 12;
+constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket (64 KB of FP64 LDS sums)
 constexpr int kPartMaxBuckets = 8192;
+constexpr int kPartAccThreads = 512;
 
 __global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
                                                          int lp, int nb_level, uint32_t* __restrict__ hist) {
@@ -389,15 +390,18 @@ __global__ __launch_bounds__(256) void part_scatter_kernel(const float* __restri
     }
 }
 
-__global__ __launch_bounds__(256) void part_accumulate_kernel(const uint32_t* __restrict__ bucket_start,
-                                                              const uint16_t* __restrict__ rslot,
-                                                              const float2* __restrict__ rval, GridArgs g, int lp,
-                                                              int nb_level, float* __restrict__ gtable) {
-    // the two features in separate arrays: a wave's random slots then spread over all 64 banks
-    __shared__ float s_ax[kPartSlots], s_ay[kPartSlots];
+// The sums are kept in FP64: LDS float adds (ds_add_f32) run at about a quarter of the rate of
+// FP64 ones (ds_add_f64) on this GPU, and the wider sum is rounded once when added to the table.
+__global__ __launch_bounds__(kPartAccThreads) void part_accumulate_kernel(const uint32_t* __restrict__ bucket_start,
+                                                                          const uint16_t* __restrict__ rslot,
+                                                                          const float2* __restrict__ rval, GridArgs g,
+                                                                          int lp, int nb_level,
+                                                                          float* __restrict__ gtable) {
+    // the two features in separate arrays: a wave's random slots then spread over the banks
+    __shared__ double s_ax[kPartSlots], s_ay[kPartSlots];
     for (int s = threadIdx.x; s < kPartSlots; s += blockDim.x) {
-        s_ax[s] = 0.f;
-        s_ay[s] = 0.f;
+        s_ax[s] = 0.0;
+        s_ay[s] = 0.0;
     }
     __syncthreads();
     const int b = blockIdx.x;
@@ -410,8 +414,8 @@ __global__ __launch_bounds__(256) void part_accumulate_kernel(const uint32_t* __
     for (uint32_t p = lo + threadIdx.x; p < a0; p += blockDim.x) {
         const int s = rslot[p];
         const float2 v = rval[p];
-        atomicAdd(&s_ax[s], v.x);
-        atomicAdd(&s_ay[s], v.y);
+        atomicAdd(&s_ax[s], (double)v.x);
+        atomicAdd(&s_ay[s], (double)v.y);
     }
     for (uint32_t p = a0 + 8u * threadIdx.x; p < a1; p += 8u * blockDim.x) {
         const uint4 sl = *reinterpret_cast<const uint4*>(rslot + p);
@@ -422,27 +426,27 @@ __global__ __launch_bounds__(256) void part_accumulate_kernel(const uint32_t* __
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int s0 = (int)(w4[k] & 0xFFFFu), s1 = (int)(w4[k] >> 16);
-            atomicAdd(&s_ax[s0], v[k].x);
-            atomicAdd(&s_ay[s0], v[k].y);
-            atomicAdd(&s_ax[s1], v[k].z);
-            atomicAdd(&s_ay[s1], v[k].w);
+            atomicAdd(&s_ax[s0], (double)v[k].x);
+            atomicAdd(&s_ay[s0], (double)v[k].y);
+            atomicAdd(&s_ax[s1], (double)v[k].z);
+            atomicAdd(&s_ay[s1], (double)v[k].w);
         }
     }
     for (uint32_t p = a1 + threadIdx.x; p < hi; p += blockDim.x) {
         const int s = rslot[p];
         const float2 v = rval[p];
-        atomicAdd(&s_ax[s], v.x);
-        atomicAdd(&s_ay[s], v.y);
+        atomicAdd(&s_ax[s], (double)v.x);
+        atomicAdd(&s_ay[s], (double)v.y);
     }
     __syncthreads();
     const int l = lp + b / nb_level;
     float* tb = gtable + (g.offs[l] + (int64_t)(b % nb_level) * kPartSlots) * 2;
     for (int s = threadIdx.x; s < kPartSlots; s += blockDim.x) {   // this workgroup owns these slots
-        const float vx = s_ax[s], vy = s_ay[s];
-        if (vx != 0.f || vy != 0.f) {
+        const double vx = s_ax[s], vy = s_ay[s];
+        if (vx != 0.0 || vy != 0.0) {
             float2 t = reinterpret_cast<float2*>(tb)[s];
-            t.x += vx;
-            t.y += vy;
+            t.x = (float)((double)t.x + vx);
+            t.y = (float)((double)t.y + vy);
             reinterpret_cast<float2*>(tb)[s] = t;
         }
     }
@@ -612,7 +616,7 @@ extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* gr
     hipLaunchKernelGGL(part_bucket_scan_kernel, dim3(1), dim3(256), 0, st, (const uint32_t*)total, P.nb, start);
     hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, grad_out_dev, g, lp,
                        P.nb_level, (const uint32_t*)hist, (const uint32_t*)start, rslot, rval);
-    hipLaunchKernelGGL(part_accumulate_kernel, dim3((unsigned)P.nb), dim3(256), 0, st, (const uint32_t*)start,
+    hipLaunchKernelGGL(part_accumulate_kernel, dim3((unsigned)P.nb), dim3(kPartAccThreads), 0, st, (const uint32_t*)start,
                        (const uint16_t*)rslot, (const float2*)rval, g, lp, P.nb_level, grad_table_dev);
     RM_CHECK_LAUNCH("rm_hashgrid_bwd_ws");
     return RM_OK;
