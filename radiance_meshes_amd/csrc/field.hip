@@ -163,6 +163,31 @@ constexpr int kAggLog2Slots = 12;
 constexpr int kAggSlots = 1 << kAggLog2Slots;  // > 8 corners x 256 points: an insert always finds room
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;   // table slots are < 2^31
 
+// DPP lane moves (VALU, no LDS): lanes without a source, and rows outside ROWS, read 0.
+template <int CTRL, int ROWS = 0xF>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// One step of a segmented inclusive scan over the wave: the lane combines with the partial
+// (value, segment-started) pair that CTRL brings in, unless its own segment already started.
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes; row_bcast:15 (rows 1, 3) and row_bcast:31
+// (rows 2, 3) then carry the rows' totals across the wave.
+template <int F, int CTRL, int ROWS>
+__device__ inline void seg_scan_step(float (&v)[F], int& seg) {
+    float u[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) u[f] = dpp_f<CTRL, ROWS>(v[f]);
+    const int su = dpp_i<CTRL, ROWS>(seg);
+#pragma unroll
+    for (int f = 0; f < F; ++f) v[f] = seg ? v[f] : v[f] + u[f];
+    seg |= su;
+}
+
 template <int F>
 __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const float* __restrict__ x, int64_t N,
                                                                        const float* __restrict__ gout, GridArgs g,
@@ -201,24 +226,18 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
             // consecutive lanes are neighbouring points: at coarse levels runs of lanes share a
             // slot, so a wave with at most 32 runs first sums each run (segmented scan) and only
             // the run's last lane inserts
-            const uint32_t hprev = __shfl_up(h, 1, 64);
+            const uint32_t hprev = (uint32_t)dpp_i<0x138>((int)h);   // wave_shr:1
             const bool head = lane == 0 || h != hprev;
             const uint64_t heads = __ballot(head);
             bool tail = true;
             if (__popcll(heads) <= 32) {
-                bool seg = head;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    float u[F];
-#pragma unroll
-                    for (int f = 0; f < F; ++f) u[f] = __shfl_up(v[f], o, 64);
-                    const bool su = __shfl_up((int)seg, o, 64);
-                    if (lane >= o && !seg) {
-#pragma unroll
-                        for (int f = 0; f < F; ++f) v[f] += u[f];
-                    }
-                    if (lane >= o) seg = seg || su;
-                }
+                int seg = head ? 1 : 0;
+                seg_scan_step<F, 0x111, 0xF>(v, seg);   // row_shr:1
+                seg_scan_step<F, 0x112, 0xF>(v, seg);   // row_shr:2
+                seg_scan_step<F, 0x114, 0xF>(v, seg);   // row_shr:4
+                seg_scan_step<F, 0x118, 0xF>(v, seg);   // row_shr:8
+                seg_scan_step<F, 0x142, 0xA>(v, seg);   // row_bcast:15 into rows 1 and 3
+                seg_scan_step<F, 0x143, 0xC>(v, seg);   // row_bcast:31 into rows 2 and 3
                 tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
             }
             bool nz = false;
