@@ -30,6 +30,11 @@ __device__ inline uint32_t corner_index(const GridArgs& g, int l, int cx, int cy
     return ((uint32_t)cx * 1u ^ (uint32_t)cy * 2654435761u ^ (uint32_t)cz * 805459861u) & g.mask;
 }
 
+// corner_index of a hashed level (no dense branch), for loops over levels known to be hashed.
+__device__ inline uint32_t hashed_index(const GridArgs& g, int cx, int cy, int cz) {
+    return ((uint32_t)cx * 1u ^ (uint32_t)cy * 2654435761u ^ (uint32_t)cz * 805459861u) & g.mask;
+}
+
 // Cell corner and fractions of a query point already clamped to [0, 1].
 __device__ inline void cell_of(const float v[3], const GridArgs& g, int l, int& bx, int& by, int& bz, float& fx,
                                float& fy, float& fz) {
@@ -159,8 +164,8 @@ __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restri
 // and the global atomics drop to the distinct slots a chunk touches (about a third of the
 // per-lane count on a 6.7M-point Morton-ordered query set).
 constexpr int kAggPoints = 256;
-constexpr int kAggLog2Slots = 12;
-constexpr int kAggSlots = 1 << kAggLog2Slots;  // > 8 corners x 256 points: an insert always finds room
+constexpr int kAggLog2Slots = 11;
+constexpr int kAggSlots = 1 << kAggLog2Slots;  // = 8 corners x 256 points: an insert always finds room
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;   // table slots are < 2^31
 
 // DPP lane moves (VALU, no LDS): lanes without a source, and rows outside ROWS, read 0.
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict
             cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const uint32_t h = corner_index(g, l, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
+                const uint32_t h = hashed_index(g, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
                 atomicAdd(&s_hist[(l - lp) * nb_level + (int)(h >> kPartLog2Slots)], 1u);
             }
         }
@@ -396,14 +401,19 @@ __global__ __launch_bounds__(256) void part_scatter_kernel(const float* __restri
             int bx, by, bz;
             float fx, fy, fz;
             cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+            // the 8 position claims go out together, then the 16 stores
+            uint32_t h[8], pos[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) h[c] = hashed_index(g, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                pos[c] = atomicAdd(&s_cur[(l - lp) * nb_level + (int)(h[c] >> kPartLog2Slots)], 1u);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
                 const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-                const uint32_t h = corner_index(g, l, bx + ox, by + oy, bz + oz);
-                const uint32_t pos = atomicAdd(&s_cur[(l - lp) * nb_level + (int)(h >> kPartLog2Slots)], 1u);
-                rslot[pos] = (uint16_t)(h & (kPartSlots - 1));
-                rval[pos] = make_float2(w * go.x, w * go.y);
+                rslot[pos[c]] = (uint16_t)(h[c] & (kPartSlots - 1));
+                rval[pos[c]] = make_float2(w * go.x, w * go.y);
             }
         }
     }
