@@ -216,14 +216,16 @@ __device__ inline void locate(const CamArgs& cam, int b, int& g, int& r, int& tx
     inside = tile_ok && px < cam.W && py < cam.H;
 }
 
-// LDS staging of one chunk: per slot, 4 planes, (sigma, folded base rgb), colour gradients.
-template <int WG, int F, bool BWD, bool P64>
+// LDS staging of one chunk: per slot, 4 planes, (sigma, folded base rgb), colour gradients; the
+// backward adds the tet id and, when it converts gradients per slot (VTX), the vertices and the
+// face sign bits.
+template <int WG, int F, bool BWD, bool P64, bool VTX = BWD>
 struct Stage {
     typename Prec<P64>::PL pl[4][WG];
     float4 fa[WG];                     // sigma, base' r, g, b
     float4 fg[(NG_<F>::NGRD + 3) / 4][WG];
-    float4 vtx[BWD ? 3 : 1][BWD ? WG : 1];  // v0..v3 packed (backward epilogue)
-    int meta[BWD ? 2 : 1][BWD ? WG : 1];     // tet id, face sign bits
+    float4 vtx[VTX ? 3 : 1][VTX ? WG : 1];  // v0..v3 packed (backward epilogue)
+    int meta[BWD ? (VTX ? 2 : 1) : 1][BWD ? WG : 1];  // tet id, face sign bits
 };
 
 template <typename T>
@@ -254,10 +256,10 @@ __device__ inline int plane_rec(float3 af, float3 bf, float3 cf, float3 of, floa
     return flip;
 }
 
-template <int WG, int F, bool BWD, bool P64>
-__device__ inline void stage_tet(Stage<WG, F, BWD, P64>& S, int slot, int tet, const CamArgs& cam,
+template <int WG, int F, bool BWD, bool P64, bool VTX>
+__device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
                                  const int4* __restrict__ indices, const float* __restrict__ verts,
-                                 const float* __restrict__ feats, bool stage_vtx = true) {
+                                 const float* __restrict__ feats) {
     using T = typename Prec<P64>::R;
     constexpr int NGRD = NG_<F>::NGRD;
     int4 q = indices[tet];
@@ -288,19 +290,17 @@ __device__ inline void stage_tet(Stage<WG, F, BWD, P64>& S, int slot, int tet, c
         S.fg[k][slot] = make_float4(g[4 * k], 4 * k + 1 < NGRD ? g[(4 * k + 1) % NGRD] : 0.f,
                                     4 * k + 2 < NGRD ? g[(4 * k + 2) % NGRD] : 0.f,
                                     4 * k + 3 < NGRD ? g[(4 * k + 3) % NGRD] : 0.f);
-    if constexpr (BWD) {
-        if (stage_vtx) {
-            S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
-            S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
-            S.vtx[2][slot] = make_float4(v2.z, v3.x, v3.y, v3.z);
-        }
-        S.meta[0][slot] = tet;
+    if constexpr (BWD) S.meta[0][slot] = tet;
+    if constexpr (VTX) {
+        S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
+        S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
+        S.vtx[2][slot] = make_float4(v2.z, v3.x, v3.y, v3.z);
         S.meta[1][slot] = signs;
     }
 }
 
-template <int WG, int F, bool BWD, bool P64>
-__device__ inline void read_grads(const Stage<WG, F, BWD, P64>& S, int slot, float* g) {
+template <int WG, int F, bool BWD, bool P64, bool VTX>
+__device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot, float* g) {
     constexpr int NGRD = NG_<F>::NGRD;
 #pragma unroll
     for (int k = 0; k < (NGRD + 3) / 4; ++k) {
@@ -356,8 +356,7 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
         if (!__syncthreads_or(alive)) break;
         int k = c * C::GROUP + r;
         if (k < n)
-            stage_tet<C::WG, F, false, P64>(S, threadIdx.x, (int)(sorted[rg.x + k] - tet_base), cam, indices, verts,
-                                            feats);
+            stage_tet(S, threadIdx.x, (int)(sorted[rg.x + k] - tet_base), cam, indices, verts, feats);
         __syncthreads();
         if (alive) {
             int cnt = n - c * C::GROUP;
@@ -373,7 +372,7 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
                 float x = fa.x * dt;
                 if (!(h.hit && x > 0.f)) continue;
                 float gr[NGRD];
-                read_grads<C::WG, F, false, P64>(S, slot, gr);
+                read_grads(S, slot, gr);
                 float em1 = expm1f(-x);
                 float alpha = 1.0f + em1;
                 float ph = phi_of(x, em1);
@@ -539,7 +538,7 @@ __device__ inline void bwd_outputs(const Stage<WG, F, true, P64>& S, int slot, c
     const float4 q0 = S.vtx[0][slot], q1 = S.vtx[1][slot], q2 = S.vtx[2][slot];
     const float3 V4[4] = {f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x), f3(q2.y, q2.z, q2.w)};
     float gr[NG_<F>::NGRD];
-    read_grads<WG, F, true, P64>(S, slot, gr);
+    read_grads(S, slot, gr);
     tet_grads<F, P64, NV, NOUT>(V4, S.meta[1][slot], gr, acc, cp, outv);
 }
 
@@ -604,7 +603,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     constexpr int NOUT = 12 + F;                // vertex + feature gradients per tet
     using T = typename Prec<P64>::R;
     using PL = typename Prec<P64>::PL;
-    __shared__ Stage<C::WG, F, true, P64> S;
+    __shared__ Stage<C::WG, F, true, P64, !ROWS> S;
     __shared__ int s_nc[C::NG];
     static_assert(NV <= kRowFloats, "row too small");
     // deferred per-lane epilogue of the direct path: the reduction group is the tile group, the
@@ -660,7 +659,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         __syncthreads();
         int k = c * C::GROUP + r;
         if (c < nchunks_g && k < nmax_g)
-            stage_tet<C::WG, F, true, P64>(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats, !ROWS);
+            stage_tet(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
         __syncthreads();
         // tile groups past their last chunk run no slots (cnt = 0) but still reach the barriers below
         int cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
@@ -683,7 +682,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                 if (h.hit && x > 0.f) {
                     active = true;
                     float gr[NGRD];
-                    read_grads<C::WG, F, true, P64>(S, slot, gr);
+                    read_grads(S, slot, gr);
                     float em1 = expm1f(-x);
                     float alpha = 1.0f + em1;
                     float ph = phi_of_rcp(x, em1);
@@ -820,27 +819,27 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         if (val != 0.f) unsafeAtomicAdd(rows + (int64_t)tet * kRowFloats + l, val);
                     }
                 }
-                continue;
-            }
-            float outv[NOUT];
-            bwd_outputs<C::WG, F, P64, NV, NOUT>(S, slot, acc, cp, outv);
-            const int tet = S.meta[0][slot];
-            // lane l of the reduction group adds outputs l, l + RED, ... (NOUT may exceed RED)
+            } else {
+                float outv[NOUT];
+                bwd_outputs<C::WG, F, P64, NV, NOUT>(S, slot, acc, cp, outv);
+                const int tet = S.meta[0][slot];
+                // lane l of the reduction group adds outputs l, l + RED, ... (NOUT may exceed RED)
 #pragma unroll
-            for (int l0 = 0; l0 < NOUT; l0 += C::RED) {
-                const int l = lane - red_base + l0;
-                if (l < NOUT) {
-                    float val = 0.f;
+                for (int l0 = 0; l0 < NOUT; l0 += C::RED) {
+                    const int l = lane - red_base + l0;
+                    if (l < NOUT) {
+                        float val = 0.f;
 #pragma unroll
-                    for (int i = 0; i < NOUT; ++i) val = (l == i) ? outv[i] : val;
-                    if (val != 0.f) {
-                        if (l < 12) {
-                            const int4 qi = indices[tet];
-                            const int vsel = l / 3;
-                            const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
-                            unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
-                        } else {
-                            unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                        for (int i = 0; i < NOUT; ++i) val = (l == i) ? outv[i] : val;
+                        if (val != 0.f) {
+                            if (l < 12) {
+                                const int4 qi = indices[tet];
+                                const int vsel = l / 3;
+                                const int vid = vsel == 0 ? qi.x : vsel == 1 ? qi.y : vsel == 2 ? qi.z : qi.w;
+                                unsafeAtomicAdd(gverts + 3 * (int64_t)vid + (l % 3), val);
+                            } else {
+                                unsafeAtomicAdd(gfeats + (int64_t)F * tet + (l - 12), val);
+                            }
                         }
                     }
                 }
