@@ -905,11 +905,18 @@ __global__ __launch_bounds__(256) void zero_f4_kernel(float4* __restrict__ p, in
 }
 
 // Vertex and feature gradients of each tet from its row of plane-record adjoint sums (the
-// layout of acc in composite_bwd_kernel), one thread per tet; tets whose row is all zero are
-// skipped. Feature gradients are added in place (one writer per row), vertex gradients with
-// float atomics.
+// layout of acc in composite_bwd_kernel), one thread per tet of a 256-tet workgroup; tets whose
+// row is all zero add nothing. Feature gradients are added in place (one writer per row). Vertex
+// gradients are first summed per workgroup in an LDS table keyed by vertex id (FP64 sums): the
+// tets of a Morton-ordered mesh that share a workgroup share most of their vertices (about 120
+// distinct vertices per 1024 corners), so one float atomic per distinct (vertex, axis) follows.
+constexpr int kTetGradTets = 256;
+constexpr int kTetGradLog2Slots = 10;
+constexpr int kTetGradSlots = 1 << kTetGradLog2Slots;   // = 4 corners x 256 tets: an insert always finds room
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
+
 template <int F, bool P64>
-__global__ __launch_bounds__(256) void composite_tet_grads_kernel(
+__global__ __launch_bounds__(kTetGradTets) void composite_tet_grads_kernel(
     const float4* __restrict__ rows, int64_t n_tets, const int4* __restrict__ indices, const float* __restrict__ verts,
     const float* __restrict__ feats, float3 cp, float* __restrict__ gverts, float* __restrict__ gfeats) {
     using T = typename Prec<P64>::R;
@@ -919,44 +926,72 @@ __global__ __launch_bounds__(256) void composite_tet_grads_kernel(
     constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;
     constexpr int NOUT = 12 + F;
     static_assert(NV % 4 == 0 && NV <= kRowFloats, "row layout");
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tets) return;
-    float acc[NV];
-    bool any = false;
-    float row[kRowFloats];
-#pragma unroll
-    for (int k = 0; k < kRowFloats / 4; ++k) {
-        const float4 v = rows[t * (kRowFloats / 4) + k];
-        row[4 * k] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
-        any = any || v.x != 0.f || v.y != 0.f || v.z != 0.f || v.w != 0.f;
+    static_assert(kTetGradSlots >= 4 * kTetGradTets, "vertex table too small");
+    __shared__ uint32_t s_vid[kTetGradSlots];
+    __shared__ double s_g[3][kTetGradSlots];
+    for (int p = threadIdx.x; p < kTetGradSlots; p += kTetGradTets) {
+        s_vid[p] = kNoVertex;
+        s_g[0][p] = 0.0; s_g[1][p] = 0.0; s_g[2][p] = 0.0;
     }
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * kTetGradTets + threadIdx.x;
+    bool any = false;
+    float acc[NV];
+    if (t < n_tets) {
+        float row[kRowFloats];
 #pragma unroll
-    for (int pos = 0; pos < kRowFloats; ++pos)
-        if (row_sum_index(pos) < NV) acc[row_sum_index(pos)] = row[pos];
-    if (!any) return;
-    const int4 q = indices[t];
-    const float3 V4[4] = {load_v(verts, q.x), load_v(verts, q.y), load_v(verts, q.z), load_v(verts, q.w)};
-    PL unused;
-    int signs = 0;
-    signs |= plane_rec<T>(V4[0], V4[1], V4[2], V4[3], cp, unused) << 0;
-    signs |= plane_rec<T>(V4[0], V4[1], V4[3], V4[2], cp, unused) << 1;
-    signs |= plane_rec<T>(V4[0], V4[2], V4[3], V4[1], cp, unused) << 2;
-    signs |= plane_rec<T>(V4[1], V4[2], V4[3], V4[0], cp, unused) << 3;
-    float* gf = gfeats + (int64_t)F * t;
-    const float* f = feats + (int64_t)F * t;
-    float gr[NGRD];
+        for (int k = 0; k < kRowFloats / 4; ++k) {
+            const float4 v = rows[t * (kRowFloats / 4) + k];
+            row[4 * k] = v.x; row[4 * k + 1] = v.y; row[4 * k + 2] = v.z; row[4 * k + 3] = v.w;
+            any = any || v.x != 0.f || v.y != 0.f || v.z != 0.f || v.w != 0.f;
+        }
 #pragma unroll
-    for (int k = 0; k < NGRD; ++k) gr[k] = f[4 + k];
-    float outv[NOUT];
-    tet_grads<F, P64, NV, NOUT>(V4, signs, gr, acc, cp, outv);
+        for (int pos = 0; pos < kRowFloats; ++pos)
+            if (row_sum_index(pos) < NV) acc[row_sum_index(pos)] = row[pos];
+    }
+    if (any) {
+        const int4 q = indices[t];
+        const float3 V4[4] = {load_v(verts, q.x), load_v(verts, q.y), load_v(verts, q.z), load_v(verts, q.w)};
+        PL unused;
+        int signs = 0;
+        signs |= plane_rec<T>(V4[0], V4[1], V4[2], V4[3], cp, unused) << 0;
+        signs |= plane_rec<T>(V4[0], V4[1], V4[3], V4[2], cp, unused) << 1;
+        signs |= plane_rec<T>(V4[0], V4[2], V4[3], V4[1], cp, unused) << 2;
+        signs |= plane_rec<T>(V4[1], V4[2], V4[3], V4[0], cp, unused) << 3;
+        float* gf = gfeats + (int64_t)F * t;
+        const float* f = feats + (int64_t)F * t;
+        float gr[NGRD];
 #pragma unroll
-    for (int k = 0; k < F; ++k) gf[k] += outv[12 + k];
-    const int vid[4] = {q.x, q.y, q.z, q.w};
+        for (int k = 0; k < NGRD; ++k) gr[k] = f[4 + k];
+        float outv[NOUT];
+        tet_grads<F, P64, NV, NOUT>(V4, signs, gr, acc, cp, outv);
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+        for (int k = 0; k < F; ++k) gf[k] += outv[12 + k];
+        const uint32_t vid[4] = {(uint32_t)q.x, (uint32_t)q.y, (uint32_t)q.z, (uint32_t)q.w};
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (outv[3 * v + k] != 0.f) unsafeAtomicAdd(gverts + 3 * (int64_t)vid[v] + k, outv[3 * v + k]);
+        for (int v = 0; v < 4; ++v) {
+            uint32_t p = (vid[v] * 2654435761u) >> (32 - kTetGradLog2Slots);
+            for (int probe = 0; probe < kTetGradSlots; ++probe) {
+                const uint32_t k = atomicCAS(&s_vid[p], kNoVertex, vid[v]);
+                if (k == kNoVertex || k == vid[v]) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) atomicAdd(&s_g[a][p], (double)outv[3 * v + a]);
+                    break;
+                }
+                p = (p + 1) & (kTetGradSlots - 1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < kTetGradSlots; p += kTetGradTets) {
+        const uint32_t k = s_vid[p];
+        if (k == kNoVertex) continue;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float v = (float)s_g[a][p];
+            if (v != 0.f) unsafeAtomicAdd(gverts + 3 * (int64_t)k + a, v);
+        }
+    }
 }
 
 template <int TS, int F, bool P64, bool ROWS>
@@ -974,7 +1009,8 @@ int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
                        n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
                        (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats, rows);
     if (ROWS && n_tets > 0)
-        hipLaunchKernelGGL((composite_tet_grads_kernel<F, P64>), dim3((unsigned)ceil_div(n_tets, 256)), dim3(256), 0,
+        hipLaunchKernelGGL((composite_tet_grads_kernel<F, P64>), dim3((unsigned)ceil_div(n_tets, kTetGradTets)),
+                           dim3(kTetGradTets), 0,
                            st, (const float4*)rows, n_tets, (const int4*)indices, verts, feats,
                            make_float3(a.cam_pos[0], a.cam_pos[1], a.cam_pos[2]), gverts, gfeats);
     return 0;
