@@ -314,6 +314,46 @@ __device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot
 
 // ------------------------------------------------------------------------------- forward --
 
+// Blend of one hit into the pixel state (integrate_tetra_segment and the distortion moments);
+// returns true when the pixel terminates (log-transmittance below -5.5) after this tet.
+template <int WG, int F, typename St>
+__device__ __forceinline__ bool blend_step(const St& S, int slot, const Hit& h, float rdx, float rdy, float rdz,
+                                           float mt, float& cr, float& cg, float& cb, float& logT, float& X,
+                                           float& Y, float& Z, float& W, float& V) {
+    constexpr int NGRD = NG_<F>::NGRD;
+    const float4 fa = S.fa[slot];
+    const float dt = h.dt;
+    const float x = fa.x * dt;
+    if (!(h.hit && x > 0.f)) return false;
+    float gr[NGRD];
+    read_grads(S, slot, gr);
+    const float em1 = expm1f(-x);
+    const float alpha = 1.0f + em1;
+    const float ph = phi_of(x, em1);
+    const float w0 = ph - alpha, w1 = 1.0f - ph;
+    const float T = __expf(logT);
+    const float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
+    float gd1 = gd0, gd2 = gd0;
+    if (F == 13) {
+        gd1 = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
+        gd2 = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
+    }
+    // w0 * c(s_x) + w1 * c(s_e) with c(s) = base' + s * gd
+    const float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
+    cr += T * (wsum * fa.y + wsl * gd0);
+    cg += T * (wsum * fa.z + wsl * gd1);
+    cb += T * (wsum * fa.w + wsl * gd2);
+    logT -= x;
+    const float te = h.se - mt, tx_ = h.sx - mt;
+    const float m1 = fa.x * (te + tx_) * 0.5f;
+    X += 2.0f * m1 * Z;
+    Y += 2.0f * fa.x * W;
+    Z += fa.x;
+    W += m1;
+    V += fa.x * fa.x * dt * (1.0f / 3.0f);
+    return logT < -5.5f;
+}
+
 // Forward of the tiles of workgroup b of one view. Tile ids are offset by tile_base, the sorted
 // list holds tet ids offset by tet_base, outputs start at pixel pix_base; jitter may be null
 // (pixel centres).
@@ -362,42 +402,23 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
             int cnt = n - c * C::GROUP;
             cnt = cnt < C::GROUP ? cnt : C::GROUP;
             const int base = g * C::GROUP;
-            for (int jj = 0; jj < cnt; ++jj) {
-                const int slot = base + jj;
+            // two slots per step: both intersections (independent of the pixel state) first,
+            // then the two blends in list order
+            for (int jj = 0; jj < cnt; jj += 2) {
+                const bool two = jj + 1 < cnt;
+                const int s0 = base + jj, s1 = two ? s0 + 1 : s0;
+                const PL p0[4] = {S.pl[0][s0], S.pl[1][s0], S.pl[2][s0], S.pl[3][s0]};
+                const PL p1[4] = {S.pl[0][s1], S.pl[1][s1], S.pl[2][s1], S.pl[3][s1]};
+                const Hit h0 = intersect(p0, ray, mt);
+                const Hit h1 = intersect(p1, ray, mt);
                 ++nc;
-                const PL pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
-                Hit h = intersect(pl, ray, mt);
-                float4 fa = S.fa[slot];
-                float dt = h.dt;
-                float x = fa.x * dt;
-                if (!(h.hit && x > 0.f)) continue;
-                float gr[NGRD];
-                read_grads(S, slot, gr);
-                float em1 = expm1f(-x);
-                float alpha = 1.0f + em1;
-                float ph = phi_of(x, em1);
-                float w0 = ph - alpha, w1 = 1.0f - ph;
-                float T = __expf(logT);
-                float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
-                float gd1 = gd0, gd2 = gd0;
-                if (F == 13) {
-                    gd1 = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
-                    gd2 = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
+                if (blend_step<C::WG, F>(S, s0, h0, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V)) {
+                    alive = false;
+                    break;
                 }
-                // w0 * c(s_x) + w1 * c(s_e) with c(s) = base' + s * gd
-                float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
-                cr += T * (wsum * fa.y + wsl * gd0);
-                cg += T * (wsum * fa.z + wsl * gd1);
-                cb += T * (wsum * fa.w + wsl * gd2);
-                logT -= x;
-                float te = h.se - mt, tx_ = h.sx - mt;
-                float m1 = fa.x * (te + tx_) * 0.5f;
-                X += 2.0f * m1 * Z;
-                Y += 2.0f * fa.x * W;
-                Z += fa.x;
-                W += m1;
-                V += fa.x * fa.x * dt * (1.0f / 3.0f);
-                if (logT < -5.5f) {
+                if (!two) break;
+                ++nc;
+                if (blend_step<C::WG, F>(S, s1, h1, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V)) {
                     alive = false;
                     break;
                 }
