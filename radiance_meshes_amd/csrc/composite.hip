@@ -686,7 +686,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         int cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
         cnt = cnt < C::GROUP ? cnt : C::GROUP;
         const int base = g * C::GROUP;
-        for (int jj = cnt - 1; jj >= 0; --jj) {
+        // slot body for slot jj with its intersection h (computed ahead, see below)
+        auto slot_body = [&](int jj, const Hit& h) {
             const int slot = base + jj;
             const int kk = c * C::GROUP + jj;
             float acc[NV];
@@ -694,8 +695,6 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             for (int i = 0; i < NV; ++i) acc[i] = 0.f;
             bool active = false;
             if (inside && kk < nc) {
-                const PL pl[4] = {S.pl[0][slot], S.pl[1][slot], S.pl[2][slot], S.pl[3][slot]};
-                Hit h = intersect(pl, ray, mt);
                 float4 fa = S.fa[slot];
                 const float sig = fa.x;
                 float dt = h.dt;
@@ -802,7 +801,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                 }
             }
             uint64_t any = __ballot(active) & red_mask;
-            if (any == 0) continue;
+            if (any == 0) return;
             if constexpr (ROWS && C::RED == 16) {
                 // transposed group sum: lane r ends with row positions r and 16 + r
                 float a32[kRowFloats];
@@ -813,7 +812,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                 float* row = rows + (int64_t)S.meta[0][slot] * kRowFloats;
                 if (o0 != 0.f) unsafeAtomicAdd(row + r, o0);
                 if (o1 != 0.f) unsafeAtomicAdd(row + 16 + r, o1);
-                continue;
+                return;
             }
             group_sum<NV>(acc, C::RED);
             if constexpr (kDefer) {
@@ -824,7 +823,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     for (int i = 0; i < NV; ++i) kept[i] = acc[i];
                     kept_on = true;
                 }
-                continue;
+                return;
             }
             if constexpr (ROWS) {
                 // lane l of the reduction group adds row position l (the sum row_sum_index(l))
@@ -865,6 +864,17 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     }
                 }
             }
+        };
+        // two slots per step: both intersections (independent of the pixel state) first, then the
+        // two slots back to front
+        for (int jj = cnt - 1; jj >= 0; jj -= 2) {
+            const int s0 = base + jj, s1 = jj > 0 ? s0 - 1 : s0;
+            const PL p0[4] = {S.pl[0][s0], S.pl[1][s0], S.pl[2][s0], S.pl[3][s0]};
+            const PL p1[4] = {S.pl[0][s1], S.pl[1][s1], S.pl[2][s1], S.pl[3][s1]};
+            const Hit h0 = intersect(p0, ray, mt);
+            const Hit h1 = intersect(p1, ray, mt);
+            slot_body(jj, h0);
+            if (jj > 0) slot_body(jj - 1, h1);
         }
         if constexpr (kDefer && !ROWS) {
             // each lane turns its kept slot's sums into the tet's gradients; the gradients then go
