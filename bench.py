@@ -114,6 +114,34 @@ def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats):
     return int(nc.sum().item())
 
 
+def time_tet_err(indices, vertices, tcam, rg, sidx, ranges, feats, reps=3):
+    """Per-tet error votes (render_err's calc_tet_err) of the benchmark view, timed with HIP events
+    on the stream they run on; pixel and SSIM errors are seeded uniform fields."""
+    import torch
+    from radiance_meshes_amd.render_err import calc_tet_err
+    H, W = rg.image_height, rg.image_width
+    g = torch.Generator(device="cuda").manual_seed(2)
+    pe = torch.rand((H, W), device="cuda", generator=g) - 0.5
+    se = torch.rand((H, W), device="cuda", generator=g)
+    T = indices.shape[0]
+    err = torch.zeros((T, 16), device="cuda")
+    cnt = torch.zeros((T, 2), device="cuda", dtype=torch.int32)
+    calc_tet_err(sidx, ranges, indices, vertices, feats, rg, tcam, pe, se, err, cnt)   # warm-up
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+    st = torch.cuda.current_stream()
+    for i in range(reps):
+        err.zero_()
+        cnt.zero_()
+        ev[2 * i].record(st)
+        calc_tet_err(sidx, ranges, indices, vertices, feats, rg, tcam, pe, se, err, cnt)
+        ev[2 * i + 1].record(st)
+    torch.cuda.synchronize()
+    ms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps)]
+    voted_tets = int((cnt[:, 0] > 0).sum().item())
+    # algorithmic output: one 64-byte vote row and one 8-byte count row per voted tet
+    return {"mean_ms": round(sum(ms) / reps, 4), "voted_tets": voted_tets, "votes": int(cnt[:, 0].sum().item())}
+
+
 def measured_traffic(workload, tile, precision, span):
     """HBM bytes per launch of `span`'s kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
     recorded for this workload (profiles/traffic_*.json), or None when none was recorded."""
@@ -186,6 +214,7 @@ def run_config(args, pts, tets, cam, tile):
     P = int(stats["P"])
     G = rg.grid_height * rg.grid_width
     N = H * W
+    votes = time_tet_err(indices, vertices.detach(), tcam, rg, *stats["last"], reps=3)
     V = vertices.shape[0]
 
     # algorithmic bytes per launch (FP32 values, int32 indices)
@@ -216,6 +245,9 @@ def run_config(args, pts, tets, cam, tile):
     return {"value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
             "hbm_frac_all_kernels": round(total_algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "breakdown_ms": breakdown,
+            "tet_err": {**votes, "algorithmic_bytes": 96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"],
+                        "achieved_gbs": round((96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"])
+                                              / (votes["mean_ms"] * 1e-3) / 1e9, 2)},
             "config": {"workload": workload, "points": int(V),
                        "tets": int(T), "visible_tets": n_vis, "tile_pairs": P, "evaluations": n_eval,
                        "width": W, "height": H, "tile": tile,
@@ -349,6 +381,7 @@ def main():
         "roofline": res["roofline"],
         "hbm_frac_all_kernels": res["hbm_frac_all_kernels"],
         "breakdown_ms": res["breakdown_ms"],
+        "tet_err": res.get("tet_err"),
         "cpu_baseline": cpu,
         "serving": serving,
     }

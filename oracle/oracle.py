@@ -70,6 +70,7 @@ def lib():
         L.orc_power_of_circumsphere.argtypes = [P, P, P, P, P, P]
         L.orc_power_of_circumsphere.restype = C.c_double
         L.orc_power_bwd.argtypes = [P, P, C.c_double, P, P]
+        L.orc_tet_err.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -116,6 +117,23 @@ def composite_fwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, 
                             _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F, _p(j),
                             _p(img), _p(dist), _p(nc), _p(margin))
     return img, dist, nc, margin
+
+
+def tet_err(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, pixel_err, ssim_err, graze=False):
+    """FP64 per-tet error votes of calc_tet_err: (err [T,16], count [T,2] int32, margin [H,W]);
+    with graze=True also grazes [T] int32, the pixels hitting each tet with a chord below 1e-12."""
+    T = indices.shape[0]
+    F = feats.shape[1]
+    err = np.zeros((T, 16))
+    count = np.zeros((T, 2), np.int32)
+    margin = np.zeros((cam.H, cam.W))
+    gz = np.zeros((T,), np.int32)
+    lib().orc_tet_err(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
+                      _p(np.ascontiguousarray(ranges, np.int32)), _p(np.ascontiguousarray(indices, np.int32)),
+                      _p(np.ascontiguousarray(vertices, np.float64)), _p(np.ascontiguousarray(feats, np.float64)),
+                      F, _p(np.ascontiguousarray(pixel_err, np.float64)),
+                      _p(np.ascontiguousarray(ssim_err, np.float64)), _p(err), _p(count), _p(margin), _p(gz))
+    return (err, count, margin, gz) if graze else (err, count, margin)
 
 
 def composite_bwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter, img, dist,

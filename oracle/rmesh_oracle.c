@@ -24,6 +24,8 @@
  *                      base + (gx,gy,gz).(p - v0) (tetra_quad.py:192-195 reads indices 4..6 of a
  *                      6-wide array; JAX clamps the gather, so it uses (gy,gz,gz)).
  *   orc_project_bwd    FP64 adjoint of the circumsphere power / circumcenter map.
+ *   orc_tet_err        FP64 per-tet error votes of calc_tet_err (alphablend_shader_interp.slang:
+ *                      433-599, render_err.py:138-150).
  *
  * Pinned by tests/test_oracle_*.py against closed forms, brute force and finite differences.
  */
@@ -656,6 +658,73 @@ void orc_composite_bwd(const orc_camera* c, const int32_t* sorted, const int32_t
                 dA = dA + dT * T;
                 dZ = dZn; dW = dWn;
             }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ FP64 error votes ----- */
+
+/* Per-tet error votes of one view: kern_calc_tet_err / calc_tet_err
+ * (alphablend_shader_interp.slang:433-599) as launched by render_err (render_err.py:138-150),
+ * in double. Pixel-centre rays (:435-450, ray jitter 0.5), the same per-tile lists and cull set
+ * as orc_composite_fwd, log-transmittance starting at 1 (curr_pixel_state = (0,0,0,1), :450),
+ * w = exp(logT) * sigma * dt for each blended tet (hit and sigma*dt > 0, :487-491), votes
+ * (:497-525) into err[T,16] and count[T,2] (count, max int(65535 w)), termination after a blend
+ * with logT < -5.5 (:529). Pixels are visited in raster order, serially (deterministic sums).
+ * margin[H*W] (optional) receives min |logT + 5.5| after any blend: the distance of the
+ * pixel's termination decision from the threshold; grazes[T] (optional) counts, per tet, the
+ * pixels whose ray hits it with a chord dt < 1e-12 before stopping (a vote needs sigma*dt > 0,
+ * so such a hit votes or not depending on the precision of the crossing distances). */
+void orc_tet_err(const orc_camera* c, const int32_t* sorted, const int32_t* ranges, const int32_t* idx,
+                 const double* verts, const double* feats, int F, const double* pixel_err,
+                 const double* ssim_err, double* err, int32_t* count, double* margin, int32_t* grazes) {
+    int H = c->H, W = c->W;
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            int tile = (y / c->th) * c->gw + (x / c->tw);
+            int s0 = ranges[2 * tile], s1 = ranges[2 * tile + 1];
+            int64_t pix = (int64_t)y * W + x;
+            ray_d r = get_ray_d(c, (double)x + 0.5, (double)y + 0.5);
+            double A = 1.0, mg = 1e30;
+            double pe = pixel_err[pix], ps = ssim_err[pix];
+            for (int k = s0; k < s1; ++k) {
+                int32_t tet = sorted[k];
+                double v[4][3];
+                load_tet_d(idx, verts, tet, v);
+                const double* f = feats + (int64_t)F * tet;
+                isect_d s = intersect_d(v, &r);
+                if (!s.hit) continue;
+                double sig = f[0] > 0.0 ? f[0] : 0.0;
+                double dt = s.exit - s.enter;
+                if (dt < 0) dt = 0;
+                if (dt < 1e-12 && grazes) grazes[tet] += 1;
+                double xx = sig * dt;
+                if (!(xx > 0.0)) continue;
+                double w = exp(A) * xx;
+                double* e = err + 16 * (int64_t)tet;
+                e[0] += w;
+                e[1] += w * pe;
+                e[2] += w * pe * pe;
+                e[5] += w * ps;
+                for (int i = 0; i < 3; ++i) {
+                    double p1 = s.enter * r.d[i] + r.o[i];
+                    double p2 = s.exit * r.d[i] + r.o[i];
+                    double pc = 0.5 * (s.enter + s.exit) * r.d[i] + r.o[i];
+                    e[6 + i] += w * p1;
+                    e[9 + i] += w * p2;
+                    e[13 + i] += w * pc;
+                }
+                e[12] += w;
+                count[2 * (int64_t)tet] += 1;
+                double wi = 65535.0 * w;
+                int32_t iw = wi >= 2147483647.0 ? 2147483647 : (int32_t)wi;
+                if (iw > count[2 * (int64_t)tet + 1]) count[2 * (int64_t)tet + 1] = iw;
+                A -= xx;
+                double am = fabs(A + 5.5);
+                if (am < mg) mg = am;
+                if (A < -5.5) break;
+            }
+            if (margin) margin[pix] = mg;
         }
     }
 }

@@ -22,6 +22,7 @@ def install_reference_aliases():
         "delaunay_rasterization.internal.tile_shader_slang",
         "delaunay_rasterization.internal.alphablend_tiled_slang_interp",
         "delaunay_rasterization.internal.sort_by_keys",
+        "delaunay_rasterization.internal.render_err",
     ]
     for n in names:
         sys.modules[n] = importlib.import_module("radiance_meshes_amd." + n)

@@ -1047,6 +1047,145 @@ int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
     return 0;
 }
 
+// ------------------------------------------------------------------------- error votes --
+
+// Integer sum / max over the reduction group (the DPP row pattern of group_sum).
+template <bool MAX>
+__device__ inline int group_red_i(int x, int width) {
+    auto op = [](int a, int b) { return MAX ? (a > b ? a : b) : a + b; };
+    x = op(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    x = op(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    x = op(x, __builtin_amdgcn_update_dpp(0, x, 0x124, 0xF, 0xF, false));
+    x = op(x, __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+    if (width > 16) {
+        x = op(x, __shfl_xor(x, 16, 64));
+        x = op(x, __shfl_xor(x, 32, 64));
+    }
+    return x;
+}
+
+// Per-tet error votes of one view: kern_calc_tet_err / calc_tet_err
+// (alphablend_shader_interp.slang:433-599), launched by render_err (render_err.py:138-150).
+// Each pixel ray (pixel centre) walks its tile's list front to back starting from
+// log-transmittance 1 -- that kernel's initial state, unlike splat_tiled's 0 -- and for every
+// blended tet (hit with sigma*dt > 0) votes w = exp(logT) * sigma * dt:
+//   tet_err[t] += (w, w*e, w*e^2, 0, 0, w*s, w*p_in, w*p_out, w, w*(p_in + p_out)/2)
+//   tet_count[t] = (count + 1, max(count_max, int(65535 w)))
+// with e / s the pixel's L1 / SSIM error and p_in / p_out the segment's end points; the walk
+// stops after a blend that takes log-transmittance below -5.5. Every lane of a reduction group
+// (one 4x4 tile per 16-lane DPP row, or a 64-lane wave of a larger tile) visits the same slot at
+// the same step -- terminated lanes vote zero -- so the group's votes are summed with DPP row
+// ops and lane k of the group adds channel k: one 64-byte atomic segment per (tile, tet) and
+// group instead of 15 atomics per (pixel, tet).
+template <int TS, int F, bool P64>
+__global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float* __restrict__ pixel_err, const float* __restrict__ ssim_err, float* __restrict__ tet_err,
+    int* __restrict__ tet_count) {
+    using C = Cfg<TS>;
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    __shared__ Stage<C::WG, F, true, P64, false> S;
+    __shared__ int s_chunks;
+    int g, r, tx, ty, px, py;
+    bool tile_ok, inside;
+    locate<TS>(cam, remap_block(blockIdx.x, gridDim.x), g, r, tx, ty, px, py, tile_ok, inside);
+    const int2 rg = load_range(ranges, tile_ok, ty * cam.gw + tx, n_sorted);
+    const int n = rg.y - rg.x;
+    if (threadIdx.x == 0) s_chunks = 0;
+    __syncthreads();
+    if (r == 0) atomicMax(&s_chunks, (n + C::GROUP - 1) / C::GROUP);
+    const int64_t pix = (int64_t)py * cam.W + px;
+    RayT<T> ray = {0, 0, 1};
+    float perr = 0.f, serr = 0.f;
+    if (inside) {
+        ray = make_ray<T>(cam, px, py, 0.5f, 0.5f);
+        perr = pixel_err[pix];
+        serr = ssim_err[pix];
+    }
+    const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
+    const float cx = cam.cam_pos[0], cy = cam.cam_pos[1], cz = cam.cam_pos[2];
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (C::RED - 1);          // lane within the reduction group
+    const int gshift = C::RED == 16 ? (lane & 48) : 0;
+    float logT = 1.0f;
+    bool alive = inside;
+    __syncthreads();
+    const int nchunks = s_chunks;
+    for (int c = 0; c < nchunks; ++c) {
+        if (!__syncthreads_or(alive)) break;
+        const int k = c * C::GROUP + r;
+        if (k < n) stage_tet(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+        __syncthreads();
+        int cnt = n - c * C::GROUP;
+        cnt = cnt < C::GROUP ? cnt : C::GROUP;
+        const int base = g * C::GROUP;
+        // the trip count is uniform over each reduction group (one tile), so every lane of a
+        // group reaches the ballot and the DPP sums of each step
+        for (int j = 0; j < cnt; ++j) {
+            const int s = base + j;
+            float w = 0.f, se = 0.f, sx = 0.f;
+            int hit = 0;
+            if (alive) {
+                const PL p[4] = {S.pl[0][s], S.pl[1][s], S.pl[2][s], S.pl[3][s]};
+                const Hit h = intersect(p, ray, cam.min_t);
+                const float x = S.fa[s].x * h.dt;
+                if (h.hit && x > 0.f) {
+                    w = expf(logT) * x;
+                    se = h.se;
+                    sx = h.sx;
+                    hit = 1;
+                    logT -= x;
+                    alive = !(logT < -5.5f);
+                }
+            }
+            const uint64_t votes = (__ballot(hit) >> gshift) & (C::RED == 16 ? 0xFFFFull : ~0ull);
+            if (votes == 0) continue;
+            float v[10] = {w, w * perr, w * perr * perr, w * serr,
+                           w * (cx + se * rdx), w * (cy + se * rdy), w * (cz + se * rdz),
+                           w * (cx + sx * rdx), w * (cy + sx * rdy), w * (cz + sx * rdz)};
+            group_sum(v, C::RED);
+            const int n_hit = __popcll(votes);
+            const int wmax = group_red_i<true>(hit ? (int)(65535.f * w) : 0, C::RED);
+            if (gl < 16) {
+                const int tet = S.meta[0][s];
+                float val;
+                switch (gl) {
+                    case 0: case 12: val = v[0]; break;
+                    case 1: val = v[1]; break;
+                    case 2: val = v[2]; break;
+                    case 5: val = v[3]; break;
+                    case 6: val = v[4]; break;
+                    case 7: val = v[5]; break;
+                    case 8: val = v[6]; break;
+                    case 9: val = v[7]; break;
+                    case 10: val = v[8]; break;
+                    case 11: val = v[9]; break;
+                    case 13: val = 0.5f * (v[4] + v[7]); break;
+                    case 14: val = 0.5f * (v[5] + v[8]); break;
+                    case 15: val = 0.5f * (v[6] + v[9]); break;
+                    default: val = 0.f; break;   // channels 3 and 4 carry no vote
+                }
+                if (gl != 3 && gl != 4) unsafeAtomicAdd(tet_err + 16 * (int64_t)tet + gl, val);
+                if (gl == 0) atomicAdd(tet_count + 2 * (int64_t)tet, n_hit);
+                if (gl == 1) atomicMax(tet_count + 2 * (int64_t)tet + 1, wmax);
+            }
+        }
+    }
+}
+
+template <int TS, int F, bool P64>
+int launch_tet_err(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices,
+                   const float* verts, const float* feats, const float* perr, const float* serr, float* err,
+                   int* count, hipStream_t st) {
+    using C = Cfg<TS>;
+    int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
+    hipLaunchKernelGGL((tet_err_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted, n_sorted,
+                       (const int2*)ranges, (const int4*)indices, verts, feats, perr, serr, err, count);
+    return 0;
+}
+
 int check_common(const rm_camera* cam, int32_t nf, const void* sorted, const void* ranges, const void* indices,
                  const void* verts, const void* feats, const void* jitter, const char* who) {
     std::string w(who);
@@ -1222,5 +1361,34 @@ extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_t
 #undef RM_BWD_P
 #undef RM_BWD
     RM_CHECK_LAUNCH("composite_bwd_kernel");
+    return RM_OK;
+}
+
+extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                          const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
+                          const float* features_dev, int32_t n_features, const float* pixel_err_dev,
+                          const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev,
+                          rm_stream_t stream) {
+    RM_CHECK_ARG(n_sorted >= 0, "rm_tet_err: negative n_sorted");
+    if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
+    // the votes use pixel-centre rays: any non-null, aligned pointer stands in for the jitter check
+    int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
+                         ranges_dev, "rm_tet_err");
+    if (e) return e;
+    RM_CHECK_ARG(pixel_err_dev && ssim_err_dev && tet_err_dev && tet_count_dev, "rm_tet_err: null buffer");
+    CamArgs a = make_cam_args(*cam);
+    hipStream_t st = as_stream(stream);
+    const int ts = cam->tile_width;
+#define RM_ERRV(TS, F, P) launch_tet_err<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
+                                        features_dev, pixel_err_dev, ssim_err_dev, tet_err_dev, tet_count_dev, st)
+#define RM_ERRV_P(TS, F) (g_fp64_planes ? RM_ERRV(TS, F, true) : RM_ERRV(TS, F, false))
+    if (n_features == 7) {
+        if (ts == 4) RM_ERRV_P(4, 7); else if (ts == 8) RM_ERRV_P(8, 7); else RM_ERRV_P(16, 7);
+    } else {
+        if (ts == 4) RM_ERRV_P(4, 13); else if (ts == 8) RM_ERRV_P(8, 13); else RM_ERRV_P(16, 13);
+    }
+#undef RM_ERRV_P
+#undef RM_ERRV
+    RM_CHECK_LAUNCH("tet_err_kernel");
     return RM_OK;
 }

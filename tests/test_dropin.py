@@ -9,6 +9,9 @@ def test_reference_module_paths_resolve():
     from delaunay_rasterization.internal.sort_by_keys import sort_by_keys_cub
     from delaunay_rasterization.internal.tile_shader_slang import VertexShader, vertex_and_tile_shader
     from delaunay_rasterization.internal.util import recombine_tensors, split_tensors
+    from delaunay_rasterization.internal.render_err import render_err
+    from radiance_meshes_amd import render_err as render_err_mod
+    assert render_err is render_err_mod.render_err
     from radiance_meshes_amd import rasterizer
     assert AlphaBlendTiledRender is rasterizer.AlphaBlendTiledRender
     assert vertex_and_tile_shader is rasterizer.vertex_and_tile_shader
