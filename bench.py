@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--cpu-baseline-rays", type=int, default=32)
     p.add_argument("--cpu-baseline-samples", type=int, default=128)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-precision-report", action="store_true",
+                   help="skip the BF16-vs-FP64 field MLP report on the benchmark mesh")
     p.add_argument("--no-layout", action="store_true", help="keep the Delaunay order of vertices and tets")
     p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
     p.add_argument("--profile-steps", type=int, default=0,
@@ -256,6 +258,68 @@ def run_config(args, pts, tets, cam, tile):
                        "parallelism": "single_gpu"}}
 
 
+def precision_report(pts, tets, cam, tile):
+    """Field MLP with BF16 operands (FP32 accumulation) against an FP64 evaluation of the same
+    weights on the same hash encodings, over every visible tet centroid of the benchmark view;
+    images composited in FP32 from either feature set. The hash table is drawn U(-1, 1) (seed 3)
+    so that the encodings, and the field, vary across the mesh; MLP weights N(0, 1/fan_in)."""
+    import torch
+    import torch.nn.functional as Fn
+    from radiance_meshes_amd import field as fieldmod
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    from radiance_meshes_amd.rasterizer import RenderGrid, _cam, vertex_and_tile_shader
+    dev = "cuda"
+    vertices = torch.from_numpy(pts).to(dev)
+    indices = torch.from_numpy(tets).to(dev)
+    H, W = cam.height, cam.width
+    tcam = {k: (v.to(dev) if hasattr(v, "to") else v) for k, v in cam.tcam(tile, 0.1).items()}
+    rg = RenderGrid(H, W, tile, tile)
+    fld = fieldmod.Field(n_features=13, precision="fp32", scale=2.0, device=dev)
+    with torch.no_grad():
+        fld.grid.table.copy_(torch.rand(fld.grid.table.shape, generator=torch.Generator().manual_seed(3)) * 2 - 1)
+        sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(indices, vertices, tcam, rg)
+        tet_ids = torch.nonzero(mask).reshape(-1).to(torch.int32)
+        h = fld.grid(fieldmod.field_points(vertices, indices, tet_ids, fld.scale))
+        m = fld.mlp
+        raw32 = m(h)
+        m.precision = 1
+        raw16 = m(h)
+        m.precision = 0
+        w = [t.double() for t in (m.w1, m.b1, m.w2, m.b2, m.w3, m.b3)]
+        raw64 = torch.empty((h.shape[0], 13), dtype=torch.float64, device=dev)
+        for a in range(0, h.shape[0], 1 << 20):
+            z = Fn.selu(h[a:a + (1 << 20)].double() @ w[0].T + w[1])
+            z = Fn.selu(z @ w[2].T + w[3])
+            raw64[a:a + (1 << 20)] = z @ w[4].T + w[5]
+
+        def params(raw):
+            d = torch.exp(raw[:, 0].double()), torch.exp(raw64[:, 0])
+            dens_rel = ((d[0] - d[1]).abs() / d[1]).float()
+            col = (raw[:, 1:].double() - raw64[:, 1:]).abs().float()
+            return {"density_rel_max": float(dens_rel.max()), "density_rel_mean": float(dens_rel.mean()),
+                    "colour_abs_max": float(col.max()), "colour_abs_mean": float(col.mean())}
+
+        def image(raw):
+            feats = torch.zeros((indices.shape[0], 13), device=dev)
+            feats[tet_ids.long()] = fieldmod.activate(raw.float(), 13)
+            img = torch.empty((H, W, 4), device=dev)
+            dist = torch.empty((H, W, 5), device=dev)
+            nc = torch.empty((H, W), device=dev, dtype=torch.int32)
+            jit = torch.full((H, W, 2), 0.5, device=dev)
+            check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(indices),
+                                         ptr(vertices), ptr(feats), 13, ptr(jit), ptr(img), ptr(dist), ptr(nc),
+                                         stream_of()), "rm_composite_fwd")
+            return img[..., :3].double()
+
+        ref = image(raw64)
+        out = {"queries": int(h.shape[0]), "table_init": "U(-1,1) seed 3", "reference": "fp64 mlp, same weights"}
+        for name, raw in (("bf16", raw16), ("fp32", raw32)):
+            e = (image(raw) - ref).abs()
+            out[name] = {**params(raw), "pixel_abs_max": float(e.max()), "pixel_abs_mean": float(e.mean())}
+    torch.cuda.synchronize()
+    return out
+
+
 def calibrate_copy():
     """A plain 256 MiB device-to-device copy, 5 launches: under `rocprofv3 --pmc FETCH_SIZE` /
     `WRITE_SIZE` it gives the counters' reading for 256 MiB read + 256 MiB written."""
@@ -357,6 +421,9 @@ def main():
         serving = serve_leg(args, [(n, *built[n]) for n in serve_points])
     elif serve_points and not args.profile_steps:
         serving = serve_leg(args, [(args.points, pts, tets)])
+    precision = None
+    if not args.serve_only and not args.no_precision_report and not args.profile_steps:
+        precision = precision_report(pts, tets, cam, args.tile)
     cpu = None
     if not args.no_cpu_baseline and not args.profile_steps:
         cpu = cpu_baseline(args, pts, tets, cam)
@@ -382,6 +449,7 @@ def main():
         "hbm_frac_all_kernels": res["hbm_frac_all_kernels"],
         "breakdown_ms": res["breakdown_ms"],
         "tet_err": res.get("tet_err"),
+        "precision": precision,
         "cpu_baseline": cpu,
         "serving": serving,
     }
