@@ -349,10 +349,14 @@ def serve_leg(args, meshes):
     resolutions = [(256, 256), (512, 512), (768, 1024), (1080, 1920)]   # (H, W)
     # warm-up outside the trace: every bucket created (graph captured), every kernel loaded
     for H, W in resolutions:
+        log(f"[bench] serving warm-up {W}x{H}: bucket, graph capture, batched and single views")
         views = [(s, scenes.orbit_camera(6.0, 10.0 * s, 10.0, W, H, 50.0)) for s in range(len(meshes))]
         srv.render_views(views)
         for s, c in views:
             srv.render_single(s, c)
+        torch.cuda.synchronize()
+        b = srv.buckets[(H, W)]
+        log(f"[bench]   {W}x{H}: {b.slots} slots, pair capacity {b.pair_cap}, {b.bytes / 2**30:.1f} GiB")
     torch.cuda.synchronize()
     trace = poisson_trace(len(meshes), resolutions, args.serve_rate, args.serve_seconds, seed=0, radius=6.0)
     log(f"[bench] serving: {len(trace)} requests over {args.serve_seconds} s at {args.serve_rate}/s")
@@ -360,7 +364,9 @@ def serve_leg(args, meshes):
     runs = {}
     for mode in ("batched", "per_frame"):
         reqs = [Request(r.rid, r.scene, r.camera, r.arrival) for r in trace]
-        runs[mode] = (srv.serve(reqs, mode=mode, keep=keep), reqs)
+        runs[mode] = (srv.serve(reqs, mode=mode, keep=keep,
+                                progress=lambda d, n, t, m=mode: log(f"[bench]   {m}: {d}/{n} done at {t:.1f} s")),
+                      reqs)
         log(f"[bench] {mode}: {json.dumps(runs[mode][0])}")
     # every kept frame: batched result against the per-frame result of the same request
     max_err, n_cmp = 0.0, 0

@@ -100,7 +100,13 @@ class Bucket:
         if info.total_tets > self.info.total_tets or len(reqs) > self.slots:
             raise RuntimeError("batch exceeds the bucket's capacity")
         hw = self.H * self.W
+        cur = torch.cuda.current_stream(self.server.device)
         with torch.cuda.stream(self.stream):
+            # The bucket's buffers come from torch's caching allocator on the current stream, which
+            # hands out blocks freed there as soon as the host drops them, even while kernels queued
+            # on that stream still use them (a single-view render's binning, say). Ordering this
+            # stream after the current one keeps the batch from writing memory those kernels use.
+            self.stream.wait_stream(cur)
             self.plan_dev.copy_(self.plan_host, non_blocking=True)
             if self.use_graph:
                 if self.graph is None:
@@ -136,6 +142,9 @@ class Bucket:
         reqs, outs, ev, keep_aux = self.busy
         ev.synchronize()
         self.busy = None
+        cur = torch.cuda.current_stream(self.server.device)
+        for o in outs:   # read on the caller's stream from here on: freed only after its work
+            o.record_stream(cur)
         n_pairs = int(self.counts_host[1])
         if n_pairs > self.pair_cap:
             if self.server.on_overflow == "raise":
@@ -266,14 +275,21 @@ class RenderServer:
         return out
 
     # ------------------------------------------------------------------ trace replay
-    def serve(self, trace, mode="batched", max_batch=None, keep=None, poll_s=2e-4):
+    def serve(self, trace, mode="batched", max_batch=None, keep=None, poll_s=2e-4, progress=None,
+              progress_every_s=5.0):
         """Replays `trace` (Requests with arrival times in seconds from now) in real time.
         mode "batched": continuous batching (at most max_batch views per batch, default the
         bucket's slots); "per_frame": one view at a time through the single-view path.
         Results are stored in Request.image for the requests `keep(request)` accepts (all when
-        keep is None). Returns stats: frames/s over the makespan, latency p50/p99 (ms), batches,
-        overflows."""
+        keep is None). progress(done, total, seconds) is called about every progress_every_s.
+        Returns stats: frames/s over the makespan, latency p50/p99 (ms), batches, overflows."""
         keep = keep or (lambda r: True)
+        last_report = [0.0]
+
+        def report(n_done, now):
+            if progress is not None and now - last_report[0] >= progress_every_s:
+                last_report[0] = now
+                progress(n_done, len(trace), now)
         torch.cuda.synchronize()
         reqs = sorted(trace, key=lambda r: r.arrival)
         pending = deque(reqs)
@@ -296,11 +312,13 @@ class RenderServer:
                     r.image = img
                 n_batches += 1
                 sizes.append(1)
+                report(n_batches, r.done)
         else:
             queue: dict = {}
             in_use = 0
             while done < len(reqs):
                 now = clock()
+                report(done, now)
                 while pending and pending[0].arrival <= now:
                     r = pending.popleft()
                     queue.setdefault((r.camera.height, r.camera.width), deque()).append(r)

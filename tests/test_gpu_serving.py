@@ -131,3 +131,26 @@ def test_serve_trace_completes_every_request():
     for r in trace[:4]:
         ref = _single_full(srv, r.scene, r.camera)[0]
         assert np.array_equal(r.image.cpu().numpy(), ref.cpu().numpy())
+
+
+def test_batch_right_after_single_view_work():
+    """A bucket created and launched while a single-view render's kernels are still queued on the
+    current stream: the bucket's buffers may be blocks those kernels were using (the caching
+    allocator hands them out when the host drops them), so the batch stream must wait for the
+    current stream. Every batched view equals the same view rendered alone."""
+    import torch
+    from radiance_meshes_amd.serving import RenderServer
+    srv = RenderServer(tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=8 << 30, max_slots=4,
+                       use_graphs=True, pair_margin=4.0, probe_radius=2.5, on_overflow="raise")
+    sc = scene("gmm", 4, 13, n_points=100000, width=320, height=240)
+    srv.add_scene(torch.from_numpy(np.ascontiguousarray(sc.vertices, np.float32)),
+                  torch.from_numpy(np.ascontiguousarray(sc.indices, np.int32)),
+                  torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)))
+    print(f"scene T={len(sc.indices)}", flush=True)
+    for k, (H, W) in enumerate([(240, 320), (200, 256), (120, 160), (240, 320)]):
+        cams = [scenes.orbit_camera(3.0, 40.0 * j + 15 * k, 10.0, W, H, 50.0) for j in range(3)]
+        srv.render_single(0, cams[0])            # left running on the current stream
+        out = srv.render_views([(0, c) for c in cams], keep_aux=True)
+        torch.cuda.synchronize()
+        for j, c in enumerate(cams):
+            _assert_same(out[j], _single_full(srv, 0, c), f"{W}x{H} view {j}")
