@@ -62,6 +62,11 @@ def parse():
     p.add_argument("--serve-slots", type=int, default=8, help="views per batch of a resolution bucket")
     p.add_argument("--serve-hbm-gib", type=float, default=160.0, help="HBM budget of the server (GiB)")
     p.add_argument("--serve-only", action="store_true", help="skip the training-step leg")
+    p.add_argument("--serve-warmup-batches", type=int, default=40,
+                   help="random batches per resolution rendered before the trace (graph capture per class)")
+    p.add_argument("--serve-saturate-rate", type=float, default=0.0,
+                   help="also replay a short trace at this rate (1/s) to measure each mode's capacity")
+    p.add_argument("--serve-saturate-seconds", type=float, default=3.0)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 256 MiB device-to-device copy 5 times (calibrates the HBM byte counters)")
     return p.parse_args()
@@ -348,15 +353,24 @@ def serve_leg(args, meshes):
         tets_of.append(len(tets))
     resolutions = [(256, 256), (512, 512), (768, 1024), (1080, 1920)]   # (H, W)
     # warm-up outside the trace: every bucket created (graph captured), every kernel loaded
+    wrng = np.random.default_rng(1)
     for H, W in resolutions:
-        log(f"[bench] serving warm-up {W}x{H}: bucket, graph capture, batched and single views")
+        log(f"[bench] serving warm-up {W}x{H}: bucket, graph capture per capacity class, batched and single views")
         views = [(s, scenes.orbit_camera(6.0, 10.0 * s, 10.0, W, H, 50.0)) for s in range(len(meshes))]
         srv.render_views(views)
         for s, c in views:
             srv.render_single(s, c)
-        torch.cuda.synchronize()
         b = srv.buckets[(H, W)]
-        log(f"[bench]   {W}x{H}: {b.slots} slots, pair capacity {b.pair_cap}, {b.bytes / 2**30:.1f} GiB")
+        # batches of the trace's composition (1..slots views of random scenes) capture the classes
+        # the trace will use, so no capture (and its device-wide synchronisation) happens inside it
+        for _ in range(args.serve_warmup_batches):
+            n = int(wrng.integers(1, b.slots + 1))
+            srv.render_views([(int(wrng.integers(len(meshes))),
+                               scenes.orbit_camera(6.0, wrng.uniform(0, 360), wrng.uniform(-20, 40), W, H, 50.0))
+                              for _ in range(n)])
+        torch.cuda.synchronize()
+        log(f"[bench]   {W}x{H}: {b.slots} slots, pair capacity {b.pair_cap}, {b.bytes / 2**30:.1f} GiB, "
+            f"{len(b.graphs)} capacity classes captured")
     torch.cuda.synchronize()
     trace = poisson_trace(len(meshes), resolutions, args.serve_rate, args.serve_seconds, seed=0, radius=6.0)
     log(f"[bench] serving: {len(trace)} requests over {args.serve_seconds} s at {args.serve_rate}/s")
@@ -375,7 +389,22 @@ def serve_leg(args, meshes):
             max_err = max(max_err, float((rb.image - rp.image).abs().max().item()))
             n_cmp += 1
     sb, sp = runs["batched"][0], runs["per_frame"][0]
-    return {"fps_batched": round(sb["fps"], 2), "fps_per_frame": round(sp["fps"], 2),
+    saturated = None
+    if args.serve_saturate_rate > 0:
+        strace = poisson_trace(len(meshes), resolutions, args.serve_saturate_rate, args.serve_saturate_seconds,
+                               seed=5, radius=6.0)
+        log(f"[bench] saturating trace: {len(strace)} requests over {args.serve_saturate_seconds} s "
+            f"at {args.serve_saturate_rate}/s")
+        saturated = {"requests": len(strace), "rate_hz": args.serve_saturate_rate,
+                     "seconds": args.serve_saturate_seconds}
+        for mode in ("batched", "per_frame"):
+            reqs = [Request(r.rid, r.scene, r.camera, r.arrival) for r in strace]
+            st = srv.serve(reqs, mode=mode, keep=lambda r: False,
+                           progress=lambda d, n, t, m=mode: log(f"[bench]   saturated {m}: {d}/{n} done at {t:.1f} s"))
+            log(f"[bench] saturated {mode}: {json.dumps(st)}")
+            saturated[mode] = {k: round(st[k], 3) for k in ("fps", "p50_ms", "p99_ms", "mean_batch")}
+        saturated["fps_ratio"] = round(saturated["batched"]["fps"] / saturated["per_frame"]["fps"], 3)
+    return {"saturated": saturated, "fps_batched": round(sb["fps"], 2), "fps_per_frame": round(sp["fps"], 2),
             "fps_ratio": round(sb["fps"] / sp["fps"], 3),
             "p50_ms_batched": round(sb["p50_ms"], 2), "p99_ms_batched": round(sb["p99_ms"], 2),
             "p50_ms_per_frame": round(sp["p50_ms"], 2), "p99_ms_per_frame": round(sp["p99_ms"], 2),

@@ -4,9 +4,10 @@ The reference renders one view per call (evaluate.py:50-63, tile_benchmark.py:59
 
 * scenes (mesh + per-tet features) are registered once and stay in HBM;
 * requests are grouped by resolution bucket; each bucket owns a fixed number of slots with
-  preallocated buffers, a pinned host plan, its own stream and (optionally) a captured HIP graph
+  preallocated buffers, a pinned host plan, its own stream and (optionally) captured HIP graphs
   of the whole batched forward (`rm_batch_render`: projection, binning and compositing of every
-  view of the batch in one launch sequence, no host synchronisation inside);
+  view of the batch in one launch sequence, no host synchronisation inside), one per capacity
+  class (views, tets and pairs rounded up to powers of two) so that launches scale with the batch;
 * a request is admitted when its bucket has a free batch and the HBM budget covers the bucket's
   reservation; requests that arrive while other batches run on the GPU are admitted as soon as a
   bucket frees up, so batches of different buckets overlap on the device (one stream each);
@@ -51,17 +52,27 @@ def _cam_struct(camera, tile, min_t):
     return _capi.camera_from_tcam(camera.tcam(tile, min_t))
 
 
-class Bucket:
-    """One resolution: `slots` views per batch. Launch geometry and buffers are sized for `slots`
-    views of the largest registered scene, so any batch of this resolution fits one captured graph."""
+def _pow2_at_least(x: int) -> int:
+    return 1 << max(0, int(x - 1).bit_length())
 
-    def __init__(self, server, H, W, slots, pair_cap, use_graph):
+
+class Bucket:
+    """One resolution: up to `slots` views per batch. The workspace is sized for `slots` views of
+    the largest registered scene. Each batch launches with the geometry of its capacity class --
+    views, tets and (tile, tet) pairs each rounded up to a power of two -- so a batch of a few small
+    views does not pay for the bucket's full capacity; one HIP graph is captured per class and all
+    classes share the workspace (a bucket runs one batch at a time)."""
+
+    def __init__(self, server, H, W, slots, pair_cap, use_graph, scene_pairs):
         self.server, self.H, self.W, self.slots = server, H, W, slots
         self.tile = server.tile
         dev = server.device
         big = max(range(len(server.scenes)), key=lambda i: server.scenes[i].rec.n_tets)
         cam = server.probe_camera(H, W)
         self.info = server.plan_info([(big, cam)] * slots)
+        self.view_tiles = int(self.info.total_tiles) // slots
+        self.view_blocks = int(self.info.total_workgroups) // slots
+        self.scene_pairs = list(scene_pairs)     # pairs of each scene seen from the probe pose
         self.plan_bytes = int(self.info.plan_bytes)
         self.plan_host = torch.empty(self.plan_bytes, dtype=torch.uint8, pin_memory=True)
         self.plan_dev = torch.empty(self.plan_bytes, dtype=torch.uint8, device=dev)
@@ -72,33 +83,50 @@ class Bucket:
         self.counts = torch.zeros(2, device=dev, dtype=torch.int64)
         self.counts_host = torch.zeros(2, dtype=torch.int64, pin_memory=True)
         self.stream = torch.cuda.Stream(device=dev)
-        self.graph = None
+        self.graphs = {}
         self.use_graph = use_graph
-        self.busy = None   # (requests, outputs, event, keep_aux) of the batch in flight
+        self.busy = None   # (requests, outputs, event, keep_aux, pair capacity) of the batch in flight
         self.ws = None
         self._alloc(pair_cap)
 
     def _alloc(self, pair_cap):
-        """(Re)sizes the workspace for `pair_cap` (tile, tet) pairs; a captured graph is dropped."""
+        """(Re)sizes the workspace for `pair_cap` (tile, tet) pairs; captured graphs are dropped."""
         self.pair_cap = int(pair_cap)
+        self.graphs = {}
         self.ws = None
         self.ws = torch.empty(lib().rm_batch_workspace_size(self.info, self.pair_cap), dtype=torch.uint8,
                               device=self.server.device)
         self.bytes = self.ws.numel() + self.img.numel() * 40 // 4 + 2 * self.plan_bytes
-        self.graph = None
 
-    def _render_call(self):
-        check(lib().rm_batch_render(ptr(self.plan_dev), self.info, self.pair_cap, ptr(self.img), ptr(self.dist),
+    def _class_of(self, reqs, full=False):
+        """(key, launch info, pair capacity) of the capacity class that holds the batch `reqs`."""
+        if full:
+            return ("full",), self.info, self.pair_cap
+        scs = self.server.scenes
+        nv = min(self.slots, _pow2_at_least(len(reqs)))
+        T = min(int(self.info.total_tets), _pow2_at_least(sum(scs[r.scene].rec.n_tets for r in reqs)))
+        est = sum(int(self.server.pair_margin * self.scene_pairs[r.scene]) + 4096 for r in reqs)
+        P = min(self.pair_cap, _pow2_at_least(est))
+        info = RmBatchInfo(n_views=nv, total_tets=T, total_tiles=nv * self.view_tiles,
+                           total_pixels=nv * self.H * self.W, total_workgroups=nv * self.view_blocks,
+                           plan_bytes=self.plan_bytes, tile=self.info.tile, n_features=self.info.n_features)
+        assert lib().rm_batch_workspace_size(info, P) <= self.ws.numel()
+        return (nv, T, P), info, P
+
+    def _render_call(self, info, cap):
+        check(lib().rm_batch_render(ptr(self.plan_dev), info, cap, ptr(self.img), ptr(self.dist),
                                     ptr(self.nc), ptr(self.counts), ptr(self.ws), self.ws.numel(),
                                     stream_of(self.server.device)), "rm_batch_render")
 
-    def launch(self, reqs, keep_aux=False):
+    def launch(self, reqs, keep_aux=False, full=False):
         """Enqueues one batch on the bucket's stream and returns at once. The images (and with
         keep_aux the distortion images and contributor counts) are copied out on the same stream,
-        so the next batch of this bucket cannot overwrite them."""
+        so the next batch of this bucket cannot overwrite them. full=True launches with the
+        bucket's whole capacity."""
         info = self.server.plan_info([(r.scene, r.camera) for r in reqs], self.plan_host)
         if info.total_tets > self.info.total_tets or len(reqs) > self.slots:
             raise RuntimeError("batch exceeds the bucket's capacity")
+        key, cinfo, cap = self._class_of(reqs, full)
         hw = self.H * self.W
         cur = torch.cuda.current_stream(self.server.device)
         with torch.cuda.stream(self.stream):
@@ -109,51 +137,57 @@ class Bucket:
             self.stream.wait_stream(cur)
             self.plan_dev.copy_(self.plan_host, non_blocking=True)
             if self.use_graph:
-                if self.graph is None:
-                    self._capture()
-                self.graph.replay()
+                g = self.graphs.get(key)
+                if g is None:
+                    g = self.graphs[key] = self._capture(cinfo, cap)
+                g.replay()
             else:
-                self._render_call()
+                self._render_call(cinfo, cap)
             self.counts_host.copy_(self.counts, non_blocking=True)
             n = len(reqs) * hw
             outs = (self.img[:n].clone(),) + ((self.dist[:n].clone(), self.nc[:n].clone()) if keep_aux else ())
             ev = torch.cuda.Event()
             ev.record(self.stream)
-        self.busy = (reqs, outs, ev, keep_aux)
+        self.busy = (reqs, outs, ev, keep_aux, cap)
         return ev
 
-    def _capture(self):
+    def _capture(self, info, cap):
         # run once eagerly (first-launch set-up stays out of the graph), then capture
-        self._render_call()
+        self._render_call(info, cap)
         self.stream.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=self.stream):
-            self._render_call()
-        self.graph = g
+            self._render_call(info, cap)
+        return g
 
     def ready(self):
         return self.busy is not None and self.busy[2].query()
 
     def finish(self):
         """([(request, ([H,W,4] image, [H,W,5] distortion, [H,W] n_contrib) or (image,)), ...],
-        overflowed) of the completed batch. A batch whose pairs exceeded the capacity is rendered
-        again after the workspace grows to fit it (or view by view through the single-view path
-        when the HBM budget does not allow the growth)."""
-        reqs, outs, ev, keep_aux = self.busy
+        overflowed) of the completed batch. A batch whose pairs exceeded its class's capacity is
+        rendered again with the bucket's whole capacity, after the workspace grows when that is
+        too small as well (or view by view through the single-view path when the HBM budget does
+        not allow the growth)."""
+        reqs, outs, ev, keep_aux, cap = self.busy
         ev.synchronize()
         self.busy = None
         cur = torch.cuda.current_stream(self.server.device)
         for o in outs:   # read on the caller's stream from here on: freed only after its work
             o.record_stream(cur)
         n_pairs = int(self.counts_host[1])
-        if n_pairs > self.pair_cap:
+        if n_pairs > cap:
+            if n_pairs <= self.pair_cap:
+                self.launch(reqs, keep_aux, full=True)
+                res, _ = self.finish()
+                return res, True
             if self.server.on_overflow == "raise":
                 raise RuntimeError(f"batch of {len(reqs)} views needs {n_pairs} pairs, capacity {self.pair_cap}")
             need = -(-n_pairs // len(reqs)) * self.slots * 5 // 4 + 4096
             grown = need < (1 << 31) - 1 and self.server.can_grow(self, need)
             if grown:
                 self._alloc(need)
-                self.launch(reqs, keep_aux)
+                self.launch(reqs, keep_aux, full=True)
                 res, _ = self.finish()
             else:
                 res = [(r, (self.server.render_single(r.scene, r.camera),)) for r in reqs]
@@ -248,7 +282,8 @@ class RenderServer:
         key = (H, W)
         if key not in self.buckets:
             probe = self.probe_camera(H, W)
-            per_view_pairs = int(self.pair_margin * max(self._pairs_of(s, probe) for s in range(len(self.scenes)))) + 4096
+            scene_pairs = [self._pairs_of(s, probe) for s in range(len(self.scenes))]
+            per_view_pairs = int(self.pair_margin * max(scene_pairs)) + 4096
             big = max(range(len(self.scenes)), key=lambda i: self.scenes[i].rec.n_tets)
             remaining = self.budget - sum(b.bytes for b in self.buckets.values())
             slots = self.max_slots
@@ -258,7 +293,7 @@ class RenderServer:
                 if need <= remaining and per_view_pairs * slots < (1 << 31) - 1:
                     break
                 slots //= 2
-            self.buckets[key] = Bucket(self, H, W, slots, per_view_pairs * slots, self.use_graphs)
+            self.buckets[key] = Bucket(self, H, W, slots, per_view_pairs * slots, self.use_graphs, scene_pairs)
         return self.buckets[key]
 
     def render_views(self, views, keep_aux=False):
