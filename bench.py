@@ -12,6 +12,11 @@ Prints one JSON line (rank 0): metric fwd+bwd Mrays/s (rays = pixels per step / 
 roofline of the dominant kernel (HIP events around its launches inside the timed region), and a
 CPU baseline: the reference's CPU method (per-ray midpoint quadrature testing every tet at every
 sample, jaxutil/tetra_quad.py) restated in C, timed on a bounded sample of rays of the same view.
+The line also carries the per-tet error-vote timing, the BF16-vs-FP64 field report and the
+serving leg: three device-resident meshes (250k, 1M, 3M points) served a Poisson trace of
+mixed-resolution views (256x256 .. 1920x1080, 200/s for --serve-seconds, then a saturating
+1000/s trace) batched by resolution bucket and per frame, with p50/p99 latency, frames/s and the
+batched-vs-per-frame image difference.
 """
 from __future__ import annotations
 
@@ -55,8 +60,8 @@ def parse():
     p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
     p.add_argument("--profile-steps", type=int, default=0,
                    help="run only this many steps after warmup, without the CPU baseline (for rocprof)")
-    p.add_argument("--serve-seconds", type=float, default=0.0,
-                   help="length of the Poisson request trace of the serving leg (default 0: leg off)")
+    p.add_argument("--serve-seconds", type=float, default=10.0,
+                   help="length of the Poisson request trace of the serving leg (0: leg off)")
     p.add_argument("--serve-rate", type=float, default=200.0, help="request arrival rate (1/s)")
     p.add_argument("--serve-points", default="250000,1000000,3000000", help="points of the served meshes")
     p.add_argument("--serve-slots", type=int, default=8, help="views per batch of a resolution bucket")
@@ -64,9 +69,9 @@ def parse():
     p.add_argument("--serve-only", action="store_true", help="skip the training-step leg")
     p.add_argument("--serve-warmup-batches", type=int, default=40,
                    help="random batches per resolution rendered before the trace (graph capture per class)")
-    p.add_argument("--serve-saturate-rate", type=float, default=0.0,
+    p.add_argument("--serve-saturate-rate", type=float, default=1000.0,
                    help="also replay a short trace at this rate (1/s) to measure each mode's capacity")
-    p.add_argument("--serve-saturate-seconds", type=float, default=3.0)
+    p.add_argument("--serve-saturate-seconds", type=float, default=2.0)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 256 MiB device-to-device copy 5 times (calibrates the HBM byte counters)")
     return p.parse_args()
@@ -445,20 +450,25 @@ def main():
             r = run_config(args, pts, tets, cam, extra)
             log(json.dumps({"sweep_tile": extra, **{k: r[k] for k in ("value", "ms_per_step", "breakdown_ms", "config")}}))
         res = run_config(args, pts, tets, cam, args.tile)
+    precision = None
+    if not args.serve_only and not args.no_precision_report and not args.profile_steps:
+        precision = precision_report(pts, tets, cam, args.tile)
     serving = None
-    if pending:
+    if serve_points and not args.profile_steps:
+        import torch
+        torch.cuda.empty_cache()
         built = {args.points: (pts, tets)}
         for job in pending:
             n, p_, t_, secs = job.get()
             log(f"[bench] serving mesh {n} points: {len(t_)} tets ({secs:.1f} s)")
             built[n] = (p_, t_)
-        pool.close()
-        serving = serve_leg(args, [(n, *built[n]) for n in serve_points])
-    elif serve_points and not args.profile_steps:
-        serving = serve_leg(args, [(args.points, pts, tets)])
-    precision = None
-    if not args.serve_only and not args.no_precision_report and not args.profile_steps:
-        precision = precision_report(pts, tets, cam, args.tile)
+        if pool is not None:
+            pool.close()
+        try:
+            serving = serve_leg(args, [(n, *built[n]) for n in serve_points])
+        except Exception as e:   # the training-step line is still printed
+            log(f"[bench] serving leg failed: {type(e).__name__}: {e}")
+            serving = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu_baseline and not args.profile_steps:
         cpu = cpu_baseline(args, pts, tets, cam)
