@@ -49,3 +49,23 @@ def grid_of(sc):
 
 def f32_bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.int32)
+
+
+def with_camera(sc, kind: str):
+    """A copy of scene sc seen through another camera model: 'distorted' (pinhole with radial and
+    tangential distortion k1, k2, p1, p2), 'fisheye' (equidistant, camera_type 1), 'inside'
+    (pinhole at the centre of the mesh) or 'away' (pinhole looking away from the mesh)."""
+    import dataclasses
+    cam = sc.camera
+    if kind == "distorted":
+        cam = dataclasses.replace(cam, distortion=np.array([0.08, -0.02, 0.004, -0.003], np.float32))
+    elif kind == "fisheye":
+        cam = dataclasses.replace(cam, camera_type=1)
+    elif kind == "inside":
+        cam = scenes.look_at_camera((0.05, -0.03, 0.02), (1.0, 0.2, -0.3), (0, 1, 0), cam.width, cam.height, 70.0)
+    elif kind == "away":
+        p = np.asarray(cam.cam_pos, np.float64)
+        cam = scenes.look_at_camera(tuple(p), tuple(2 * p), (0, 1, 0), cam.width, cam.height, 40.0)
+    else:
+        raise ValueError(kind)
+    return dataclasses.replace(sc, camera=cam, name=f"{sc.name}_{kind}")

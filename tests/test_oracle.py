@@ -309,3 +309,33 @@ def test_backward_matches_central_differences(O, nf):
     err = np.abs(a[:, 0] - a[:, 1])
     assert scale > 0
     assert np.all(err <= 1e-5 * np.maximum(np.abs(a[:, 1]), 1e-2 * scale)), a[err.argmax()]
+
+
+@pytest.mark.parametrize("kind", ["distorted", "fisheye", "inside"])
+def test_projection_cover_other_cameras(O, small, kind):
+    """Distorted pinhole, fisheye and a camera inside the mesh: rects stay inside the grid, and every
+    pixel ray that crosses a tet (FP64) falls in one of the tet's tiles (the reference bounds the
+    curved images of edges by 3 or 4 samples per edge; for these cameras that bound holds)."""
+    from tests._util import with_camera
+    sc = with_camera(small, kind)
+    cam = O.make_camera(sc.camera, sc.tile_size, sc.min_t)
+    tiles, rect, vs, cc = O.project(cam, sc.indices, sc.vertices)
+    vis = tiles > 0
+    assert vis.sum() > 50
+    assert np.all(rect[vis, 0] >= 0) and np.all(rect[vis, 2] <= cam.gw)
+    assert np.all(rect[vis, 1] >= 0) and np.all(rect[vis, 3] <= cam.gh)
+    V = sc.vertices.astype(np.float64)[sc.indices]
+    H, W, ts = cam.H, cam.W, cam.th
+    rays = [O.get_ray(cam, x + 0.5, y + 0.5) for y in range(H) for x in range(W)]
+    rng = np.random.default_rng(1)
+    n_hit = 0
+    for t in rng.choice(len(sc.indices), 120, replace=False):
+        for p, (o, d) in enumerate(rays):
+            hit, e, x_ = O.intersect(V[t].reshape(12), o, d)
+            if not hit or x_ - e < 1e-7:
+                continue
+            n_hit += 1
+            px, py = p % W, p // W
+            assert vis[t], f"tet {t} hit by pixel {p} but culled"
+            assert rect[t, 0] <= px // ts < rect[t, 2] and rect[t, 1] <= py // ts < rect[t, 3], (t, p)
+    assert n_hit > 20
