@@ -17,8 +17,9 @@
 // 16*(s/4) + 4q + (s%4) (f32) or 16*s + 4q + e (bf16), which is exactly its own register r (or e)
 // of block s/4 (or s). Activations never leave registers; the input rows come from global memory
 // in that layout (two float4 per lane) and dX leaves the same way.
-// The backward recomputes the forward per tile (as the reference's checkpointed field), chains
-// dY -> dH2 -> dH1 -> dX through the transposed weights, and forms the weight gradients
+// The backward recomputes the forward per tile (as the reference's checkpointed field) keeping
+// the hidden activations (selu' is a function of them), chains dY -> dH2 -> dH1 -> dX through
+// the transposed weights, and forms the weight gradients
 // gW = dH act^T as MFMA tiles contracted over the tile's points; those need the point index on
 // the lanes' k, so dH and act blocks are transposed through a 16x17 LDS block per 16 units.
 // Weight-gradient tiles accumulate in registers across all tiles of a wave, are summed across the
@@ -48,7 +49,8 @@ __device__ inline void wave_sync() {
 }
 
 __device__ inline float selu(float x) { return kSeluScale * (x > 0.f ? x : kSeluAlpha * expm1f(x)); }
-__device__ inline float dselu(float x) { return kSeluScale * (x > 0.f ? 1.f : kSeluAlpha * __expf(x)); }
+// selu'(h) from a = selu(h): a has the sign of h, and for h <= 0 selu'(h) = scale alpha e^h = a + scale alpha
+__device__ inline float dselu_of_act(float a) { return a > 0.f ? kSeluScale : a + kSeluScale * kSeluAlpha; }
 
 __device__ inline short to_bf16(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -153,28 +155,22 @@ __device__ inline void load_x_chain(const float* __restrict__ x, int64_t p, int6
     }
 }
 
-// Forward of one tile: pre-activations h1, h2 (chain layout, biases added) and the output block.
+// Forward of one tile: hidden activations a1, a2 (chain layout) and the output block. The
+// backward takes selu' from the activations (dselu_of_act), so no pre-activation is kept.
 template <int P>
-__device__ inline f4 tile_forward(const Smem& S, const f4 (&xin)[2], f4 (&h1)[4], f4 (&h2)[4]) {
+__device__ inline f4 tile_forward(const Smem& S, const f4 (&xin)[2], f4 (&a1)[4], f4 (&a2)[4]) {
     const int lane = threadIdx.x & 63, q = lane >> 4;
-    f4 a1[4], a2[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
         f4 d = chain<P, 2>(xin, [&](int i, int k) { return S.w1[(16 * ot + i) * SW1 + k]; });
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            h1[ot][r] = d[r] + S.b1[16 * ot + 4 * q + r];
-            a1[ot][r] = selu(h1[ot][r]);
-        }
+        for (int r = 0; r < 4; ++r) a1[ot][r] = selu(d[r] + S.b1[16 * ot + 4 * q + r]);
     }
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
         f4 d = chain<P, 4>(a1, [&](int i, int k) { return S.w2[(16 * ot + i) * SW2 + k]; });
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            h2[ot][r] = d[r] + S.b2[16 * ot + 4 * q + r];
-            a2[ot][r] = selu(h2[ot][r]);
-        }
+        for (int r = 0; r < 4; ++r) a2[ot][r] = selu(d[r] + S.b2[16 * ot + 4 * q + r]);
     }
     f4 y = chain<P, 4>(a2, [&](int i, int k) { return S.w3[i * SW3 + k]; });
 #pragma unroll
@@ -195,9 +191,9 @@ __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(const float* __restri
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
         asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
         const int64_t p = t * 16 + c;
-        f4 xin[2], h1[4], h2[4];
+        f4 xin[2], a1[4], a2[4];
         load_x_chain(x, p, n, xin);
-        const f4 out = tile_forward<P>(S, xin, h1, h2);
+        const f4 out = tile_forward<P>(S, xin, a1, a2);
         if (p < n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -238,9 +234,9 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
         asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
         const int64_t p = t * 16 + c;
-        f4 xin[2], h1[4], h2[4];
+        f4 xin[2], a1[4], a2[4];
         load_x_chain(x, p, n, xin);
-        tile_forward<P>(S, xin, h1, h2);
+        tile_forward<P>(S, xin, a1, a2);
         // dY in the chain layout (unit 4q + r of point c) and as a point fragment (unit c, points 4q + s)
         f4 dyc[1];
         float dyp[4];
@@ -254,12 +250,7 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
         gb3p += dyc[0];
         // gW3 += dY A2^T
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
-            f4 a2;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) a2[r] = selu(h2[kt][r]);
-            put_block(tb + kt * kTBlock, a2);
-        }
+        for (int kt = 0; kt < 4; ++kt) put_block(tb + kt * kTBlock, a2[kt]);
         wave_sync();
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
@@ -273,18 +264,15 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
         for (int ot = 0; ot < 4; ++ot) {
             f4 d = chain<P, 1>(dyc, [&](int i, int k) { return S.w3[k * SW3 + 16 * ot + i]; });
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * dselu(h2[ot][r]);
+            for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * dselu_of_act(a2[ot][r]);
             gb2p[ot] += dh2[ot];
         }
         wave_sync();
         // gW2 += dH2 A1^T
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            f4 a1;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) a1[r] = selu(h1[b][r]);
             put_block(tb + b * kTBlock, dh2[b]);
-            put_block(tb + (4 + b) * kTBlock, a1);
+            put_block(tb + (4 + b) * kTBlock, a1[b]);
         }
         wave_sync();
         {
@@ -305,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
         for (int ot = 0; ot < 4; ++ot) {
             f4 d = chain<P, 4>(dh2, [&](int i, int k) { return S.w2[k * SW2 + 16 * ot + i]; });
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * dselu(h1[ot][r]);
+            for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * dselu_of_act(a1[ot][r]);
             gb1p[ot] += dh1[ot];
         }
         // dX = W1^T dH1
