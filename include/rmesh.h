@@ -162,6 +162,25 @@ int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int
                         float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
                         size_t workspace_bytes, rm_stream_t stream);
 
+/* ---- per-tet feature activation of the reference's field model ------------------------
+ * Replaces activate_output (utils/model_util.py:93-105) with offset_normalize (:81-91),
+ * compute_vertex_colors_from_field (:62-78), eval_sh (utils/eval_sh_py.py:137-195) and safe_div
+ * (utils/safe_math.py:143-171): features_dev [T,7] = [density, softplus_10(colour), normed_grd].
+ * camera_center_dev: 3 floats in device memory; density [T], rgb [T,3], grd [T,3], sh [T,K,3]
+ * (K >= (sh_degree+1)^2 - 1, K <= 15, only the first (sh_degree+1)^2 - 1 used; may be null when
+ * K = 0), circumcenters [T,3], vertices [V,3]. The backward writes the per-tet gradients (any may
+ * be null) and adds the vertex gradients into grad_vertices_dev (caller-zeroed, may be null). */
+int rm_activate_fwd(const float* camera_center_dev, const float* density_dev, const float* rgb_dev,
+                    const float* grd_dev, const float* sh_dev, int32_t sh_coeffs, int32_t sh_degree,
+                    const int32_t* indices_dev, const float* circumcenters_dev, const float* vertices_dev,
+                    int64_t n_tets, float* features_dev, rm_stream_t stream);
+int rm_activate_bwd(const float* camera_center_dev, const float* density_dev, const float* rgb_dev,
+                    const float* grd_dev, const float* sh_dev, int32_t sh_coeffs, int32_t sh_degree,
+                    const int32_t* indices_dev, const float* circumcenters_dev, const float* vertices_dev,
+                    int64_t n_tets, const float* grad_features_dev, float* grad_density_dev, float* grad_rgb_dev,
+                    float* grad_grd_dev, float* grad_sh_dev, float* grad_vertices_dev,
+                    float* grad_circumcenters_dev, rm_stream_t stream);
+
 /* ---- per-tet error votes (densification statistics) ------------------------------------
  * Replaces calc_tet_err / kern_calc_tet_err (alphablend_shader_interp.slang:433-599), launched
  * by render_err (delaunay_rasterization/internal/render_err.py:138-150). Pixel-centre rays walk

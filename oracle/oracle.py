@@ -291,3 +291,46 @@ def mlp_fwd(x, w1, b1, w2, b2, w3, b3):
     h1 = x @ w1.T + b1
     h2 = selu(h1) @ w2.T + b2
     return selu(h2) @ w3.T + b3
+
+
+# --------------------------------------------------------------------- per-tet feature activation
+
+def _sh_real_scipy(dirs, deg):
+    """Real SH basis for degrees 1..deg at unit directions [N,3], from scipy's complex harmonics
+    (Condon-Shortley phase): sqrt(2) Im (m < 0), Re (m = 0), sqrt(2) Re (m > 0)."""
+    from scipy import special
+    polar = np.arccos(np.clip(dirs[:, 2], -1, 1))
+    azim = np.arctan2(dirs[:, 1], dirs[:, 0])
+    cols = []
+    for l in range(1, deg + 1):
+        for m in range(-l, l + 1):
+            Y = special.sph_harm_y(l, abs(m), polar, azim)
+            cols.append(np.sqrt(2) * Y.imag if m < 0 else (Y.real if m == 0 else np.sqrt(2) * Y.real))
+    return np.stack(cols, 1) if cols else np.zeros((len(dirs), 0))
+
+
+def activate_output(camera_center, density, rgb, grd, sh, indices, circumcenters, vertices, deg, offset=None):
+    """FP64 per-tet features [T,7] of the reference's activate_output (utils/model_util.py:93-105,
+    offset_normalize :81-91, compute_vertex_colors_from_field :62-78, eval_sh
+    utils/eval_sh_py.py:137-195, softplus beta 10 threshold 20; safe_div utils/safe_math.py:143-171).
+    offset [T,3], when given, replaces v0 - c in the vertex-0 colour: the reference detaches the
+    vertices and circumcenters there, so central differences of its gradient hold it fixed."""
+    V = np.asarray(vertices, np.float64)
+    tets = V[np.asarray(indices, np.int64)]                       # [T,4,3]
+    rgb = np.asarray(rgb, np.float64)
+    grd = np.asarray(grd, np.float64).reshape(-1, 3)
+    cc = np.asarray(circumcenters, np.float64)
+    g2 = grd * rgb.min(axis=1, keepdims=True)                     # grd * min channel
+    radius = np.linalg.norm(tets[:, 0] - cc, axis=1, keepdims=True)
+    tiny = np.finfo(np.float32).tiny
+    ng = np.where(np.abs(radius) < tiny, 0.0, g2 / np.where(np.abs(radius) < tiny, tiny, radius))
+    off = tets[:, 0] - cc if offset is None else np.asarray(offset, np.float64)
+    base = rgb + (ng * off).sum(1, keepdims=True)                 # vertex-0 colour of the linear field
+    md = tets.mean(1) - np.asarray(camera_center, np.float64).reshape(1, 3)
+    dirs = md / np.maximum(np.linalg.norm(md, axis=1, keepdims=True), 1e-8)
+    raw = 0.28209479177387814 * ((base - 0.5) / 0.28209479177387814) + 0.5
+    if deg > 0:
+        B = _sh_real_scipy(dirs, deg)
+        raw = raw + np.einsum("tk,tkc->tc", B, np.asarray(sh, np.float64)[:, :B.shape[1]])
+    col = np.where(10 * raw > 20, raw, np.log1p(np.exp(np.minimum(10 * raw, 20))) / 10)
+    return np.concatenate([np.asarray(density, np.float64).reshape(-1, 1), col, ng], 1)
