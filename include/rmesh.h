@@ -292,6 +292,33 @@ int rm_field_scatter_fwd(const float* raw_dev, const int32_t* tet_ids_dev, int64
 int rm_field_scatter_bwd(const float* raw_dev, const int32_t* tet_ids_dev, int64_t n, int32_t n_features,
                          const float* grad_features_dev, float* grad_raw_dev, rm_stream_t stream);
 
+/* ---- Delaunay tetrahedralisation ------------------------------------------------------------
+ * Replaces the triangulation step of models/ingp_color.py:523-544: gdel3d `Del(V).compute(verts)`
+ * (or scipy.spatial.Delaunay(verts).simplices when high_precision), the filter dropping tets with
+ * the point at infinity, and the swap of columns 0 and 1 of negatively oriented tets
+ * (topo_utils.tet_volumes, utils/topo_utils.py:9-24). points_dev is [n_points, 3] f32; tets_dev
+ * receives the Delaunay tets as [n, 4] i32, each with positive volume, grouped by their smallest
+ * vertex index in increasing order, at most tet_capacity of them. info_dev [8] i64 receives:
+ *   0: n (the full count, which may exceed tet_capacity: run again with a larger capacity),
+ *   1: Voronoi vertices found over all cells (4 n when the cells agree),
+ *   2: cells over the per-cell capacity (256 vertices, 128 planes; non-zero = failed),
+ *   3: duplicate points dropped (the copy with the lowest index keeps the cell),
+ *   4: predicate evaluations that needed exact arithmetic,
+ *   5: of those, inputs outside its range (coordinate exponents spanning more than 99 bits),
+ *   6: cell topology errors (non-zero = failed), 7: largest cell, in vertices.
+ * Exact ties (co-spherical or coplanar points) are resolved by a symbolic perturbation, so the
+ * result is one valid Delaunay tetrahedralisation also for degenerate input. The point at infinity
+ * is modelled by four far points (a tetrahedron 1e8 times the point-set extent): a hull tet whose
+ * circumsphere would reach them (circumradius above ~5e7 extents) is replaced by tets at infinity.
+ * No host synchronisation inside. */
+size_t rm_delaunay_workspace_size(int64_t n_points, int64_t tet_capacity);
+int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* tets_dev, int64_t tet_capacity,
+                int64_t* info_dev, void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+/* Host evaluation of the insphere predicate rm_delaunay uses (for tests without a GPU): 1 when
+ * point 4 of xyz [5,3] is strictly inside the circumsphere of the positively oriented tet of
+ * points 0-3 under the perturbation indexed by ids [5], else 0. */
+int rm_delaunay_insphere_host(const int32_t* ids, const float* xyz);
+
 #ifdef __cplusplus
 }
 #endif

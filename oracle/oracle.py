@@ -334,3 +334,130 @@ def activate_output(camera_center, density, rgb, grd, sh, indices, circumcenters
         raw = raw + np.einsum("tk,tkc->tc", B, np.asarray(sh, np.float64)[:, :B.shape[1]])
     col = np.where(10 * raw > 20, raw, np.log1p(np.exp(np.minimum(10 * raw, 20))) / 10)
     return np.concatenate([np.asarray(density, np.float64).reshape(-1, 1), col, ng], 1)
+
+
+# ---- Delaunay tetrahedralisation (models/ingp_color.py:523-544) ---------------------------------
+def delaunay(points):
+    """The reference's high-precision triangulation branch: scipy.spatial.Delaunay (Qhull) simplices
+    of the points (models/ingp_color.py:526-528), then the orientation fix of :536-540 (tets with
+    topo_utils.tet_volumes < 0, utils/topo_utils.py:9-24, get columns 0 and 1 swapped). int64 [T,4]."""
+    from scipy.spatial import Delaunay
+    p = np.asarray(points, np.float64)
+    t = Delaunay(p).simplices.astype(np.int64)
+    a, b, c, d = (p[t[:, k]] for k in range(4))
+    neg = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) < 0
+    t[neg] = t[neg][:, [1, 0, 2, 3]]
+    return t
+
+
+def canonical_tets(t):
+    """Tets as a set: vertex ids sorted within each tet, rows sorted lexicographically
+    (the normalisation of tests/delaunay_update_test.py:12-13 and tests/test_del.py:45-46)."""
+    s = np.sort(np.asarray(t, np.int64).reshape(-1, 4), axis=1)
+    return s[np.lexsort(s.T[::-1])] if len(s) else s
+
+
+def _fracs(points):
+    from fractions import Fraction
+    return [[Fraction(float(v)) for v in row] for row in np.asarray(points, np.float32)]
+
+
+def _det(m):
+    """Exact determinant (Fraction entries) by fraction-free elimination."""
+    m = [list(r) for r in m]
+    n, sign, det = len(m), 1, 1
+    for c in range(n):
+        piv = next((r for r in range(c, n) if m[r][c] != 0), None)
+        if piv is None:
+            return 0
+        if piv != c:
+            m[c], m[piv] = m[piv], m[c]
+            sign = -sign
+        det *= m[c][c]
+        for r in range(c + 1, n):
+            f = m[r][c] / m[c][c]
+            if f:
+                for k in range(c, n):
+                    m[r][k] -= f * m[c][k]
+    return sign * det
+
+
+def orient_exact(P, i, j, k, l):
+    """Sign of det(P_j - P_i, P_k - P_i, P_l - P_i) in exact arithmetic (P from _fracs)."""
+    a, b, c, d = P[i], P[j], P[k], P[l]
+    d3 = _det([[b[x] - a[x] for x in range(3)], [c[x] - a[x] for x in range(3)], [d[x] - a[x] for x in range(3)]])
+    return (d3 > 0) - (d3 < 0)
+
+
+def inside_sphere_exact(P, tet, q):
+    """True when point q is strictly inside the circumsphere of tet (exact, no perturbation):
+    |q - c|^2 < |a - c|^2 with the circumcentre c solved exactly."""
+    a = P[tet[0]]
+    rows, rhs = [], []
+    for v in tet[1:]:
+        b = P[v]
+        rows.append([b[x] - a[x] for x in range(3)])
+        rhs.append(sum(b[x] * b[x] - a[x] * a[x] for x in range(3)) / 2)
+    D = _det(rows)
+    c = []
+    for x in range(3):
+        m = [list(r) for r in rows]
+        for r in range(3):
+            m[r][x] = rhs[r]
+        c.append(_det(m) / D)
+    r2 = sum((a[x] - c[x]) ** 2 for x in range(3))
+    return sum((P[q][x] - c[x]) ** 2 for x in range(3)) < r2
+
+
+def delaunay_brute(points):
+    """Every 4-subset of non-coplanar points whose circumsphere holds no other point inside or on
+    it, positively oriented; exact. For points in general position this is the Delaunay
+    tetrahedralisation itself (small inputs only: O(N^5))."""
+    import itertools
+    P = _fracs(points)
+    n = len(P)
+    out = []
+    for t in itertools.combinations(range(n), 4):
+        o = orient_exact(P, *t)
+        if o == 0:
+            continue
+        tt = t if o > 0 else (t[1], t[0], t[2], t[3])
+        a = P[tt[0]]
+        rows, rhs = [], []
+        for v in tt[1:]:
+            b = P[v]
+            rows.append([b[x] - a[x] for x in range(3)])
+            rhs.append(sum(b[x] * b[x] - a[x] * a[x] for x in range(3)) / 2)
+        D = _det(rows)
+        c = []
+        for x in range(3):
+            m = [list(r) for r in rows]
+            for r in range(3):
+                m[r][x] = rhs[r]
+            c.append(_det(m) / D)
+        r2 = sum((a[x] - c[x]) ** 2 for x in range(3))
+        if all(sum((P[q][x] - c[x]) ** 2 for x in range(3)) > r2 for q in range(n) if q not in t):
+            out.append(tt)
+    return np.array(out, np.int64).reshape(-1, 4)
+
+
+def perturbed_inside(points5, ids5):
+    """Whether point 4 is inside the circumsphere of the positively oriented tet of points 0-3
+    under the lift perturbation lift_k -= eps^id_k (eps -> 0+), from the 5x5 lifted determinant and
+    its lift-column cofactors, each computed as an exact determinant (rows in the given order)."""
+    P = _fracs(points5)
+    M = [[P[r][0], P[r][1], P[r][2], sum(v * v for v in P[r]), 1] for r in range(5)]
+    D = _det(M)
+    if D != 0:
+        s = (D > 0) - (D < 0)
+    else:
+        s = 0
+        for r in sorted(range(5), key=lambda k: ids5[k]):
+            Mc = [list(row) for row in M]
+            for k in range(5):
+                Mc[k][3] = 1 if k == r else 0
+            C = _det(Mc)
+            if C != 0:
+                s = -((C > 0) - (C < 0))   # d/d(lift_r) = C_r and lift_r decreases
+                break
+    return s < 0

@@ -1,8 +1,9 @@
 """MI355X (gfx950) implementation of Radiance Meshes' differentiable tetrahedral tile
 rasterizer and hash-grid field, behind the reference's Python interface.
 
-`install_reference_aliases()` registers this package's `delaunay_rasterization` mirror under the
-reference's import path, so `train.py` / `evaluate.py` style callers run on it unchanged.
+`install_reference_aliases()` registers this package's `delaunay_rasterization` mirror (and its
+Delaunay step as `gdel3d`) under the reference's import paths, so `train.py` / `evaluate.py` style
+callers run on it unchanged.
 """
 from __future__ import annotations
 
@@ -26,3 +27,5 @@ def install_reference_aliases():
     ]
     for n in names:
         sys.modules[n] = importlib.import_module("radiance_meshes_amd." + n)
+    # gdel3d's Del (models/ingp_color.py:5) -> the GPU Delaunay
+    sys.modules["gdel3d"] = importlib.import_module("radiance_meshes_amd.delaunay")

@@ -82,6 +82,9 @@ SIGNATURES = {
     "rm_field_points": (C.c_int, [_P, _P, _P, _I64, C.c_float, _P, _P]),
     "rm_field_scatter_fwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P]),
     "rm_field_scatter_bwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P]),
+    "rm_delaunay_workspace_size": (_SZ, [_I64, _I64]),
+    "rm_delaunay": (C.c_int, [_P, _I64, _P, _I64, _P, _P, _SZ, _P]),
+    "rm_delaunay_insphere_host": (C.c_int, [_P, _P]),
 }
 
 _lib = None

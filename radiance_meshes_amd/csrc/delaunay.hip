@@ -1,0 +1,961 @@
+// Delaunay tetrahedralisation of a point set on the GPU, by Voronoi cell clipping (gfx950, wave64).
+//
+// Replaces the triangulation step of models/ingp_color.py:523-544 (gdel3d `Del(V).compute` of the
+// vertices, or scipy.spatial.Delaunay on the high-precision path) together with its
+// post-processing: tets with the point at infinity are not produced, and every tet is positively
+// oriented, det(v1-v0, v2-v0, v3-v0) > 0 as utils/topo_utils.py:9-24 measures it.
+//
+// Method. The Delaunay tets are the vertices of the Voronoi diagram. One wave computes the Voronoi
+// cell of one point p: it starts from p's cell among four far points (the corners of a tetrahedron
+// 1e8 times the size of the point set, indices n..n+3, treated as ordinary points) and clips it
+// with the bisector plane of each neighbour q, visiting the cells of a uniform grid in shells of growing Chebyshev radius until no
+// unvisited point can reach the cell (q cuts a cell vertex v only if |q-p| < 2|v-p|). The cell is
+// held in dual form in LDS: a table of planes and a list of triangles, one per cell vertex, naming
+// the three planes that meet there, ordered counter-clockwise seen from outside. A vertex whose
+// three planes are bisectors with points a, b, c is the circumcentre of the Delaunay tet
+// (p, a, b, c); p writes that tet when it is the smallest of the four indices and none is a far
+// point (tets with a far point are the reference's tets at infinity, which it drops).
+//
+// Consistency. Whether q cuts vertex (p, a, b, c) is the insphere predicate of five points. Where
+// the cached vertex position cannot decide it safely, it is evaluated the same way in every cell
+// that asks: points sorted by index and translated to the first, FP64 with an error bound, else
+// exactly in 512-bit integers, and exact ties are broken by symbolic perturbation of the lifted
+// coordinate (lift_k -= eps^k for point index k). That is a real (infinitesimally weighted)
+// configuration, so co-spherical sets (lattices, cube corners) and duplicate points (the lowest
+// index keeps its cell) come out as one valid triangulation, the same seen from every cell.
+#include "common.h"
+
+#include <float.h>
+
+namespace rmesh {
+namespace {
+
+constexpr int kMaxT = 256;                  // cell vertices (dual triangles) per cell
+constexpr int kMaxP = 128;                  // planes per cell, the 4 far points' included
+constexpr int kTPerLane = kMaxT / kWave;    // 4
+constexpr double kQuickMargin = 1e-7;       // relative margin of the geometric cut test
+constexpr double kFragile = 1e-6;           // |det(n1,n2,n3)| / (|n1||n2||n3|) below this: exact test
+constexpr double kReach = 1.0 + 1e-6;       // safety factor on the 2|v-p| reach bound
+
+// info_dev slots (int64)
+enum { kInfoTets = 0, kInfoRealVerts, kInfoOverflow, kInfoDup, kInfoExact, kInfoRange, kInfoTopo, kInfoMaxT,
+       kInfoCount = 8 };
+
+struct GridParams {
+    double lo[3], h[3], inv_h[3];
+    float far[4][3];  // the four far points
+    int g;            // cells per axis
+    int pad;
+};
+
+// ------------------------------------------------------------------------------------------------
+// Exact arithmetic: 512-bit two's complement integers (rare path: loops are kept rolled).
+constexpr int kLimbs = 8;
+struct Big {
+    uint64_t w[kLimbs];
+};
+
+__host__ __device__ inline Big big_from(int64_t v) {
+    Big b;
+    const uint64_t s = v < 0 ? ~0ull : 0ull;
+    b.w[0] = (uint64_t)v;
+#pragma unroll 1
+    for (int i = 1; i < kLimbs; ++i) b.w[i] = s;
+    return b;
+}
+__host__ __device__ inline bool big_isneg(const Big& a) { return (int64_t)a.w[kLimbs - 1] < 0; }
+__host__ __device__ inline Big big_add(const Big& a, const Big& b) {
+    Big r;
+    uint64_t c = 0;
+#pragma unroll 1
+    for (int i = 0; i < kLimbs; ++i) {
+        const uint64_t t = a.w[i] + b.w[i];
+        const uint64_t c1 = t < a.w[i];
+        r.w[i] = t + c;
+        c = c1 + (r.w[i] < t);
+    }
+    return r;
+}
+__host__ __device__ inline Big big_neg(const Big& a) {
+    Big r;
+    uint64_t c = 1;
+#pragma unroll 1
+    for (int i = 0; i < kLimbs; ++i) {
+        const uint64_t t = ~a.w[i];
+        r.w[i] = t + c;
+        c = r.w[i] < t;
+    }
+    return r;
+}
+__host__ __device__ inline Big big_sub(const Big& a, const Big& b) { return big_add(a, big_neg(b)); }
+// Non-negative a shifted left by sh < 64 * kLimbs bits.
+__host__ __device__ inline Big big_shl(const Big& a, int sh) {
+    Big r;
+    const int q = sh >> 6, m = sh & 63;
+#pragma unroll 1
+    for (int i = kLimbs - 1; i >= 0; --i) {
+        const int j = i - q;
+        const uint64_t hi = j >= 0 ? a.w[j] : 0ull;
+        const uint64_t lo = (j - 1 >= 0 && m) ? a.w[j - 1] >> (64 - m) : 0ull;
+        r.w[i] = (m ? hi << m : hi) | lo;
+    }
+    return r;
+}
+// Product truncated to 512 bits: exact whenever the true product fits, as every product here does.
+__host__ __device__ inline Big big_mul(const Big& a, const Big& b) {
+    const bool na = big_isneg(a), nb = big_isneg(b);
+    const Big x = na ? big_neg(a) : a;
+    const Big y = nb ? big_neg(b) : b;
+    Big r;
+#pragma unroll 1
+    for (int i = 0; i < kLimbs; ++i) r.w[i] = 0;
+#pragma unroll 1
+    for (int i = 0; i < kLimbs; ++i) {
+        if (x.w[i] == 0) continue;
+        uint64_t carry = 0;
+#pragma unroll 1
+        for (int j = 0; i + j < kLimbs; ++j) {
+            const unsigned __int128 pr = (unsigned __int128)x.w[i] * y.w[j];
+            const uint64_t lo = (uint64_t)pr, hi = (uint64_t)(pr >> 64);
+            const uint64_t t = r.w[i + j] + lo;
+            uint64_t c1 = t < lo;
+            const uint64_t t2 = t + carry;
+            c1 += t2 < t;
+            r.w[i + j] = t2;
+            carry = hi + c1;
+        }
+    }
+    return na != nb ? big_neg(r) : r;
+}
+__host__ __device__ inline int big_sign(const Big& a) {
+    if (big_isneg(a)) return -1;
+    uint64_t o = 0;
+#pragma unroll 1
+    for (int i = 0; i < kLimbs; ++i) o |= a.w[i];
+    return o ? 1 : 0;
+}
+__host__ __device__ inline Big big_det3(const Big* a, const Big* b, const Big* c) {
+    const Big m0 = big_sub(big_mul(b[1], c[2]), big_mul(b[2], c[1]));
+    const Big m1 = big_sub(big_mul(b[0], c[2]), big_mul(b[2], c[0]));
+    const Big m2 = big_sub(big_mul(b[0], c[1]), big_mul(b[1], c[0]));
+    return big_add(big_sub(big_mul(a[0], m0), big_mul(a[1], m1)), big_mul(a[2], m2));
+}
+
+struct Five {
+    float x[5][3];   // points sorted by index
+};
+
+// Exact sign of the lifted determinant of five points (rows sorted by point index), with ties
+// broken by lift_k -= eps^k. The coordinates are scaled to integers (every float is an integer
+// multiple of the smallest ulp among them); magnitudes below 2^W with W <= 99 keep every
+// intermediate (at most 5 W + 12 bits) inside 512 bits. Returns 2 beyond that range.
+__host__ __device__ __attribute__((noinline)) int insphere_exact(const Five f) {
+    int emin = 1 << 20, emax = -(1 << 20);
+    for (int i = 0; i < 5; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t bits = __builtin_bit_cast(uint32_t, f.x[i][c]);
+            if ((bits & 0x7fffffffu) == 0) continue;
+            const int e = (bits >> 23) & 255;
+            const int lsb = e == 0 ? -149 : e - 150;
+            const int top = e == 0 ? -126 : e - 126;
+            emin = lsb < emin ? lsb : emin;
+            emax = top > emax ? top : emax;
+        }
+    if (emax < emin) return 0;   // all five points at the origin
+    if (emax - emin > 99) return 2;
+    Big X[5][3];
+#pragma unroll 1
+    for (int i = 0; i < 5; ++i)
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t bits = __builtin_bit_cast(uint32_t, f.x[i][c]);
+            Big v = big_from(0);
+            if (bits & 0x7fffffffu) {
+                const int e = (bits >> 23) & 255;
+                const int64_t m = e == 0 ? (int64_t)(bits & 0x7fffffu) : (int64_t)((bits & 0x7fffffu) | 0x800000u);
+                const int lsb = e == 0 ? -149 : e - 150;
+                v = big_shl(big_from(m), lsb - emin);
+                if (bits >> 31) v = big_neg(v);
+            }
+            X[i][c] = v;
+        }
+    Big A[4][3];
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r)
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) A[r][c] = big_sub(X[r + 1][c], X[0][c]);
+    // det[a_r, |a_r|^2] expanded along the lift column (signs -, +, -, +), one row at a time
+    Big D = big_from(0);
+    int sM[4];
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+        int rows[3], k = 0;
+        for (int j = 0; j < 4; ++j)
+            if (j != r) rows[k++] = j;
+        const Big M = big_det3(A[rows[0]], A[rows[1]], A[rows[2]]);
+        sM[r] = big_sign(M);
+        const Big L = big_add(big_add(big_mul(A[r][0], A[r][0]), big_mul(A[r][1], A[r][1])), big_mul(A[r][2], A[r][2]));
+        const Big t = big_mul(L, M);
+        D = (r & 1) ? big_add(D, t) : big_sub(D, t);
+    }
+    const int s = big_sign(D);
+    if (s != 0) return s;
+    // D_eps = D - sum_k eps^k C_k: the first non-zero cofactor, lowest point index first, decides.
+    // C_0 = det3(x2-x1, x3-x1, x4-x1); C_k = (-1)^k M_{k-1} for k >= 1.
+    Big B[3][3];
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        B[0][c] = big_sub(A[1][c], A[0][c]);
+        B[1][c] = big_sub(A[2][c], A[0][c]);
+        B[2][c] = big_sub(A[3][c], A[0][c]);
+    }
+    const int c0 = big_sign(big_det3(B[0], B[1], B[2]));
+    if (c0 != 0) return -c0;
+    for (int k = 1; k < 5; ++k) {
+        const int ck = sM[k - 1];
+        if (ck != 0) return (k & 1) ? ck : -ck;
+    }
+    return 0;   // all five coplanar
+}
+
+__host__ __device__ inline double det3d(const double* a, const double* b, const double* c) {
+    return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+           a[2] * (b[0] * c[1] - b[1] * c[0]);
+}
+__host__ __device__ inline double perm3d(const double* a, const double* b, const double* c) {
+    return fabs(a[0]) * (fabs(b[1] * c[2]) + fabs(b[2] * c[1])) + fabs(a[1]) * (fabs(b[0] * c[2]) + fabs(b[2] * c[0])) +
+           fabs(a[2]) * (fabs(b[0] * c[1]) + fabs(b[1] * c[0]));
+}
+
+// Sign of the perturbed lifted determinant of five points given in increasing index order.
+// *exact counts evaluations that needed the integer path; *range those outside its range.
+__host__ __device__ inline int insphere_sorted(const Five& f, int* exact, int* range) {
+    double a[4][3], L[4];
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 3; ++c) a[r][c] = (double)f.x[r + 1][c] - (double)f.x[0][c];
+        L[r] = a[r][0] * a[r][0] + a[r][1] * a[r][1] + a[r][2] * a[r][2];
+    }
+    const double M0 = det3d(a[1], a[2], a[3]), M1 = det3d(a[0], a[2], a[3]);
+    const double M2 = det3d(a[0], a[1], a[3]), M3 = det3d(a[0], a[1], a[2]);
+    const double D = (L[1] * M1 - L[0] * M0) + (L[3] * M3 - L[2] * M2);
+    const double P = L[0] * perm3d(a[1], a[2], a[3]) + L[1] * perm3d(a[0], a[2], a[3]) +
+                     L[2] * perm3d(a[0], a[1], a[3]) + L[3] * perm3d(a[0], a[1], a[2]);
+    if (fabs(D) > 1e-13 * P) return D > 0 ? 1 : -1;
+    ++*exact;
+    const int s = insphere_exact(f);
+    if (s == 2) {
+        ++*range;
+        return D > 0 ? 1 : (D < 0 ? -1 : 0);
+    }
+    return s;
+}
+
+// True when q lies strictly inside the circumsphere of the positively oriented tet (p, a, b, c),
+// under the perturbation: the lifted determinant D(p, a, b, c, q) is negative.
+__host__ __device__ __attribute__((noinline)) bool insphere_inside(const int* id_in, const float (*x_in)[3], int* exact, int* range) {
+    int id[5];
+    int o[5] = {0, 1, 2, 3, 4};
+    for (int i = 0; i < 5; ++i) id[i] = id_in[i];
+    bool odd = false;
+    // insertion sort of five indices (tracking the permutation parity)
+    for (int i = 1; i < 5; ++i)
+        for (int j = i; j > 0 && id[j - 1] > id[j]; --j) {
+            const int t = id[j]; id[j] = id[j - 1]; id[j - 1] = t;
+            const int u = o[j]; o[j] = o[j - 1]; o[j - 1] = u;
+            odd = !odd;
+        }
+    Five f;
+    for (int i = 0; i < 5; ++i)
+        for (int c = 0; c < 3; ++c) f.x[i][c] = x_in[o[i]][c];
+    const int s = insphere_sorted(f, exact, range);
+    return (odd ? -s : s) < 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Grid construction.
+__global__ __launch_bounds__(256) void dt_stats_kernel(const float* __restrict__ pts, int64_t n, double* partial) {
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX}, s[3] = {0, 0, 0},
+           s2[3] = {0, 0, 0};
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        for (int c = 0; c < 3; ++c) {
+            const double v = pts[3 * i + c];
+            mn[c] = fmin(mn[c], v);
+            mx[c] = fmax(mx[c], v);
+            s[c] += v;
+            s2[c] += v * v;
+        }
+    }
+    __shared__ double red[12][256];
+    for (int c = 0; c < 3; ++c) {
+        red[c][threadIdx.x] = mn[c];
+        red[3 + c][threadIdx.x] = mx[c];
+        red[6 + c][threadIdx.x] = s[c];
+        red[9 + c][threadIdx.x] = s2[c];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 12; ++k) {
+                const double u = red[k][threadIdx.x + w], &v = red[k][threadIdx.x];
+                red[k][threadIdx.x] = k < 3 ? fmin(v, u) : (k < 6 ? fmax(v, u) : v + u);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) partial[12 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void dt_grid_kernel(const double* partial, int nb, int64_t n, int g, GridParams* gp,
+                               int64_t* info) {
+    if (threadIdx.x != 0) return;
+    double mn[3], mx[3], s[3], s2[3];
+    for (int c = 0; c < 3; ++c) {
+        mn[c] = DBL_MAX; mx[c] = -DBL_MAX; s[c] = 0; s2[c] = 0;
+    }
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], partial[12 * b + c]);
+            mx[c] = fmax(mx[c], partial[12 * b + 3 + c]);
+            s[c] += partial[12 * b + 6 + c];
+            s2[c] += partial[12 * b + 9 + c];
+        }
+    GridParams p;
+    double ext = 0, mag = 0;
+    for (int c = 0; c < 3; ++c) {
+        ext = fmax(ext, mx[c] - mn[c]);
+        mag = fmax(mag, fmax(fabs(mn[c]), fabs(mx[c])));
+    }
+    for (int c = 0; c < 3; ++c) {
+        // robust extent: outliers beyond mean +- 3 sigma are clamped into the boundary cells
+        const double mean = s[c] / (double)n;
+        const double sd = sqrt(fmax(s2[c] / (double)n - mean * mean, 0.0));
+        double lo = fmax(mn[c], mean - 3 * sd), hi = fmin(mx[c], mean + 3 * sd);
+        if (!(hi - lo > 1e-30 + 1e-12 * mag)) {
+            lo = mn[c];
+            hi = lo + fmax(ext, fmax(1e-30, 1e-12 * mag));
+        }
+        p.lo[c] = lo;
+        p.h[c] = (hi - lo) / g;
+        p.inv_h[c] = 1.0 / p.h[c];
+    }
+    // far points: a regular tetrahedron around the bounding box, 1e8 times its size
+    const double R = 1e8 * fmax(ext, fmax(1e-30, 1e-12 * mag));
+    const double u[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+    for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < 3; ++c) p.far[k][c] = (float)(0.5 * (mn[c] + mx[c]) + R * u[k][c]);
+    p.g = g;
+    p.pad = 0;
+    *gp = p;
+    for (int k = 0; k < kInfoCount; ++k) info[k] = 0;
+}
+
+__device__ inline int cell_coord(double v, const GridParams& g, int c) {
+    const double t = floor((v - g.lo[c]) * g.inv_h[c]);
+    return t < 0 ? 0 : (t > g.g - 1 ? g.g - 1 : (int)t);
+}
+
+__global__ __launch_bounds__(256) void dt_keys_kernel(const float* __restrict__ pts, int64_t n,
+                                                      const GridParams* __restrict__ gp, int64_t* keys,
+                                                      int32_t* vals) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n) return;
+    const GridParams g = *gp;
+    const int cx = cell_coord(pts[3 * i], g, 0), cy = cell_coord(pts[3 * i + 1], g, 1),
+              cz = cell_coord(pts[3 * i + 2], g, 2);
+    keys[i] = ((int64_t)cz * g.g + cy) * g.g + cx;
+    vals[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void dt_zero_ranges_kernel(int2* r, int64_t m) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i < m) r[i] = make_int2(0, 0);
+}
+
+__global__ __launch_bounds__(256) void dt_ranges_kernel(const int64_t* __restrict__ keys, int64_t n, int2* r) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) r[k].x = (int)i;
+    if (i == n - 1 || keys[i + 1] != k) r[k].y = (int)(i + 1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Cell kernel: one wave (one workgroup) per point.
+struct __align__(16) CellSmem {
+    float4 pl[kMaxP];          // xyz: neighbour position; w: its index (int bits; far points n..n+3)
+    uint32_t tri[kMaxT];       // plane slots a | b << 8 | c << 16, bit 24: vertex ill-conditioned
+    double vx[kMaxT], vy[kMaxT], vz[kMaxT];   // vertex position relative to p
+    uint32_t newtri[kMaxT];
+    uint8_t clip[kMaxT];
+    double cd2[kWave];         // candidates of one round, unsorted
+    int cid[kWave];
+    double sd2[kWave];         // the same, by increasing (distance, index)
+    int sid[kWave];
+    float4 spt[kWave];
+    int n_new;
+};
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline uint32_t wave_or(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+__device__ inline int lanes_below(uint64_t m, int lane) {
+    return __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Bisector of p and the point in plane slot `slot`, as n . x <= r with x relative to p.
+__device__ inline void plane_of(const CellSmem& S, int slot, double px, double py, double pz, double n[3],
+                                double& r) {
+    const float4 P = S.pl[slot];
+    n[0] = (double)P.x - px;
+    n[1] = (double)P.y - py;
+    n[2] = (double)P.z - pz;
+    r = 0.5 * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+}
+
+// Vertex where three planes meet (Cramer's rule); sets *fragile when ill-conditioned.
+__device__ inline void vertex_of(const double a[3], double ra, const double b[3], double rb, const double c[3],
+                                 double rc, double& x, double& y, double& z, bool& fragile) {
+    const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]),
+                 nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    fragile = !(fabs(det) > kFragile * na * nb * nc);
+    if (!(fabs(det) > 0.0)) {
+        x = y = 0.0;
+        z = 1e150;
+        return;
+    }
+    const double inv = 1.0 / det;
+    x = (ra * bc[0] + rb * ca[0] + rc * ab[0]) * inv;
+    y = (ra * bc[1] + rb * ca[1] + rc * ab[1]) * inv;
+    z = (ra * bc[2] + rb * ca[2] + rc * ab[2]) * inv;
+    if (!(fabs(x) < 1e150 && fabs(y) < 1e150 && fabs(z) < 1e150)) {
+        x = y = 0.0;
+        z = 1e150;
+        fragile = true;
+    }
+}
+
+// Shell cell idx (0 <= idx < 24 r^2 + 2) at Chebyshev radius r >= 1.
+__device__ inline void shell_cell(int r, int idx, int& dx, int& dy, int& dz) {
+    const int a = 2 * r + 1, b = 2 * r - 1;
+    if (idx < 2 * a * a) {
+        dz = idx < a * a ? -r : r;
+        const int k = idx % (a * a);
+        dx = k % a - r;
+        dy = k / a - r;
+        return;
+    }
+    idx -= 2 * a * a;
+    if (idx < 2 * a * b) {
+        dy = idx < a * b ? -r : r;
+        const int k = idx % (a * b);
+        dx = k % a - r;
+        dz = k / a - (r - 1);
+        return;
+    }
+    idx -= 2 * a * b;
+    dx = idx < b * b ? -r : r;
+    const int k = idx % (b * b);
+    dy = k % b - (r - 1);
+    dz = k / b - (r - 1);
+}
+
+__global__ __launch_bounds__(64) void dt_cell_kernel(const float* __restrict__ pts, int64_t n,
+                                                     const int32_t* __restrict__ cell_pts,
+                                                     const int2* __restrict__ cell_rng,
+                                                     const GridParams* __restrict__ gpp, int4* __restrict__ app,
+                                                     int64_t app_cap, unsigned long long* __restrict__ app_count,
+                                                     int32_t* __restrict__ pt_count, int64_t* __restrict__ pt_off,
+                                                     int64_t* __restrict__ info) {
+    __shared__ CellSmem S;
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    const GridParams g = *gpp;
+    const float pfx = pts[3 * (int64_t)p], pfy = pts[3 * (int64_t)p + 1], pfz = pts[3 * (int64_t)p + 2];
+    const double px = pfx, py = pfy, pz = pfz;
+    const int cx = cell_coord(px, g, 0), cy = cell_coord(py, g, 1), cz = cell_coord(pz, g, 2);
+    const double pc[3] = {px, py, pz};
+    const int cc[3] = {cx, cy, cz};
+
+    // starting cell: p's Voronoi cell among the four far points (plane slots 0..3), a tetrahedron
+    // with one vertex per triple of far points, each triple ordered counter-clockwise seen from
+    // outside (det of the three plane normals > 0)
+    if (lane < 4)
+        S.pl[lane] = make_float4(g.far[lane][0], g.far[lane][1], g.far[lane][2], __int_as_float((int)n + lane));
+    __syncthreads();
+    if (lane < 4) {
+        uint32_t s3[3];
+        int k = 0;
+        for (int j = 0; j < 4; ++j)
+            if (j != lane) s3[k++] = (uint32_t)j;
+        double na[3], nb[3], nc[3], ra, rb, rc;
+        plane_of(S, s3[0], px, py, pz, na, ra);
+        plane_of(S, s3[1], px, py, pz, nb, rb);
+        plane_of(S, s3[2], px, py, pz, nc, rc);
+        if (det3d(na, nb, nc) < 0) {
+            const uint32_t t = s3[0]; s3[0] = s3[1]; s3[1] = t;
+            for (int c = 0; c < 3; ++c) { const double u = na[c]; na[c] = nb[c]; nb[c] = u; }
+            const double u = ra; ra = rb; rb = u;
+        }
+        double x, y, z;
+        bool fr;
+        vertex_of(na, ra, nb, rb, nc, rc, x, y, z, fr);
+        S.tri[lane] = s3[0] | s3[1] << 8 | s3[2] << 16 | (fr ? 1u << 24 : 0u);
+        S.vx[lane] = x;
+        S.vy[lane] = y;
+        S.vz[lane] = z;
+    }
+    int nt = 4;
+    uint32_t used[4] = {0xfu, 0u, 0u, 0u};
+    bool dead = false, fail = false;
+    int exact = 0, range = 0, topo = 0;
+    __syncthreads();
+    double maxr2;
+    {
+        double r2 = 0.0;
+        if (lane < 4) {
+            const double rr = S.vx[lane] * S.vx[lane] + S.vy[lane] * S.vy[lane] + S.vz[lane] * S.vz[lane];
+            r2 = ((S.tri[lane] >> 24) & 1u) ? 4.0 * rr : rr;
+        }
+        maxr2 = wave_max(r2);
+    }
+
+    for (int r = 0; !dead && !fail; ++r) {
+        if (r > 0) {
+            // lower bound on |q - p| for every point in a cell at Chebyshev distance >= r
+            double L = DBL_MAX;
+            for (int c = 0; c < 3; ++c) {
+                if (cc[c] + r <= g.g - 1) L = fmin(L, g.lo[c] + (cc[c] + r) * g.h[c] - pc[c]);
+                if (cc[c] - r >= 0) L = fmin(L, pc[c] - (g.lo[c] + (cc[c] - r + 1) * g.h[c]));
+            }
+            if (L == DBL_MAX) break;   // every cell visited
+            L *= 1.0 - 1e-12;
+            if (L > 0 && L * L > 4.0 * maxr2 * kReach) break;
+        }
+        const int count = r == 0 ? 1 : 24 * r * r + 2;
+        for (int base = 0; base < count && !dead && !fail; base += kWave) {
+            int s = 0, e = 0;
+            const int idx = base + lane;
+            if (idx < count) {
+                int dx = 0, dy = 0, dz = 0;
+                if (r > 0) shell_cell(r, idx, dx, dy, dz);
+                const int x = cx + dx, y = cy + dy, z = cz + dz;
+                if (x >= 0 && y >= 0 && z >= 0 && x < g.g && y < g.g && z < g.g) {
+                    const int2 rg = cell_rng[((int64_t)z * g.g + y) * g.g + x];
+                    s = rg.x;
+                    e = rg.y;
+                }
+            }
+            for (int m = 0;; ++m) {
+                const bool have = s + m < e;
+                if (!__ballot(have)) break;
+                int q = -1;
+                float qx = 0.f, qy = 0.f, qz = 0.f;
+                double d2 = 0.0;
+                bool valid = false, dup = false;
+                if (have) {
+                    q = cell_pts[s + m];
+                    qx = pts[3 * (int64_t)q];
+                    qy = pts[3 * (int64_t)q + 1];
+                    qz = pts[3 * (int64_t)q + 2];
+                    const double dx = (double)qx - px, dy = (double)qy - py, dz = (double)qz - pz;
+                    d2 = dx * dx + dy * dy + dz * dz;
+                    if (q != p) {
+                        if (d2 == 0.0) {
+                            dup = q < p;   // a lower-index copy of p owns this position
+                        } else if (d2 < 4.0 * maxr2 * kReach) {
+                            // q can cut the cell only if some vertex is closer to q than to p
+                            valid = false;
+                            for (int t = 0; t < nt && !valid; ++t) {
+                                const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
+                                const double sdist = vx * dx + vy * dy + vz * dz - 0.5 * d2;
+                                const double mag = sqrt((vx * vx + vy * vy + vz * vz) * d2) + d2;
+                                if (sdist > -kQuickMargin * mag) valid = true;
+                                else if ((S.tri[t] >> 24) & 1u)
+                                    valid = d2 < 16.0 * (vx * vx + vy * vy + vz * vz) * kReach;
+                            }
+                        }
+                    }
+                }
+                if (__ballot(dup)) {
+                    dead = true;
+                    break;
+                }
+                const uint64_t vm = __ballot(valid);
+                const int nv = __popcll(vm);
+                if (nv == 0) continue;
+                S.cd2[lane] = valid ? d2 : DBL_MAX;
+                S.cid[lane] = valid ? q : 0x7fffffff;
+                __syncthreads();
+                if (valid) {
+                    int rank = 0;
+                    for (int j = 0; j < kWave; ++j) {
+                        const double dj = S.cd2[j];
+                        const int ij = S.cid[j];
+                        rank += (dj < d2) || (dj == d2 && ij < q);
+                    }
+                    S.sd2[rank] = d2;
+                    S.sid[rank] = q;
+                    S.spt[rank] = make_float4(qx, qy, qz, 0.f);
+                }
+                __syncthreads();
+                for (int k = 0; k < nv && !fail; ++k) {
+                    // ---- clip the cell with the bisector of p and candidate k
+                    const int cq = S.sid[k];
+                    const float4 Q = S.spt[k];
+                    const double ddx = (double)Q.x - px, ddy = (double)Q.y - py, ddz = (double)Q.z - pz;
+                    const double cd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (!(cd2 < 4.0 * maxr2 * kReach)) continue;
+                    bool mc[kTPerLane];
+                    uint32_t mtw[kTPerLane];
+                    double mx[kTPerLane], my[kTPerLane], mz[kTPerLane];
+                    bool any = false;
+#pragma unroll
+                    for (int j = 0; j < kTPerLane; ++j) {
+                        const int t = lane + kWave * j;
+                        bool c = false;
+                        mtw[j] = 0;
+                        mx[j] = my[j] = mz[j] = 0.0;
+                        if (t < nt) {
+                            const uint32_t tw = S.tri[t];
+                            mtw[j] = tw;
+                            const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
+                            mx[j] = vx;
+                            my[j] = vy;
+                            mz[j] = vz;
+                            const double sdist = vx * ddx + vy * ddy + vz * ddz - 0.5 * cd2;
+                            const double mag = sqrt((vx * vx + vy * vy + vz * vz) * cd2) + cd2;
+                            if (!((tw >> 24) & 1u) && fabs(sdist) > kQuickMargin * mag) {
+                                c = sdist > 0.0;
+                            } else {
+                                const float4 A = S.pl[tw & 255u], B = S.pl[(tw >> 8) & 255u], C = S.pl[(tw >> 16) & 255u];
+                                const int ids[5] = {p, __float_as_int(A.w), __float_as_int(B.w), __float_as_int(C.w), cq};
+                                const float xs[5][3] = {{pfx, pfy, pfz}, {A.x, A.y, A.z}, {B.x, B.y, B.z},
+                                                        {C.x, C.y, C.z}, {Q.x, Q.y, Q.z}};
+                                c = insphere_inside(ids, xs, &exact, &range);
+                            }
+                            S.clip[t] = c ? 1 : 0;
+                        }
+                        mc[j] = c;
+                        any |= c;
+                    }
+                    if (!__ballot(any)) continue;
+                    // a free plane slot for q
+                    int slot = -1;
+#pragma unroll
+                    for (int w = 3; w >= 0; --w)
+                        if (used[w] != 0xffffffffu) slot = 32 * w + __builtin_ctz(~used[w]);
+                    if (slot < 0) {
+                        fail = true;
+                        break;
+                    }
+                    if (lane == 0) {
+                        S.pl[slot] = make_float4(Q.x, Q.y, Q.z, __int_as_float(cq));
+                        S.n_new = 0;
+                    }
+                    __syncthreads();
+                    // boundary edges of the clipped region: edge (i -> j) of a clipped triangle whose
+                    // neighbour (the triangle holding j -> i) survives gives the new triangle (i, j, slot)
+#pragma unroll
+                    for (int j = 0; j < kTPerLane; ++j) {
+                        if (!__ballot(mc[j])) continue;
+                        const uint32_t a = mtw[j] & 255u, b = (mtw[j] >> 8) & 255u, c = (mtw[j] >> 16) & 255u;
+                        int nb0 = -1, nb1 = -1, nb2 = -1;   // clip state of the neighbour across ab, bc, ca
+                        for (int u = 0; u < nt; ++u) {
+                            const uint32_t uw = S.tri[u];
+                            const uint32_t u0 = uw & 255u, u1 = (uw >> 8) & 255u, u2 = (uw >> 16) & 255u;
+                            const int uc = S.clip[u];
+                            if ((u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a)) nb0 = uc;
+                            if ((u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b)) nb1 = uc;
+                            if ((u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c)) nb2 = uc;
+                        }
+                        if (mc[j]) {
+                            if (nb0 < 0 || nb1 < 0 || nb2 < 0) ++topo;
+                            const uint32_t sl = (uint32_t)slot << 16;
+                            if (nb0 == 0) {
+                                const int pos = atomicAdd(&S.n_new, 1);
+                                if (pos < kMaxT) S.newtri[pos] = a | b << 8 | sl;
+                            }
+                            if (nb1 == 0) {
+                                const int pos = atomicAdd(&S.n_new, 1);
+                                if (pos < kMaxT) S.newtri[pos] = b | c << 8 | sl;
+                            }
+                            if (nb2 == 0) {
+                                const int pos = atomicAdd(&S.n_new, 1);
+                                if (pos < kMaxT) S.newtri[pos] = c | a << 8 | sl;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const int n_new = S.n_new;
+                    // compaction: survivors first, in order, then the new triangles
+                    int kept = 0;
+                    int pos[kTPerLane];
+#pragma unroll
+                    for (int j = 0; j < kTPerLane; ++j) {
+                        const bool keep = lane + kWave * j < nt && !mc[j];
+                        const uint64_t km = __ballot(keep);
+                        pos[j] = keep ? kept + lanes_below(km, lane) : -1;
+                        kept += __popcll(km);
+                    }
+                    if (kept + n_new > kMaxT) {
+                        fail = true;
+                        break;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < kTPerLane; ++j)
+                        if (pos[j] >= 0) {
+                            S.tri[pos[j]] = mtw[j];
+                            S.vx[pos[j]] = mx[j];
+                            S.vy[pos[j]] = my[j];
+                            S.vz[pos[j]] = mz[j];
+                        }
+                    for (int t = lane; t < n_new; t += kWave) {
+                        const uint32_t tw = S.newtri[t];
+                        double na[3], nb[3], nc[3], ra, rb, rc;
+                        plane_of(S, tw & 255u, px, py, pz, na, ra);
+                        plane_of(S, (tw >> 8) & 255u, px, py, pz, nb, rb);
+                        plane_of(S, (tw >> 16) & 255u, px, py, pz, nc, rc);
+                        double x, y, z;
+                        bool fr;
+                        vertex_of(na, ra, nb, rb, nc, rc, x, y, z, fr);
+                        S.tri[kept + t] = tw | (fr ? 1u << 24 : 0u);
+                        S.vx[kept + t] = x;
+                        S.vy[kept + t] = y;
+                        S.vz[kept + t] = z;
+                    }
+                    nt = kept + n_new;
+                    __syncthreads();
+                    // planes still in use, and the reach of the cell
+                    uint32_t u4[4] = {0u, 0u, 0u, 0u};
+                    double r2 = 0.0;
+#pragma unroll
+                    for (int j = 0; j < kTPerLane; ++j) {
+                        const int t = lane + kWave * j;
+                        if (t < nt) {
+                            const uint32_t tw = S.tri[t];
+                            const uint32_t sa = tw & 255u, sb = (tw >> 8) & 255u, sc = (tw >> 16) & 255u;
+                            u4[sa >> 5] |= 1u << (sa & 31);
+                            u4[sb >> 5] |= 1u << (sb & 31);
+                            u4[sc >> 5] |= 1u << (sc & 31);
+                            const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
+                            const double rr = vx * vx + vy * vy + vz * vz;
+                            r2 = fmax(r2, ((tw >> 24) & 1u) ? 4.0 * rr : rr);
+                        }
+                    }
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) used[w] = wave_or(u4[w]);
+                    maxr2 = wave_max(r2);
+                }
+                __syncthreads();
+                if (fail) break;
+            }
+        }
+    }
+
+    // write the tets this point owns
+    int nreal = 0, nemit = 0;
+    bool em[kTPerLane];
+    int4 tet[kTPerLane];
+    if (!dead && !fail) {
+#pragma unroll
+        for (int j = 0; j < kTPerLane; ++j) {
+            const int t = lane + kWave * j;
+            em[j] = false;
+            tet[j] = make_int4(0, 0, 0, 0);
+            bool real = false;
+            if (t < nt) {
+                const uint32_t tw = S.tri[t];
+                const int a = __float_as_int(S.pl[tw & 255u].w), b = __float_as_int(S.pl[(tw >> 8) & 255u].w),
+                          c = __float_as_int(S.pl[(tw >> 16) & 255u].w);
+                real = a < n && b < n && c < n;   // no far point
+                if (real && p < a && p < b && p < c) {
+                    em[j] = true;
+                    // rotate (a, b, c) cyclically to put the smallest first: orientation is kept
+                    if (a < b && a < c) tet[j] = make_int4(p, a, b, c);
+                    else if (b < c) tet[j] = make_int4(p, b, c, a);
+                    else tet[j] = make_int4(p, c, a, b);
+                }
+            }
+            nreal += __popcll(__ballot(real));
+            nemit += __popcll(__ballot(em[j]));
+        }
+    }
+    unsigned long long off = 0;
+    if (lane == 0) {
+        if (nemit > 0) off = atomicAdd(app_count, (unsigned long long)nemit);
+        pt_count[p] = nemit;
+        pt_off[p] = (int64_t)off;
+        if (nreal) atomicAdd((unsigned long long*)&info[kInfoRealVerts], (unsigned long long)nreal);
+        if (dead) atomicAdd((unsigned long long*)&info[kInfoDup], 1ull);
+        if (fail) atomicAdd((unsigned long long*)&info[kInfoOverflow], 1ull);
+        atomicMax((unsigned long long*)&info[kInfoMaxT], (unsigned long long)nt);
+    }
+    off = __shfl(off, 0, 64);
+    if (!dead && !fail && nemit > 0) {
+        int before = 0;
+#pragma unroll
+        for (int j = 0; j < kTPerLane; ++j) {
+            const uint64_t m = __ballot(em[j]);
+            if (em[j]) {
+                const unsigned long long o = off + before + lanes_below(m, lane);
+                if (o < (unsigned long long)app_cap) app[o] = tet[j];
+            }
+            before += __popcll(m);
+        }
+    }
+    // per-lane counters
+    if (exact) atomicAdd((unsigned long long*)&info[kInfoExact], (unsigned long long)exact);
+    if (range) atomicAdd((unsigned long long*)&info[kInfoRange], (unsigned long long)range);
+    if (topo) atomicAdd((unsigned long long*)&info[kInfoTopo], (unsigned long long)topo);
+}
+
+__global__ __launch_bounds__(256) void dt_gather_kernel(const int4* __restrict__ app, int64_t app_cap,
+                                                        const int32_t* __restrict__ pt_count,
+                                                        const int32_t* __restrict__ pt_incl,
+                                                        const int64_t* __restrict__ pt_off, int64_t n,
+                                                        int4* __restrict__ out, int64_t cap, int64_t* info) {
+    const int64_t p = blockIdx.x * 256ll + threadIdx.x;
+    if (p >= n) return;
+    const int c = pt_count[p];
+    const int64_t dst = (int64_t)pt_incl[p] - c, src = pt_off[p];
+    for (int j = 0; j < c; ++j)
+        if (dst + j < cap && src + j < app_cap) out[dst + j] = app[src + j];
+    if (p == n - 1) info[kInfoTets] = pt_incl[p];
+}
+
+inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline int grid_dim(int64_t n) {
+    int g = (int)llround(cbrt((double)n / 2.0));
+    return g < 1 ? 1 : (g > 512 ? 512 : g);
+}
+
+struct DtLayout {
+    size_t keys, keys2, vals, vals2, sortws, ranges, partial, gp, app, cnt, incl, off, scanws, appcount, total;
+    int g, nb;
+};
+
+inline DtLayout dt_layout(int64_t n, int64_t cap) {
+    DtLayout L;
+    L.g = grid_dim(n);
+    L.nb = (int)std::min<int64_t>(1024, (n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
+    const int64_t cells = (int64_t)L.g * L.g * L.g;
+    size_t o = 0;
+    L.keys = o; o += al(8 * (size_t)n);
+    L.keys2 = o; o += al(8 * (size_t)n);
+    L.vals = o; o += al(4 * (size_t)n);
+    L.vals2 = o; o += al(4 * (size_t)n);
+    L.sortws = o; o += al(rm_sort_pairs_workspace_size(n));
+    L.ranges = o; o += al(8 * (size_t)cells);
+    L.partial = o; o += al(12 * 8 * (size_t)L.nb);
+    L.gp = o; o += al(sizeof(GridParams));
+    L.app = o; o += al(16 * (size_t)(cap > 0 ? cap : 1));
+    L.cnt = o; o += al(4 * (size_t)n);
+    L.incl = o; o += al(4 * (size_t)n);
+    L.off = o; o += al(8 * (size_t)n);
+    L.scanws = o; o += al(rm_scan_workspace_size(n));
+    L.appcount = o; o += al(8);
+    L.total = o;
+    return L;
+}
+
+__global__ void dt_zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
+__global__ void dt_zero_info_kernel(int64_t* info) {
+    if (threadIdx.x < kInfoCount) info[threadIdx.x] = 0;
+}
+
+}  // namespace
+}  // namespace rmesh
+
+using namespace rmesh;
+
+extern "C" size_t rm_delaunay_workspace_size(int64_t n_points, int64_t tet_capacity) {
+    if (n_points < 1) n_points = 1;
+    return dt_layout(n_points, tet_capacity).total;
+}
+
+extern "C" int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* tets_dev, int64_t tet_capacity,
+                           int64_t* info_dev, void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n_points >= 0 && n_points < (1ll << 30), "rm_delaunay: n_points out of range");
+    RM_CHECK_ARG(tet_capacity >= 0 && tet_capacity < (1ll << 31), "rm_delaunay: tet_capacity out of range");
+    RM_CHECK_ARG(info_dev != nullptr, "rm_delaunay: null info");
+    hipStream_t st = as_stream(stream);
+    const int64_t n = n_points;
+    if (n == 0) {
+        // nothing to triangulate: clear the counters so the caller reads zero tets
+        hipLaunchKernelGGL(dt_zero_info_kernel, dim3(1), dim3(kInfoCount), 0, st, info_dev);
+        RM_CHECK_LAUNCH("rm_delaunay");
+        return RM_OK;
+    }
+    RM_CHECK_ARG(points_dev && workspace_dev, "rm_delaunay: null buffer");
+    RM_CHECK_ARG(tet_capacity == 0 || tets_dev != nullptr, "rm_delaunay: null tets");
+    const DtLayout L = dt_layout(n, tet_capacity);
+    if (workspace_bytes < L.total) {
+        set_error("rm_delaunay: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    char* ws = (char*)workspace_dev;
+    int64_t* keys = (int64_t*)(ws + L.keys);
+    int64_t* keys2 = (int64_t*)(ws + L.keys2);
+    int32_t* vals = (int32_t*)(ws + L.vals);
+    int32_t* vals2 = (int32_t*)(ws + L.vals2);
+    int2* ranges = (int2*)(ws + L.ranges);
+    double* partial = (double*)(ws + L.partial);
+    GridParams* gp = (GridParams*)(ws + L.gp);
+    int4* app = (int4*)(ws + L.app);
+    int32_t* cnt = (int32_t*)(ws + L.cnt);
+    int32_t* incl = (int32_t*)(ws + L.incl);
+    int64_t* off = (int64_t*)(ws + L.off);
+    unsigned long long* appcount = (unsigned long long*)(ws + L.appcount);
+    const int64_t cells = (int64_t)L.g * L.g * L.g;
+    const int64_t app_cap = tet_capacity > 0 ? tet_capacity : 1;
+
+    hipLaunchKernelGGL(dt_stats_kernel, dim3(L.nb), dim3(256), 0, st, points_dev, n, partial);
+    hipLaunchKernelGGL(dt_grid_kernel, dim3(1), dim3(1), 0, st, (const double*)partial, L.nb, n, L.g, gp, info_dev);
+    hipLaunchKernelGGL(dt_keys_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, points_dev, n,
+                       (const GridParams*)gp, keys, vals);
+    RM_CHECK_LAUNCH("rm_delaunay grid");
+    int end_bit = 1;
+    while ((1ll << end_bit) < cells) ++end_bit;
+    int rc = rm_sort_pairs(keys, vals, keys2, vals2, n, end_bit, ws + L.sortws, rm_sort_pairs_workspace_size(n),
+                           stream);
+    if (rc != RM_OK) return rc;
+    hipLaunchKernelGGL(dt_zero_ranges_kernel, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, st, ranges, cells);
+    hipLaunchKernelGGL(dt_ranges_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, (const int64_t*)keys2,
+                       n, ranges);
+    hipLaunchKernelGGL(dt_zero_u64_kernel, dim3(1), dim3(1), 0, st, appcount);
+    hipLaunchKernelGGL(dt_cell_kernel, dim3((unsigned)n), dim3(kWave), 0, st, points_dev, n,
+                       (const int32_t*)vals2, (const int2*)ranges, (const GridParams*)gp, app, app_cap, appcount,
+                       cnt, off, info_dev);
+    RM_CHECK_LAUNCH("rm_delaunay cells");
+    rc = rm_inclusive_scan_i32(cnt, incl, n, ws + L.scanws, rm_scan_workspace_size(n), stream);
+    if (rc != RM_OK) return rc;
+    hipLaunchKernelGGL(dt_gather_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, (const int4*)app,
+                       app_cap, (const int32_t*)cnt, (const int32_t*)incl, (const int64_t*)off, n, (int4*)tets_dev,
+                       tet_capacity, info_dev);
+    RM_CHECK_LAUNCH("rm_delaunay gather");
+    return RM_OK;
+}
+
+// Host evaluation of the cell kernel's insphere predicate (same code), for tests without a GPU:
+// 1 when x[4] lies strictly inside the circumsphere of the positively oriented tet x[0..3] under
+// the index perturbation, else 0. ids[5], xyz[15].
+extern "C" int rm_delaunay_insphere_host(const int32_t* ids, const float* xyz) {
+    float x[5][3];
+    for (int i = 0; i < 5; ++i)
+        for (int c = 0; c < 3; ++c) x[i][c] = xyz[3 * i + c];
+    int e = 0, r = 0;
+    return insphere_inside(ids, x, &e, &r) ? 1 : 0;
+}

@@ -1,0 +1,153 @@
+"""CPU checks of the Delaunay oracle (scipy's Qhull, the reference's high-precision branch at
+models/ingp_color.py:526-528) against exact brute force, and of the insphere predicate the GPU
+construction uses (rm_delaunay_insphere_host runs the kernel's own code on the host) against exact
+rational arithmetic, including the symbolic tie-breaking on co-spherical and duplicate points."""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_scipy_oracle_matches_exact_brute_force(seed):
+    p = np.random.default_rng(seed).uniform(-1, 1, (16, 3)).astype(np.float32)
+    a = O.canonical_tets(O.delaunay(p))
+    b = O.canonical_tets(O.delaunay_brute(p))
+    assert a.shape == b.shape and (a == b).all()
+
+
+def test_scipy_oracle_orientation_and_tiling():
+    """Every oracle tet has positive volume (the reference's post-processing) and the tets tile
+    the convex hull: volumes sum to its volume, interior triangles are shared by exactly two."""
+    from scipy.spatial import ConvexHull
+    p = np.random.default_rng(5).normal(0, 1, (300, 3)).astype(np.float32).astype(np.float64)
+    t = O.delaunay(p)
+    a, b, c, d = (p[t[:, k]] for k in range(4))
+    vol = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) / 6
+    assert (vol > 0).all()
+    assert abs(vol.sum() - ConvexHull(p).volume) < 1e-9 * ConvexHull(p).volume
+    faces = np.sort(np.concatenate([t[:, [1, 2, 3]], t[:, [0, 2, 3]], t[:, [0, 1, 3]], t[:, [0, 1, 2]]]), axis=1)
+    _, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert set(np.unique(cnt)) <= {1, 2}
+    assert (cnt == 1).sum() == len(ConvexHull(p).simplices)
+
+
+def test_brute_force_rejects_a_tet_with_a_point_in_its_sphere():
+    """Point 4 at the circumcentre of tet (0,1,2,3): that tet is not Delaunay, and every tet the
+    brute force keeps has an empty circumsphere."""
+    p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0.5, 0.5]], np.float32)
+    P = O._fracs(p)
+    assert O.inside_sphere_exact(P, (0, 1, 2, 3), 4)
+    t = O.canonical_tets(O.delaunay_brute(p))
+    assert len(t) and not any((row == [0, 1, 2, 3]).all() for row in t)
+    for tet in t:
+        assert not any(O.inside_sphere_exact(P, tuple(tet), q) for q in range(5) if q not in tet)
+
+
+def _host(lib, ids, xyz):
+    ids = np.ascontiguousarray(ids, np.int32)
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    return lib.rm_delaunay_insphere_host(ids.ctypes.data_as(C.c_void_p), xyz.ctypes.data_as(C.c_void_p))
+
+
+@pytest.fixture(scope="module")
+def hostlib():
+    from radiance_meshes_amd import _capi
+    return _capi.lib()
+
+
+def _oriented(pts5):
+    P = O._fracs(pts5)
+    if O.orient_exact(P, 0, 1, 2, 3) < 0:
+        pts5 = pts5[[1, 0, 2, 3, 4]]
+    return pts5
+
+
+def test_predicate_general_position(hostlib):
+    rng = np.random.default_rng(0)
+    n_in = 0
+    for _ in range(400):
+        x = rng.uniform(-3, 3, (5, 3)).astype(np.float32)
+        if O.orient_exact(O._fracs(x), 0, 1, 2, 3) == 0:
+            continue
+        x = _oriented(x)
+        ids = rng.permutation(1000)[:5]
+        want = O.inside_sphere_exact(O._fracs(x), (0, 1, 2, 3), 4)
+        n_in += want
+        assert _host(hostlib, ids, x) == int(want)
+    assert 0 < n_in < 400
+
+
+def test_predicate_near_ties_use_exact_arithmetic(hostlib):
+    """Points a few ulps off a common sphere: FP64 cannot decide, the integer path must."""
+    rng = np.random.default_rng(1)
+    checked = 0
+    for _ in range(300):
+        d = rng.normal(0, 1, (5, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        x = (np.array([0.3, -0.2, 0.1]) + 1.7 * d).astype(np.float32)
+        x[4] = np.nextafter(x[4], x[4] + rng.choice([-1, 1], 3).astype(np.float32))
+        P = O._fracs(x)
+        if O.orient_exact(P, 0, 1, 2, 3) == 0:
+            continue
+        x = _oriented(x)
+        ids = rng.permutation(50)[:5]
+        assert _host(hostlib, ids, x) == int(O.perturbed_inside(x, list(ids)))
+        checked += 1
+    assert checked > 250
+
+
+def test_predicate_ties_follow_the_index_perturbation(hostlib):
+    """Exactly co-spherical five-point sets (cube corners, lattice points on one sphere): the
+    decision equals the sign of the perturbed lifted determinant (lift_k -= eps^id_k), computed
+    independently from the 5x5 determinant's cofactors, for every assignment of indices."""
+    corners = np.array([[i, j, k] for i in (-1, 1) for j in (-1, 1) for k in (-1, 1)], np.float32) * 20
+    lattice = np.array([v for v in itertools.product(range(-2, 3), repeat=3) if sum(c * c for c in v) == 5],
+                       np.float32)
+    rng = np.random.default_rng(2)
+    n = flipped = 0
+    for pool in (corners, lattice):
+        for _ in range(120):
+            x = pool[rng.choice(len(pool), 5, replace=False)]
+            P = O._fracs(x)
+            if O.orient_exact(P, 0, 1, 2, 3) == 0:
+                continue
+            x = _oriented(x)
+            for _ in range(3):
+                ids = list(rng.permutation(40)[:5])
+                want = O.perturbed_inside(x, ids)
+                assert _host(hostlib, ids, x) == int(want), (x, ids)
+                # the opposite tie rule (highest index first) decides some of these differently,
+                # so a sign error in the rule could not pass unnoticed
+                flipped += O.perturbed_inside(x, [-i for i in ids]) != want
+                n += 1
+    assert n > 300 and flipped > 30
+
+
+def test_predicate_duplicates_lowest_index_wins(hostlib):
+    """q at the position of p: inside p's circumspheres exactly when q's index is the lower."""
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], np.float32)
+    assert _host(hostlib, [5, 10, 11, 12, 9], x) == 0
+    assert _host(hostlib, [5, 10, 11, 12, 1], x) == 1
+    assert int(O.perturbed_inside(x, [5, 10, 11, 12, 9])) == 0
+    assert int(O.perturbed_inside(x, [5, 10, 11, 12, 1])) == 1
+
+
+def test_predicate_wide_exponent_range(hostlib):
+    """Far-point magnitudes (2^30) next to 2^-20-sized offsets, on or a hair off a common sphere:
+    the exact path needs ~75-bit integer coordinates here."""
+    s = np.float32(2.0 ** 30)
+    base = np.array([[0, 0, 0], [s, 0, 0], [0, s, 0], [0, 0, s], [s, s, s]], np.float32)
+    rng = np.random.default_rng(7)
+    for k in range(40):
+        x = base.copy()
+        x[rng.integers(0, 5), rng.integers(0, 3)] += np.float32(rng.choice([-1, 0, 1]) * 2.0 ** -20)
+        P = O._fracs(x)
+        if O.orient_exact(P, 0, 1, 2, 3) == 0:
+            continue
+        x = _oriented(x)
+        ids = list(rng.permutation(20)[:5])
+        assert _host(hostlib, ids, x) == int(O.perturbed_inside(x, ids)), (k, x, ids)
