@@ -34,7 +34,8 @@ constexpr int kMaxT = 256;                  // cell vertices (dual triangles) pe
 constexpr int kMaxP = 128;                  // planes per cell, the 4 far points' included
 constexpr int kTPerLane = kMaxT / kWave;    // 4
 constexpr double kQuickMargin = 1e-7;       // relative margin of the geometric cut test
-constexpr double kFragile = 1e-6;           // |det(n1,n2,n3)| / (|n1||n2||n3|) below this: exact test
+constexpr double kCullMargin = 1e-10;       // relative margin of the conservative "cannot cut" tests
+constexpr double kLost = 1e-3;              // vertex error estimate beyond which its position is not used
 constexpr double kReach = 1.0 + 1e-6;       // safety factor on the 2|v-p| reach bound
 
 // info_dev slots (int64)
@@ -264,6 +265,9 @@ __host__ __device__ __attribute__((noinline)) bool insphere_inside(const int* id
             const int u = o[j]; o[j] = o[j - 1]; o[j - 1] = u;
             odd = !odd;
         }
+    // a point that is one of the tet's own vertices is never strictly inside its sphere
+    for (int i = 1; i < 5; ++i)
+        if (id[i] == id[i - 1]) return false;
     Five f;
     for (int i = 0; i < 5; ++i)
         for (int c = 0; c < 3; ++c) f.x[i][c] = x_in[o[i]][c];
@@ -348,6 +352,50 @@ __global__ void dt_grid_kernel(const double* partial, int nb, int64_t n, int g, 
     for (int k = 0; k < kInfoCount; ++k) info[k] = 0;
 }
 
+// Extreme points of the set in 26 directions (the 3x3x3 stencil without its centre): seeds for
+// cells of points outside the grid's range, whose neighbours are the set's outer points.
+constexpr int kExt = 26;
+
+__device__ inline void ext_dir(int k, double u[3]) {
+    const int m = k < 13 ? k : k + 1;   // skip the centre of the stencil
+    u[0] = m % 3 - 1;
+    u[1] = (m / 3) % 3 - 1;
+    u[2] = m / 9 - 1;
+}
+
+__global__ __launch_bounds__(256) void dt_extreme_kernel(const float* __restrict__ pts, int64_t n,
+                                                         unsigned long long* keys) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    const bool ok = i < n;
+    double x[3] = {0, 0, 0};
+    if (ok)
+        for (int c = 0; c < 3; ++c) x[c] = pts[3 * i + c];
+    for (int k = 0; k < kExt; ++k) {
+        double u[3];
+        ext_dir(k, u);
+        unsigned long long key = 0ull;
+        if (ok) {
+            // largest projection first, lowest index among equal (float-rounded) projections
+            const uint32_t b = __float_as_uint((float)(x[0] * u[0] + x[1] * u[1] + x[2] * u[2]));
+            const uint32_t ord = (b >> 31) ? ~b : (b | 0x80000000u);
+            key = ((unsigned long long)ord << 32) | (0xffffffffull - (unsigned long long)i);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long y = __shfl_xor(key, o, 64);
+            key = y > key ? y : key;
+        }
+        if ((threadIdx.x & 63) == 0 && key) atomicMax(&keys[k], key);
+    }
+}
+
+__global__ void dt_extreme_ids_kernel(const unsigned long long* keys, int64_t n, int32_t* ids) {
+    const int k = threadIdx.x;
+    if (k < kExt) {
+        const int64_t i = 0xffffffffll - (int64_t)(keys[k] & 0xffffffffull);
+        ids[k] = (i >= 0 && i < n) ? (int32_t)i : 0;
+    }
+}
+
 __device__ inline int cell_coord(double v, const GridParams& g, int c) {
     const double t = floor((v - g.lo[c]) * g.inv_h[c]);
     return t < 0 ? 0 : (t > g.g - 1 ? g.g - 1 : (int)t);
@@ -379,18 +427,28 @@ __global__ __launch_bounds__(256) void dt_ranges_kernel(const int64_t* __restric
 }
 
 // ------------------------------------------------------------------------------------------------
-// Cell kernel: one wave (one workgroup) per point.
+// Cell kernels: one wave (one workgroup) per point.
+//
+// The cell lives in LDS: plane table pl (point position and index), triangles tri (three 8-bit
+// plane slots and a flag for ill-conditioned vertices) with their vertex positions relative to p.
+// The main pass holds up to 256 vertices and 128 planes; cells that outgrow it are queued and
+// recomputed by a retry pass holding up to 1024 vertices and 256 planes.
+template <int MAXT, int MAXP>
 struct __align__(16) CellSmem {
-    float4 pl[kMaxP];          // xyz: neighbour position; w: its index (int bits; far points n..n+3)
-    uint32_t tri[kMaxT];       // plane slots a | b << 8 | c << 16, bit 24: vertex ill-conditioned
-    double vx[kMaxT], vy[kMaxT], vz[kMaxT];   // vertex position relative to p
-    uint32_t newtri[kMaxT];
-    uint8_t clip[kMaxT];
+    static_assert(MAXP <= 256, "plane slots are 8-bit");
+    float4 pl[MAXP];           // xyz: neighbour position; w: its index (int bits; far points n..n+3)
+    uint32_t tri[MAXT];        // plane slots a | b << 8 | c << 16
+    double4 vt[MAXT];          // vertex position relative to p (xyz), relative error bound of it (w)
+    uint32_t newtri[MAXT];     // triangles created by one clip
+    uint32_t ctri[MAXT];       // triangles removed by one clip
+    uint8_t clip[MAXT];
     double cd2[kWave];         // candidates of one round, unsorted
     int cid[kWave];
     double sd2[kWave];         // the same, by increasing (distance, index)
     int sid[kWave];
     float4 spt[kWave];
+    int bpre[kWave];           // a batch of grid cells: first flat index of each lane's cell
+    int bst[kWave];            // and its first position in the sorted point list
     int n_new;
 };
 
@@ -409,28 +467,32 @@ __device__ inline int lanes_below(uint64_t m, int lane) {
 }
 
 // Bisector of p and the point in plane slot `slot`, as n . x <= r with x relative to p.
-__device__ inline void plane_of(const CellSmem& S, int slot, double px, double py, double pz, double n[3],
+__device__ inline void plane_of(const float4* pl, int slot, double px, double py, double pz, double n[3],
                                 double& r) {
-    const float4 P = S.pl[slot];
+    const float4 P = pl[slot];
     n[0] = (double)P.x - px;
     n[1] = (double)P.y - py;
     n[2] = (double)P.z - pz;
     r = 0.5 * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
 }
 
-// Vertex where three planes meet (Cramer's rule); sets *fragile when ill-conditioned.
+// Vertex where three planes meet (Cramer's rule). err: bound on its relative position error,
+// 1e-15 / kappa with kappa = |det(n1,n2,n3)| / (|n1||n2||n3|) (the conditioning of the system
+// with unit normals, ten times the FP64 rounding estimate); 1 when the planes are degenerate.
 __device__ inline void vertex_of(const double a[3], double ra, const double b[3], double rb, const double c[3],
-                                 double rc, double& x, double& y, double& z, bool& fragile) {
+                                 double rc, double& x, double& y, double& z, float& err) {
     const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
     const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
     const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
     const double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
     const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]),
                  nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    fragile = !(fabs(det) > kFragile * na * nb * nc);
+    const double kappa = fabs(det) / (na * nb * nc);
+    err = kappa > 1e-15 ? (float)fmin(1e-15 / kappa, 1.0) : 1.0f;
     if (!(fabs(det) > 0.0)) {
         x = y = 0.0;
         z = 1e150;
+        err = 1.0f;
         return;
     }
     const double inv = 1.0 / det;
@@ -440,7 +502,7 @@ __device__ inline void vertex_of(const double a[3], double ra, const double b[3]
     if (!(fabs(x) < 1e150 && fabs(y) < 1e150 && fabs(z) < 1e150)) {
         x = y = 0.0;
         z = 1e150;
-        fragile = true;
+        err = 1.0f;
     }
 }
 
@@ -469,28 +531,43 @@ __device__ inline void shell_cell(int r, int idx, int& dx, int& dy, int& dz) {
     dz = k / b - (r - 1);
 }
 
-__global__ __launch_bounds__(64) void dt_cell_kernel(const float* __restrict__ pts, int64_t n,
-                                                     const int32_t* __restrict__ cell_pts,
-                                                     const int2* __restrict__ cell_rng,
-                                                     const GridParams* __restrict__ gpp, int4* __restrict__ app,
-                                                     int64_t app_cap, unsigned long long* __restrict__ app_count,
-                                                     int32_t* __restrict__ pt_count, int64_t* __restrict__ pt_off,
-                                                     int64_t* __restrict__ info) {
-    __shared__ CellSmem S;
+struct CellArgs {
+    const float* pts;
+    int64_t n;
+    const int32_t* cell_pts;
+    const int2* cell_rng;
+    const GridParams* gp;
+    int4* app;
+    int64_t app_cap;
+    unsigned long long* app_count;
+    int32_t* pt_count;
+    int64_t* pt_off;
+    int64_t* info;
+    const int32_t* ext;                // kExt extreme points
+    int32_t* retry;                    // cells that outgrew the main pass (null in the retry pass)
+    unsigned long long* retry_count;
+};
+
+template <int MAXT, int MAXP>
+__device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A) {
+    constexpr int TPL = MAXT / kWave;
+    constexpr int UW = MAXP / 32;
     const int lane = threadIdx.x;
-    const int p = blockIdx.x;
-    const GridParams g = *gpp;
+    const GridParams& gpr = *A.gp;
+    const int gg = gpr.g;
+    const int64_t n = A.n;
+    const float* pts = A.pts;
     const float pfx = pts[3 * (int64_t)p], pfy = pts[3 * (int64_t)p + 1], pfz = pts[3 * (int64_t)p + 2];
     const double px = pfx, py = pfy, pz = pfz;
-    const int cx = cell_coord(px, g, 0), cy = cell_coord(py, g, 1), cz = cell_coord(pz, g, 2);
     const double pc[3] = {px, py, pz};
+    const int cx = cell_coord(px, gpr, 0), cy = cell_coord(py, gpr, 1), cz = cell_coord(pz, gpr, 2);
     const int cc[3] = {cx, cy, cz};
 
     // starting cell: p's Voronoi cell among the four far points (plane slots 0..3), a tetrahedron
     // with one vertex per triple of far points, each triple ordered counter-clockwise seen from
     // outside (det of the three plane normals > 0)
     if (lane < 4)
-        S.pl[lane] = make_float4(g.far[lane][0], g.far[lane][1], g.far[lane][2], __int_as_float((int)n + lane));
+        S.pl[lane] = make_float4(gpr.far[lane][0], gpr.far[lane][1], gpr.far[lane][2], __int_as_float((int)n + lane));
     __syncthreads();
     if (lane < 4) {
         uint32_t s3[3];
@@ -498,326 +575,504 @@ __global__ __launch_bounds__(64) void dt_cell_kernel(const float* __restrict__ p
         for (int j = 0; j < 4; ++j)
             if (j != lane) s3[k++] = (uint32_t)j;
         double na[3], nb[3], nc[3], ra, rb, rc;
-        plane_of(S, s3[0], px, py, pz, na, ra);
-        plane_of(S, s3[1], px, py, pz, nb, rb);
-        plane_of(S, s3[2], px, py, pz, nc, rc);
+        plane_of(S.pl, s3[0], px, py, pz, na, ra);
+        plane_of(S.pl, s3[1], px, py, pz, nb, rb);
+        plane_of(S.pl, s3[2], px, py, pz, nc, rc);
         if (det3d(na, nb, nc) < 0) {
             const uint32_t t = s3[0]; s3[0] = s3[1]; s3[1] = t;
             for (int c = 0; c < 3; ++c) { const double u = na[c]; na[c] = nb[c]; nb[c] = u; }
             const double u = ra; ra = rb; rb = u;
         }
         double x, y, z;
-        bool fr;
-        vertex_of(na, ra, nb, rb, nc, rc, x, y, z, fr);
-        S.tri[lane] = s3[0] | s3[1] << 8 | s3[2] << 16 | (fr ? 1u << 24 : 0u);
-        S.vx[lane] = x;
-        S.vy[lane] = y;
-        S.vz[lane] = z;
+        float er;
+        vertex_of(na, ra, nb, rb, nc, rc, x, y, z, er);
+        S.tri[lane] = s3[0] | s3[1] << 8 | s3[2] << 16;
+        S.vt[lane] = make_double4(x, y, z, er);
     }
     int nt = 4;
-    uint32_t used[4] = {0xfu, 0u, 0u, 0u};
+    uint32_t used[UW];
+#pragma unroll
+    for (int w = 0; w < UW; ++w) used[w] = w == 0 ? 0xfu : 0u;
     bool dead = false, fail = false;
-    int exact = 0, range = 0, topo = 0;
+    int exact = 0, range = 0;
     __syncthreads();
     double maxr2;
     {
         double r2 = 0.0;
         if (lane < 4) {
-            const double rr = S.vx[lane] * S.vx[lane] + S.vy[lane] * S.vy[lane] + S.vz[lane] * S.vz[lane];
-            r2 = ((S.tri[lane] >> 24) & 1u) ? 4.0 * rr : rr;
+            const double4 V = S.vt[lane];
+            const double rr = V.x * V.x + V.y * V.y + V.z * V.z;
+            const double er = V.w;
+            r2 = er < kLost ? rr * (1.0 + 16.0 * er) : 4.0 * rr;
         }
         maxr2 = wave_max(r2);
     }
 
-    for (int r = 0; !dead && !fail; ++r) {
+    // a box (relative to p) can hold a point that cuts the cell only if it meets some ball
+    // B(v, |v - p|) over the cell's vertices v
+    auto box_reach = [&](const double* bl, const double* bh) -> bool {
+        bool reach = false;
+        for (int t = 0; t < nt && !reach; ++t) {
+            const double4 V = S.vt[t];
+            double vx = V.x, vy = V.y, vz = V.z;
+            double rr = vx * vx + vy * vy + vz * vz;
+            const double er = V.w;
+            if (er < kLost) {
+                rr *= 1.0 + 16.0 * er;
+            } else {   // position unreliable: the ball B(p, 2|v - p|)
+                vx = vy = vz = 0.0;
+                rr *= 4.0;
+            }
+            const double ex = fmax(fmax(bl[0] - vx, vx - bh[0]), 0.0);
+            const double ey = fmax(fmax(bl[1] - vy, vy - bh[1]), 0.0);
+            const double ez = fmax(fmax(bl[2] - vz, vz - bh[2]), 0.0);
+            reach = ex * ex + ey * ey + ez * ez <= rr * (1.0 + 1e-9);
+        }
+        return reach;
+    };
+    // box of grid cells [i0, i1] per axis, relative to p; boundary cells are open on their outer
+    // side (they hold the clamped outliers)
+    auto cell_box = [&](const int* i0, const int* i1, double* bl, double* bh) {
+        for (int c = 0; c < 3; ++c) {
+            bl[c] = i0[c] == 0 ? -DBL_MAX : gpr.lo[c] + i0[c] * gpr.h[c] - pc[c];
+            bh[c] = i1[c] == gg - 1 ? DBL_MAX : gpr.lo[c] + (i1[c] + 1) * gpr.h[c] - pc[c];
+        }
+    };
+    // clip with every point of the lanes' point ranges [s, e) (one grid cell per lane)
+    auto process_batch = [&](const int32_t* list, int s, int e) {
+        // the batch's points as one flat list spread over all lanes
+        const int cntl = e - s;
+        int incl = cntl;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const int total = __shfl(incl, kWave - 1, 64);
+        if (total == 0) return;
+        __syncthreads();   // the previous batch's readers are done
+        S.bpre[lane] = incl - cntl;
+        S.bst[lane] = s;
+        __syncthreads();
+        for (int c0 = 0; c0 < total; c0 += kWave) {
+            const int kf = c0 + lane;
+            const bool have = kf < total;
+            int q = -1;
+            float qx = 0.f, qy = 0.f, qz = 0.f;
+            double d2 = 0.0;
+            bool valid = false, dup = false;
+            if (have) {
+                // the last lane whose cell starts at or before kf holds it
+                int lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (S.bpre[mid] <= kf) lo = mid;
+                    else hi = mid - 1;
+                }
+                q = list[S.bst[lo] + (kf - S.bpre[lo])];
+                qx = pts[3 * (int64_t)q];
+                qy = pts[3 * (int64_t)q + 1];
+                qz = pts[3 * (int64_t)q + 2];
+                const double dx = (double)qx - px, dy = (double)qy - py, dz = (double)qz - pz;
+                d2 = dx * dx + dy * dy + dz * dz;
+                if (q != p) {
+                    if (d2 == 0.0) {
+                        dup = q < p;   // a lower-index copy of p owns this position
+                    } else if (d2 < 4.0 * maxr2 * kReach) {
+                        // q can cut the cell only if some vertex is closer to q than to p
+                        for (int t = 0; t < nt && !valid; ++t) {
+                            const double4 V = S.vt[t];
+                            const double vx = V.x, vy = V.y, vz = V.z;
+                            const double sdist = vx * dx + vy * dy + vz * dz - 0.5 * d2;
+                            const double mag = sqrt((vx * vx + vy * vy + vz * vz) * d2) + d2;
+                            const double er = V.w;
+                            if (er < kLost) valid = sdist > -(kCullMargin + 8.0 * er) * mag;
+                            else valid = d2 < 16.0 * (vx * vx + vy * vy + vz * vz) * kReach;
+                        }
+                    }
+                }
+            }
+            if (__ballot(dup)) {
+                dead = true;
+                break;
+            }
+            const int nv = __popcll(__ballot(valid));
+            if (nv == 0) continue;
+            S.cd2[lane] = valid ? d2 : DBL_MAX;
+            S.cid[lane] = valid ? q : 0x7fffffff;
+            __syncthreads();
+            if (valid) {
+                int rank = 0;
+                for (int j = 0; j < kWave; ++j) {
+                    const double dj = S.cd2[j];
+                    const int ij = S.cid[j];
+                    rank += (dj < d2) || (dj == d2 && ij < q);
+                }
+                S.sd2[rank] = d2;
+                S.sid[rank] = q;
+                S.spt[rank] = make_float4(qx, qy, qz, 0.f);
+            }
+            __syncthreads();
+            for (int k = 0; k < nv && !fail; ++k) {
+                // ---- clip the cell with the bisector of p and candidate k
+                const int cq = S.sid[k];
+                const float4 Q = S.spt[k];
+                const double ddx = (double)Q.x - px, ddy = (double)Q.y - py, ddz = (double)Q.z - pz;
+                const double cd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (!(cd2 < 4.0 * maxr2 * kReach)) continue;
+                // 1. which vertices does the plane cut (q strictly inside their circumsphere)
+                int mine = 0;
+#pragma unroll
+                for (int j = 0; j < TPL; ++j) {
+                    const int t = lane + kWave * j;
+                    if (t < nt) {
+                        const uint32_t tw = S.tri[t];
+                        const double4 V = S.vt[t];
+                        const double vx = V.x, vy = V.y, vz = V.z;
+                        const double sdist = vx * ddx + vy * ddy + vz * ddz - 0.5 * cd2;
+                        const double mag = sqrt((vx * vx + vy * vy + vz * vz) * cd2) + cd2;
+                        const double er = V.w;
+                        bool c;
+                        if (er < kLost && fabs(sdist) > (kQuickMargin + 8.0 * er) * mag) {
+                            c = sdist > 0.0;
+                        } else {
+                            const float4 Pa = S.pl[tw & 255u], Pb = S.pl[(tw >> 8) & 255u], Pc = S.pl[(tw >> 16) & 255u];
+                            const int ids[5] = {p, __float_as_int(Pa.w), __float_as_int(Pb.w), __float_as_int(Pc.w), cq};
+                            const float xs[5][3] = {{pfx, pfy, pfz}, {Pa.x, Pa.y, Pa.z}, {Pb.x, Pb.y, Pb.z},
+                                                    {Pc.x, Pc.y, Pc.z}, {Q.x, Q.y, Q.z}};
+                            c = insphere_inside(ids, xs, &exact, &range);
+                        }
+                        S.clip[t] = c ? 1 : 0;
+                        mine += c;
+                    }
+                }
+                if (!__ballot(mine > 0)) continue;
+                // a free plane slot for q
+                int slot = -1;
+#pragma unroll
+                for (int w = UW - 1; w >= 0; --w)
+                    if (used[w] != 0xffffffffu) slot = 32 * w + __builtin_ctz(~used[w]);
+                if (slot < 0) {
+                    fail = true;
+                    break;
+                }
+                // 2. list the removed triangles
+                int nclip = 0;
+#pragma unroll
+                for (int j = 0; j < TPL; ++j) {
+                    const int t = lane + kWave * j;
+                    const bool c = t < nt && S.clip[t];
+                    const uint64_t cm = __ballot(c);
+                    if (c) S.ctri[nclip + lanes_below(cm, lane)] = S.tri[t];
+                    nclip += __popcll(cm);
+                }
+                if (lane == 0) {
+                    S.pl[slot] = make_float4(Q.x, Q.y, Q.z, __int_as_float(cq));
+                    S.n_new = 0;
+                }
+                __syncthreads();
+                // 3. boundary of the removed region: edge (i -> j) of a removed triangle whose
+                // neighbour (the triangle holding j -> i) is not removed gives the new triangle
+                // (i, j, slot); the neighbour is removed iff some removed triangle holds j -> i
+                for (int i = lane; i < nclip; i += kWave) {
+                    const uint32_t tw = S.ctri[i];
+                    const uint32_t a = tw & 255u, b = (tw >> 8) & 255u, c = (tw >> 16) & 255u;
+                    bool f0 = false, f1 = false, f2 = false;
+                    for (int u = 0; u < nclip; ++u) {
+                        const uint32_t uw = S.ctri[u];
+                        const uint32_t u0 = uw & 255u, u1 = (uw >> 8) & 255u, u2 = (uw >> 16) & 255u;
+                        f0 |= (u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a);
+                        f1 |= (u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b);
+                        f2 |= (u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c);
+                    }
+                    const uint32_t sl = (uint32_t)slot << 16;
+                    if (!f0) {
+                        const int pos = atomicAdd(&S.n_new, 1);
+                        if (pos < MAXT) S.newtri[pos] = a | b << 8 | sl;
+                    }
+                    if (!f1) {
+                        const int pos = atomicAdd(&S.n_new, 1);
+                        if (pos < MAXT) S.newtri[pos] = b | c << 8 | sl;
+                    }
+                    if (!f2) {
+                        const int pos = atomicAdd(&S.n_new, 1);
+                        if (pos < MAXT) S.newtri[pos] = c | a << 8 | sl;
+                    }
+                }
+                __syncthreads();
+                const int n_new = S.n_new;
+                const int kept = nt - nclip;
+                if (kept + n_new > MAXT) {
+                    fail = true;
+                    break;
+                }
+                // 4. compaction, one chunk of 64 at a time (writes of a chunk land below it)
+                int base2 = 0;
+                for (int c0 = 0; c0 < nt; c0 += kWave) {
+                    const int t = c0 + lane;
+                    const bool keep = t < nt && !S.clip[t];
+                    uint32_t tw = 0;
+                    double4 V = make_double4(0.0, 0.0, 0.0, 0.0);
+                    if (keep) {
+                        tw = S.tri[t];
+                        V = S.vt[t];
+                    }
+                    const uint64_t km = __ballot(keep);
+                    __syncthreads();
+                    if (keep) {
+                        const int o = base2 + lanes_below(km, lane);
+                        S.tri[o] = tw;
+                        S.vt[o] = V;
+                    }
+                    base2 += __popcll(km);
+                }
+                // 5. the new triangles after the survivors
+                for (int t = lane; t < n_new; t += kWave) {
+                    const uint32_t tw = S.newtri[t];
+                    double na[3], nb[3], nc[3], ra, rb, rc;
+                    plane_of(S.pl, tw & 255u, px, py, pz, na, ra);
+                    plane_of(S.pl, (tw >> 8) & 255u, px, py, pz, nb, rb);
+                    plane_of(S.pl, (tw >> 16) & 255u, px, py, pz, nc, rc);
+                    double x, y, z;
+                    float er;
+                    vertex_of(na, ra, nb, rb, nc, rc, x, y, z, er);
+                    S.tri[kept + t] = tw;
+                    S.vt[kept + t] = make_double4(x, y, z, er);
+                }
+                nt = kept + n_new;
+                __syncthreads();
+                // 6. planes still in use, and the reach of the cell
+                uint32_t uu[UW];
+#pragma unroll
+                for (int w = 0; w < UW; ++w) uu[w] = 0u;
+                double r2 = 0.0;
+#pragma unroll
+                for (int j = 0; j < TPL; ++j) {
+                    const int t = lane + kWave * j;
+                    if (t < nt) {
+                        const uint32_t tw = S.tri[t];
+                        const uint32_t sa = tw & 255u, sb = (tw >> 8) & 255u, sc = (tw >> 16) & 255u;
+#pragma unroll
+                        for (int w = 0; w < UW; ++w) {
+                            uu[w] |= (sa >> 5) == (uint32_t)w ? 1u << (sa & 31) : 0u;
+                            uu[w] |= (sb >> 5) == (uint32_t)w ? 1u << (sb & 31) : 0u;
+                            uu[w] |= (sc >> 5) == (uint32_t)w ? 1u << (sc & 31) : 0u;
+                        }
+                        const double4 V = S.vt[t];
+                        const double rr = V.x * V.x + V.y * V.y + V.z * V.z;
+                        const double er = V.w;
+                        r2 = fmax(r2, er < kLost ? rr * (1.0 + 16.0 * er) : 4.0 * rr);
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < UW; ++w) used[w] = wave_or(uu[w]);
+                maxr2 = wave_max(r2);
+            }
+            __syncthreads();
+            if (fail) break;
+        }
+    };
+
+    // a point outside the grid's range (clamped into a boundary cell) starts from the set's
+    // extreme points, which bound its cell far sooner than the shells around its clamped cell
+    {
+        bool outside = false;
+        for (int c = 0; c < 3; ++c) outside |= pc[c] < gpr.lo[c] || pc[c] > gpr.lo[c] + gg * gpr.h[c];
+        if (outside) process_batch(A.ext, lane < kExt ? lane : 0, lane < kExt ? lane + 1 : 0);
+    }
+
+    // fine shells of grid cells around p's cell, then shells of 4x4x4-cell blocks
+    constexpr int kFineRings = 3;
+    bool done = false;
+    for (int r = 0; r <= kFineRings && !dead && !fail; ++r) {
         if (r > 0) {
             // lower bound on |q - p| for every point in a cell at Chebyshev distance >= r
             double L = DBL_MAX;
             for (int c = 0; c < 3; ++c) {
-                if (cc[c] + r <= g.g - 1) L = fmin(L, g.lo[c] + (cc[c] + r) * g.h[c] - pc[c]);
-                if (cc[c] - r >= 0) L = fmin(L, pc[c] - (g.lo[c] + (cc[c] - r + 1) * g.h[c]));
+                if (cc[c] + r <= gg - 1) L = fmin(L, gpr.lo[c] + (cc[c] + r) * gpr.h[c] - pc[c]);
+                if (cc[c] - r >= 0) L = fmin(L, pc[c] - (gpr.lo[c] + (cc[c] - r + 1) * gpr.h[c]));
             }
-            if (L == DBL_MAX) break;   // every cell visited
+            if (L == DBL_MAX) {   // every cell visited
+                done = true;
+                break;
+            }
             L *= 1.0 - 1e-12;
-            if (L > 0 && L * L > 4.0 * maxr2 * kReach) break;
+            if (L > 0 && L * L > 4.0 * maxr2 * kReach) {
+                done = true;
+                break;
+            }
         }
         const int count = r == 0 ? 1 : 24 * r * r + 2;
+        bool ring_reached = false;
         for (int base = 0; base < count && !dead && !fail; base += kWave) {
             int s = 0, e = 0;
             const int idx = base + lane;
             if (idx < count) {
                 int dx = 0, dy = 0, dz = 0;
                 if (r > 0) shell_cell(r, idx, dx, dy, dz);
-                const int x = cx + dx, y = cy + dy, z = cz + dz;
-                if (x >= 0 && y >= 0 && z >= 0 && x < g.g && y < g.g && z < g.g) {
-                    const int2 rg = cell_rng[((int64_t)z * g.g + y) * g.g + x];
-                    s = rg.x;
-                    e = rg.y;
+                const int ci[3] = {cx + dx, cy + dy, cz + dz};
+                if (ci[0] >= 0 && ci[1] >= 0 && ci[2] >= 0 && ci[0] < gg && ci[1] < gg && ci[2] < gg) {
+                    double bl[3], bh[3];
+                    cell_box(ci, ci, bl, bh);
+                    const bool reach = box_reach(bl, bh);
+                    if (reach) {
+                        const int2 rg = A.cell_rng[((int64_t)ci[2] * gg + ci[1]) * gg + ci[0]];
+                        s = rg.x;
+                        e = rg.y;
+                    }
+                    ring_reached |= reach;
                 }
             }
-            for (int m = 0;; ++m) {
-                const bool have = s + m < e;
-                if (!__ballot(have)) break;
-                int q = -1;
-                float qx = 0.f, qy = 0.f, qz = 0.f;
-                double d2 = 0.0;
-                bool valid = false, dup = false;
-                if (have) {
-                    q = cell_pts[s + m];
-                    qx = pts[3 * (int64_t)q];
-                    qy = pts[3 * (int64_t)q + 1];
-                    qz = pts[3 * (int64_t)q + 2];
-                    const double dx = (double)qx - px, dy = (double)qy - py, dz = (double)qz - pz;
-                    d2 = dx * dx + dy * dy + dz * dz;
-                    if (q != p) {
-                        if (d2 == 0.0) {
-                            dup = q < p;   // a lower-index copy of p owns this position
-                        } else if (d2 < 4.0 * maxr2 * kReach) {
-                            // q can cut the cell only if some vertex is closer to q than to p
-                            valid = false;
-                            for (int t = 0; t < nt && !valid; ++t) {
-                                const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
-                                const double sdist = vx * dx + vy * dy + vz * dz - 0.5 * d2;
-                                const double mag = sqrt((vx * vx + vy * vy + vz * vz) * d2) + d2;
-                                if (sdist > -kQuickMargin * mag) valid = true;
-                                else if ((S.tri[t] >> 24) & 1u)
-                                    valid = d2 < 16.0 * (vx * vx + vy * vy + vz * vz) * kReach;
-                            }
-                        }
-                    }
+            process_batch(A.cell_pts, s, e);
+        }
+        // every ball touches p, so a ball reaching ring r + 1 crosses ring r: a ring no ball
+        // reaches ends the search
+        if (r > 0 && !__ballot(ring_reached)) {
+            done = true;
+            break;
+        }
+    }
+    if (!done && !dead && !fail) {
+        const int gb = (gg + 3) >> 2;
+        const int bc[3] = {cx >> 2, cy >> 2, cz >> 2};
+        for (int rb = 0; !dead && !fail; ++rb) {
+            if (rb > 0) {
+                double L = DBL_MAX;
+                for (int c = 0; c < 3; ++c) {
+                    const int hiCell = (bc[c] + rb) * 4, loCell = (bc[c] - rb) * 4 + 3;
+                    if (bc[c] + rb <= gb - 1) L = fmin(L, gpr.lo[c] + hiCell * gpr.h[c] - pc[c]);
+                    if (bc[c] - rb >= 0) L = fmin(L, pc[c] - (gpr.lo[c] + (loCell + 1) * gpr.h[c]));
                 }
-                if (__ballot(dup)) {
-                    dead = true;
-                    break;
-                }
-                const uint64_t vm = __ballot(valid);
-                const int nv = __popcll(vm);
-                if (nv == 0) continue;
-                S.cd2[lane] = valid ? d2 : DBL_MAX;
-                S.cid[lane] = valid ? q : 0x7fffffff;
-                __syncthreads();
-                if (valid) {
-                    int rank = 0;
-                    for (int j = 0; j < kWave; ++j) {
-                        const double dj = S.cd2[j];
-                        const int ij = S.cid[j];
-                        rank += (dj < d2) || (dj == d2 && ij < q);
-                    }
-                    S.sd2[rank] = d2;
-                    S.sid[rank] = q;
-                    S.spt[rank] = make_float4(qx, qy, qz, 0.f);
-                }
-                __syncthreads();
-                for (int k = 0; k < nv && !fail; ++k) {
-                    // ---- clip the cell with the bisector of p and candidate k
-                    const int cq = S.sid[k];
-                    const float4 Q = S.spt[k];
-                    const double ddx = (double)Q.x - px, ddy = (double)Q.y - py, ddz = (double)Q.z - pz;
-                    const double cd2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (!(cd2 < 4.0 * maxr2 * kReach)) continue;
-                    bool mc[kTPerLane];
-                    uint32_t mtw[kTPerLane];
-                    double mx[kTPerLane], my[kTPerLane], mz[kTPerLane];
-                    bool any = false;
-#pragma unroll
-                    for (int j = 0; j < kTPerLane; ++j) {
-                        const int t = lane + kWave * j;
-                        bool c = false;
-                        mtw[j] = 0;
-                        mx[j] = my[j] = mz[j] = 0.0;
-                        if (t < nt) {
-                            const uint32_t tw = S.tri[t];
-                            mtw[j] = tw;
-                            const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
-                            mx[j] = vx;
-                            my[j] = vy;
-                            mz[j] = vz;
-                            const double sdist = vx * ddx + vy * ddy + vz * ddz - 0.5 * cd2;
-                            const double mag = sqrt((vx * vx + vy * vy + vz * vz) * cd2) + cd2;
-                            if (!((tw >> 24) & 1u) && fabs(sdist) > kQuickMargin * mag) {
-                                c = sdist > 0.0;
-                            } else {
-                                const float4 A = S.pl[tw & 255u], B = S.pl[(tw >> 8) & 255u], C = S.pl[(tw >> 16) & 255u];
-                                const int ids[5] = {p, __float_as_int(A.w), __float_as_int(B.w), __float_as_int(C.w), cq};
-                                const float xs[5][3] = {{pfx, pfy, pfz}, {A.x, A.y, A.z}, {B.x, B.y, B.z},
-                                                        {C.x, C.y, C.z}, {Q.x, Q.y, Q.z}};
-                                c = insphere_inside(ids, xs, &exact, &range);
-                            }
-                            S.clip[t] = c ? 1 : 0;
-                        }
-                        mc[j] = c;
-                        any |= c;
-                    }
-                    if (!__ballot(any)) continue;
-                    // a free plane slot for q
-                    int slot = -1;
-#pragma unroll
-                    for (int w = 3; w >= 0; --w)
-                        if (used[w] != 0xffffffffu) slot = 32 * w + __builtin_ctz(~used[w]);
-                    if (slot < 0) {
-                        fail = true;
-                        break;
-                    }
-                    if (lane == 0) {
-                        S.pl[slot] = make_float4(Q.x, Q.y, Q.z, __int_as_float(cq));
-                        S.n_new = 0;
-                    }
-                    __syncthreads();
-                    // boundary edges of the clipped region: edge (i -> j) of a clipped triangle whose
-                    // neighbour (the triangle holding j -> i) survives gives the new triangle (i, j, slot)
-#pragma unroll
-                    for (int j = 0; j < kTPerLane; ++j) {
-                        if (!__ballot(mc[j])) continue;
-                        const uint32_t a = mtw[j] & 255u, b = (mtw[j] >> 8) & 255u, c = (mtw[j] >> 16) & 255u;
-                        int nb0 = -1, nb1 = -1, nb2 = -1;   // clip state of the neighbour across ab, bc, ca
-                        for (int u = 0; u < nt; ++u) {
-                            const uint32_t uw = S.tri[u];
-                            const uint32_t u0 = uw & 255u, u1 = (uw >> 8) & 255u, u2 = (uw >> 16) & 255u;
-                            const int uc = S.clip[u];
-                            if ((u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a)) nb0 = uc;
-                            if ((u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b)) nb1 = uc;
-                            if ((u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c)) nb2 = uc;
-                        }
-                        if (mc[j]) {
-                            if (nb0 < 0 || nb1 < 0 || nb2 < 0) ++topo;
-                            const uint32_t sl = (uint32_t)slot << 16;
-                            if (nb0 == 0) {
-                                const int pos = atomicAdd(&S.n_new, 1);
-                                if (pos < kMaxT) S.newtri[pos] = a | b << 8 | sl;
-                            }
-                            if (nb1 == 0) {
-                                const int pos = atomicAdd(&S.n_new, 1);
-                                if (pos < kMaxT) S.newtri[pos] = b | c << 8 | sl;
-                            }
-                            if (nb2 == 0) {
-                                const int pos = atomicAdd(&S.n_new, 1);
-                                if (pos < kMaxT) S.newtri[pos] = c | a << 8 | sl;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                    const int n_new = S.n_new;
-                    // compaction: survivors first, in order, then the new triangles
-                    int kept = 0;
-                    int pos[kTPerLane];
-#pragma unroll
-                    for (int j = 0; j < kTPerLane; ++j) {
-                        const bool keep = lane + kWave * j < nt && !mc[j];
-                        const uint64_t km = __ballot(keep);
-                        pos[j] = keep ? kept + lanes_below(km, lane) : -1;
-                        kept += __popcll(km);
-                    }
-                    if (kept + n_new > kMaxT) {
-                        fail = true;
-                        break;
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int j = 0; j < kTPerLane; ++j)
-                        if (pos[j] >= 0) {
-                            S.tri[pos[j]] = mtw[j];
-                            S.vx[pos[j]] = mx[j];
-                            S.vy[pos[j]] = my[j];
-                            S.vz[pos[j]] = mz[j];
-                        }
-                    for (int t = lane; t < n_new; t += kWave) {
-                        const uint32_t tw = S.newtri[t];
-                        double na[3], nb[3], nc[3], ra, rb, rc;
-                        plane_of(S, tw & 255u, px, py, pz, na, ra);
-                        plane_of(S, (tw >> 8) & 255u, px, py, pz, nb, rb);
-                        plane_of(S, (tw >> 16) & 255u, px, py, pz, nc, rc);
-                        double x, y, z;
-                        bool fr;
-                        vertex_of(na, ra, nb, rb, nc, rc, x, y, z, fr);
-                        S.tri[kept + t] = tw | (fr ? 1u << 24 : 0u);
-                        S.vx[kept + t] = x;
-                        S.vy[kept + t] = y;
-                        S.vz[kept + t] = z;
-                    }
-                    nt = kept + n_new;
-                    __syncthreads();
-                    // planes still in use, and the reach of the cell
-                    uint32_t u4[4] = {0u, 0u, 0u, 0u};
-                    double r2 = 0.0;
-#pragma unroll
-                    for (int j = 0; j < kTPerLane; ++j) {
-                        const int t = lane + kWave * j;
-                        if (t < nt) {
-                            const uint32_t tw = S.tri[t];
-                            const uint32_t sa = tw & 255u, sb = (tw >> 8) & 255u, sc = (tw >> 16) & 255u;
-                            u4[sa >> 5] |= 1u << (sa & 31);
-                            u4[sb >> 5] |= 1u << (sb & 31);
-                            u4[sc >> 5] |= 1u << (sc & 31);
-                            const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
-                            const double rr = vx * vx + vy * vy + vz * vz;
-                            r2 = fmax(r2, ((tw >> 24) & 1u) ? 4.0 * rr : rr);
-                        }
-                    }
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) used[w] = wave_or(u4[w]);
-                    maxr2 = wave_max(r2);
-                }
-                __syncthreads();
-                if (fail) break;
+                if (L == DBL_MAX) break;   // every block visited
+                L *= 1.0 - 1e-12;
+                if (L > 0 && L * L > 4.0 * maxr2 * kReach) break;
             }
+            const int count = rb == 0 ? 1 : 24 * rb * rb + 2;
+            bool ring_reached = false;
+            for (int base = 0; base < count && !dead && !fail; base += kWave) {
+                // one block per lane: does it meet a ball?
+                const int idx = base + lane;
+                int b3[3] = {0, 0, 0};
+                bool reach = false;
+                if (idx < count) {
+                    int dx = 0, dy = 0, dz = 0;
+                    if (rb > 0) shell_cell(rb, idx, dx, dy, dz);
+                    b3[0] = bc[0] + dx;
+                    b3[1] = bc[1] + dy;
+                    b3[2] = bc[2] + dz;
+                    if (b3[0] >= 0 && b3[1] >= 0 && b3[2] >= 0 && b3[0] < gb && b3[1] < gb && b3[2] < gb) {
+                        int i0[3], i1[3];
+                        for (int c = 0; c < 3; ++c) {
+                            i0[c] = b3[c] * 4;
+                            i1[c] = min(b3[c] * 4 + 3, gg - 1);
+                        }
+                        double bl[3], bh[3];
+                        cell_box(i0, i1, bl, bh);
+                        reach = box_reach(bl, bh);
+                    }
+                }
+                ring_reached |= reach;
+                uint64_t bm = __ballot(reach);
+                while (bm && !dead && !fail) {
+                    const int j = __builtin_ctzll(bm);
+                    bm &= bm - 1;
+                    const int bx = __shfl(b3[0], j, 64), by = __shfl(b3[1], j, 64), bz = __shfl(b3[2], j, 64);
+                    // the block's 64 cells, one per lane; cells of the fine shells are done
+                    const int ci[3] = {bx * 4 + (lane & 3), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane >> 4)};
+                    int s = 0, e = 0;
+                    if (ci[0] < gg && ci[1] < gg && ci[2] < gg &&
+                        max(max(abs(ci[0] - cx), abs(ci[1] - cy)), abs(ci[2] - cz)) > kFineRings) {
+                        double bl[3], bh[3];
+                        cell_box(ci, ci, bl, bh);
+                        if (box_reach(bl, bh)) {
+                            const int2 rg = A.cell_rng[((int64_t)ci[2] * gg + ci[1]) * gg + ci[0]];
+                            s = rg.x;
+                            e = rg.y;
+                        }
+                    }
+                    process_batch(A.cell_pts, s, e);
+                }
+            }
+            if (rb > 0 && !__ballot(ring_reached)) break;
         }
     }
 
-    // write the tets this point owns
+    // the tets this point owns: vertices of three real planes with indices all above p
     int nreal = 0, nemit = 0;
-    bool em[kTPerLane];
-    int4 tet[kTPerLane];
     if (!dead && !fail) {
 #pragma unroll
-        for (int j = 0; j < kTPerLane; ++j) {
+        for (int j = 0; j < TPL; ++j) {
             const int t = lane + kWave * j;
-            em[j] = false;
-            tet[j] = make_int4(0, 0, 0, 0);
-            bool real = false;
+            bool real = false, em = false;
             if (t < nt) {
                 const uint32_t tw = S.tri[t];
                 const int a = __float_as_int(S.pl[tw & 255u].w), b = __float_as_int(S.pl[(tw >> 8) & 255u].w),
                           c = __float_as_int(S.pl[(tw >> 16) & 255u].w);
                 real = a < n && b < n && c < n;   // no far point
-                if (real && p < a && p < b && p < c) {
-                    em[j] = true;
-                    // rotate (a, b, c) cyclically to put the smallest first: orientation is kept
-                    if (a < b && a < c) tet[j] = make_int4(p, a, b, c);
-                    else if (b < c) tet[j] = make_int4(p, b, c, a);
-                    else tet[j] = make_int4(p, c, a, b);
-                }
+                em = real && p < a && p < b && p < c;
             }
             nreal += __popcll(__ballot(real));
-            nemit += __popcll(__ballot(em[j]));
+            nemit += __popcll(__ballot(em));
         }
     }
+    const bool requeue = fail && A.retry != nullptr;
     unsigned long long off = 0;
     if (lane == 0) {
-        if (nemit > 0) off = atomicAdd(app_count, (unsigned long long)nemit);
-        pt_count[p] = nemit;
-        pt_off[p] = (int64_t)off;
-        if (nreal) atomicAdd((unsigned long long*)&info[kInfoRealVerts], (unsigned long long)nreal);
-        if (dead) atomicAdd((unsigned long long*)&info[kInfoDup], 1ull);
-        if (fail) atomicAdd((unsigned long long*)&info[kInfoOverflow], 1ull);
-        atomicMax((unsigned long long*)&info[kInfoMaxT], (unsigned long long)nt);
+        if (nemit > 0) off = atomicAdd(A.app_count, (unsigned long long)nemit);
+        A.pt_count[p] = nemit;
+        A.pt_off[p] = (int64_t)off;
+        if (nreal) atomicAdd((unsigned long long*)&A.info[kInfoRealVerts], (unsigned long long)nreal);
+        if (dead) atomicAdd((unsigned long long*)&A.info[kInfoDup], 1ull);
+        if (requeue) {
+            const unsigned long long slot = atomicAdd(A.retry_count, 1ull);
+            A.retry[slot] = p;   // at most one entry per point: the list holds n
+        } else if (fail) {
+            atomicAdd((unsigned long long*)&A.info[kInfoOverflow], 1ull);
+        }
+        atomicMax((unsigned long long*)&A.info[kInfoMaxT], (unsigned long long)nt);
     }
     off = __shfl(off, 0, 64);
     if (!dead && !fail && nemit > 0) {
         int before = 0;
 #pragma unroll
-        for (int j = 0; j < kTPerLane; ++j) {
-            const uint64_t m = __ballot(em[j]);
-            if (em[j]) {
+        for (int j = 0; j < TPL; ++j) {
+            const int t = lane + kWave * j;
+            bool em = false;
+            int4 tet = make_int4(0, 0, 0, 0);
+            if (t < nt) {
+                const uint32_t tw = S.tri[t];
+                const int a = __float_as_int(S.pl[tw & 255u].w), b = __float_as_int(S.pl[(tw >> 8) & 255u].w),
+                          c = __float_as_int(S.pl[(tw >> 16) & 255u].w);
+                em = a < n && b < n && c < n && p < a && p < b && p < c;
+                // rotate (a, b, c) cyclically to put the smallest first: orientation is kept
+                if (a < b && a < c) tet = make_int4(p, a, b, c);
+                else if (b < c) tet = make_int4(p, b, c, a);
+                else tet = make_int4(p, c, a, b);
+            }
+            const uint64_t m = __ballot(em);
+            if (em) {
                 const unsigned long long o = off + before + lanes_below(m, lane);
-                if (o < (unsigned long long)app_cap) app[o] = tet[j];
+                if (o < (unsigned long long)A.app_cap) A.app[o] = tet;
             }
             before += __popcll(m);
         }
     }
-    // per-lane counters
-    if (exact) atomicAdd((unsigned long long*)&info[kInfoExact], (unsigned long long)exact);
-    if (range) atomicAdd((unsigned long long*)&info[kInfoRange], (unsigned long long)range);
-    if (topo) atomicAdd((unsigned long long*)&info[kInfoTopo], (unsigned long long)topo);
+    if (exact) atomicAdd((unsigned long long*)&A.info[kInfoExact], (unsigned long long)exact);
+    if (range) atomicAdd((unsigned long long*)&A.info[kInfoRange], (unsigned long long)range);
+    __syncthreads();   // the LDS cell may be reused by the next point of this workgroup
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dt_cell_kernel(CellArgs A) {
+    __shared__ CellSmem<256, 128> S;
+    cell_run(S, (int)blockIdx.x, A);
+}
+
+// Cells that outgrew the main pass, with four times the room; a fixed grid walks the queue.
+__global__ __launch_bounds__(64) void dt_cell_retry_kernel(CellArgs A, const int32_t* __restrict__ queue,
+                                                           const unsigned long long* __restrict__ queue_len) {
+    __shared__ CellSmem<1024, 256> S;
+    const unsigned long long len = *queue_len;
+    for (unsigned long long i = blockIdx.x; i < len; i += gridDim.x) cell_run(S, queue[i], A);
 }
 
 __global__ __launch_bounds__(256) void dt_gather_kernel(const int4* __restrict__ app, int64_t app_cap,
@@ -842,7 +1097,7 @@ inline int grid_dim(int64_t n) {
 }
 
 struct DtLayout {
-    size_t keys, keys2, vals, vals2, sortws, ranges, partial, gp, app, cnt, incl, off, scanws, appcount, total;
+    size_t keys, keys2, vals, vals2, sortws, ranges, partial, gp, app, cnt, incl, off, scanws, appcount, retry, ext, total;
     int g, nb;
 };
 
@@ -865,12 +1120,18 @@ inline DtLayout dt_layout(int64_t n, int64_t cap) {
     L.incl = o; o += al(4 * (size_t)n);
     L.off = o; o += al(8 * (size_t)n);
     L.scanws = o; o += al(rm_scan_workspace_size(n));
-    L.appcount = o; o += al(8);
+    L.appcount = o; o += al(16);   // append counter, retry-queue length
+    L.retry = o; o += al(4 * (size_t)n);
+    L.ext = o; o += al(8 * kExt + 4 * kExt);   // keys, then ids
     L.total = o;
     return L;
 }
 
-__global__ void dt_zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+__global__ void dt_zero_u64_kernel(unsigned long long* p, unsigned long long* ext_keys) {
+    p[0] = 0ull;
+    p[1] = 0ull;
+    for (int k = 0; k < kExt; ++k) ext_keys[k] = 0ull;
+}
 
 __global__ void dt_zero_info_kernel(int64_t* info) {
     if (threadIdx.x < kInfoCount) info[threadIdx.x] = 0;
@@ -919,6 +1180,10 @@ extern "C" int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* t
     int32_t* incl = (int32_t*)(ws + L.incl);
     int64_t* off = (int64_t*)(ws + L.off);
     unsigned long long* appcount = (unsigned long long*)(ws + L.appcount);
+    unsigned long long* retrycount = appcount + 1;
+    int32_t* retry = (int32_t*)(ws + L.retry);
+    unsigned long long* ext_keys = (unsigned long long*)(ws + L.ext);
+    int32_t* ext_ids = (int32_t*)(ws + L.ext + 8 * kExt);
     const int64_t cells = (int64_t)L.g * L.g * L.g;
     const int64_t app_cap = tet_capacity > 0 ? tet_capacity : 1;
 
@@ -935,10 +1200,31 @@ extern "C" int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* t
     hipLaunchKernelGGL(dt_zero_ranges_kernel, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, st, ranges, cells);
     hipLaunchKernelGGL(dt_ranges_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, (const int64_t*)keys2,
                        n, ranges);
-    hipLaunchKernelGGL(dt_zero_u64_kernel, dim3(1), dim3(1), 0, st, appcount);
-    hipLaunchKernelGGL(dt_cell_kernel, dim3((unsigned)n), dim3(kWave), 0, st, points_dev, n,
-                       (const int32_t*)vals2, (const int2*)ranges, (const GridParams*)gp, app, app_cap, appcount,
-                       cnt, off, info_dev);
+    hipLaunchKernelGGL(dt_zero_u64_kernel, dim3(1), dim3(1), 0, st, appcount, ext_keys);
+    hipLaunchKernelGGL(dt_extreme_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, points_dev, n, ext_keys);
+    hipLaunchKernelGGL(dt_extreme_ids_kernel, dim3(1), dim3(64), 0, st, (const unsigned long long*)ext_keys, n,
+                       ext_ids);
+    CellArgs ca;
+    ca.pts = points_dev;
+    ca.n = n;
+    ca.cell_pts = vals2;
+    ca.cell_rng = ranges;
+    ca.gp = gp;
+    ca.app = app;
+    ca.app_cap = app_cap;
+    ca.app_count = appcount;
+    ca.pt_count = cnt;
+    ca.pt_off = off;
+    ca.info = info_dev;
+    ca.ext = ext_ids;
+    ca.retry = retry;
+    ca.retry_count = retrycount;
+    hipLaunchKernelGGL(dt_cell_kernel, dim3((unsigned)n), dim3(kWave), 0, st, ca);
+    CellArgs cr = ca;
+    cr.retry = nullptr;
+    cr.retry_count = nullptr;
+    hipLaunchKernelGGL(dt_cell_retry_kernel, dim3(1024), dim3(kWave), 0, st, cr, (const int32_t*)retry,
+                       (const unsigned long long*)retrycount);
     RM_CHECK_LAUNCH("rm_delaunay cells");
     rc = rm_inclusive_scan_i32(cnt, incl, n, ws + L.scanws, rm_scan_workspace_size(n), stream);
     if (rc != RM_OK) return rc;

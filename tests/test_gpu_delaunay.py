@@ -97,12 +97,15 @@ def _check_empty_spheres(pts, tets):
             assert not O.inside_sphere_exact(P, tuple(t), int(q)), (t, q)
 
 
-def test_cospherical_lattice_is_a_valid_delaunay_tetrahedralisation():
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4])
+def test_cospherical_lattice_is_a_valid_delaunay_tetrahedralisation(seed):
     """A 4x4x4 integer lattice: every cube's 8 corners are co-spherical, so Delaunay is not unique.
-    The result must be positively oriented, have empty circumspheres and tile the cube [0,3]^3."""
+    The result must be positively oriented, have empty circumspheres and tile the cube [0,3]^3,
+    whatever the order of the points (the tie-breaking follows the indices)."""
     g = np.array([(x, y, z) for x in range(4) for y in range(4) for z in range(4)], np.float32)
-    perm = np.random.default_rng(0).permutation(len(g))
+    perm = np.random.default_rng(seed).permutation(len(g))
     pts = g[perm]
+    print(f"lattice seed {seed}", flush=True)
     t, info = _run(pts)
     print(f"lattice: {len(t)} tets, info {info}", flush=True)
     vol = _volumes(pts, t)
