@@ -12,8 +12,9 @@ Prints one JSON line (rank 0): metric fwd+bwd Mrays/s (rays = pixels per step / 
 roofline of the dominant kernel (HIP events around its launches inside the timed region), and a
 CPU baseline: the reference's CPU method (per-ray midpoint quadrature testing every tet at every
 sample, jaxutil/tetra_quad.py) restated in C, timed on a bounded sample of rays of the same view.
-The line also carries the per-tet error-vote timing, the BF16-vs-FP64 field report and the
-serving leg: three device-resident meshes (250k, 1M, 3M points) served a Poisson trace of
+The line also carries the per-tet error-vote timing, the BF16-vs-FP64 field report, the GPU
+Delaunay tetrahedralisation of the benchmark's points (wall time per call, set difference with the
+benchmark mesh, scipy/Qhull on a 100k-point sample as the CPU reference) and the serving leg: three device-resident meshes (250k, 1M, 3M points) served a Poisson trace of
 mixed-resolution views (256x256 .. 1920x1080, 200/s for --serve-seconds, then a saturating
 1000/s trace) batched by resolution bucket and per frame, with p50/p99 latency, frames/s and the
 batched-vs-per-frame image difference.
@@ -57,6 +58,9 @@ def parse():
     p.add_argument("--no-precision-report", action="store_true",
                    help="skip the BF16-vs-FP64 field MLP report on the benchmark mesh")
     p.add_argument("--no-layout", action="store_true", help="keep the Delaunay order of vertices and tets")
+    p.add_argument("--no-delaunay", action="store_true", help="skip the GPU Delaunay leg")
+    p.add_argument("--delaunay-cpu-points", type=int, default=100_000,
+                   help="points of the scipy (Qhull) CPU reference sample in the Delaunay leg")
     p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
     p.add_argument("--profile-steps", type=int, default=0,
                    help="run only this many steps after warmup, without the CPU baseline (for rocprof)")
@@ -152,6 +156,51 @@ def time_tet_err(indices, vertices, tcam, rg, sidx, ranges, feats, reps=3):
     voted_tets = int((cnt[:, 0] > 0).sum().item())
     # algorithmic output: one 64-byte vote row and one 8-byte count row per voted tet
     return {"mean_ms": round(sum(ms) / reps, 4), "voted_tets": voted_tets, "votes": int(cnt[:, 0].sum().item())}
+
+
+def delaunay_leg(args, pts, tets, reps=3):
+    """The triangulation step of update_triangulation (models/ingp_color.py:523-544) on the
+    benchmark's points: rm_delaunay, wall time per call including the host's read of the tet count
+    (the reference's gdel3d/scipy calls also hand back host arrays); the result compared as a set
+    with the benchmark mesh (scipy/Qhull on the float64 points before their float32 rounding);
+    scipy timed on a sample of the same points as the CPU reference (the reference's
+    high_precision branch)."""
+    import torch
+    from radiance_meshes_amd.delaunay import delaunay_tets
+    P = torch.from_numpy(np.ascontiguousarray(pts, np.float32)).cuda()
+    t, info = delaunay_tets(P, return_info=True)   # warm-up: workspace, code objects
+    torch.cuda.synchronize()
+    wall = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t, info = delaunay_tets(P, return_info=True)
+        torch.cuda.synchronize()
+        wall.append(time.perf_counter() - t0)
+    ms = 1e3 * sum(wall) / reps
+
+    def canon(a):
+        a = np.sort(np.asarray(a, np.int64), axis=1)
+        return np.ascontiguousarray(a).view(np.dtype((np.void, 32))).ravel()
+    g, r = canon(t.cpu().numpy()), canon(tets)
+    gpu_only = int(np.setdiff1d(g, r, assume_unique=True).size)
+    ref_only = int(np.setdiff1d(r, g, assume_unique=True).size)
+    del P, t
+    torch.cuda.empty_cache()
+    from scipy.spatial import Delaunay
+    n_cpu = min(args.delaunay_cpu_points, len(pts))
+    sample = np.asarray(pts[np.random.default_rng(0).choice(len(pts), n_cpu, replace=False)], np.float64)
+    t0 = time.perf_counter()
+    Delaunay(sample)
+    cpu_s = time.perf_counter() - t0
+    log(f"[bench] delaunay: {ms:.1f} ms per call on {len(pts)} points; scipy {cpu_s:.2f} s on {n_cpu}")
+    return {"points": int(len(pts)), "tets": int(info["n_tets"]), "ms_per_call": round(ms, 2),
+            "points_per_s": round(len(pts) / (ms * 1e-3)), "info": info,
+            "vs_benchmark_mesh": {"gpu_only_tets": gpu_only, "mesh_only_tets": ref_only,
+                                  "note": "mesh: scipy/Qhull on the float64 points; GPU: their float32 values"},
+            "cpu_baseline": {"kind": "reference", "what": "scipy.spatial.Delaunay (Qhull), one thread",
+                             "points": n_cpu, "seconds": round(cpu_s, 3),
+                             "points_per_s": round(n_cpu / cpu_s)}}
 
 
 def measured_traffic(workload, tile, precision, span):
@@ -453,6 +502,13 @@ def main():
     precision = None
     if not args.serve_only and not args.no_precision_report and not args.profile_steps:
         precision = precision_report(pts, tets, cam, args.tile)
+    tri = None
+    if not args.serve_only and not args.no_delaunay and not args.profile_steps:
+        try:
+            tri = delaunay_leg(args, pts, tets)
+        except Exception as e:   # the training-step line is still printed
+            log(f"[bench] delaunay leg failed: {type(e).__name__}: {e}")
+            tri = {"error": f"{type(e).__name__}: {e}"}
     serving = None
     if serve_points and not args.profile_steps:
         import torch
@@ -495,6 +551,7 @@ def main():
         "breakdown_ms": res["breakdown_ms"],
         "tet_err": res.get("tet_err"),
         "precision": precision,
+        "delaunay": tri,
         "cpu_baseline": cpu,
         "serving": serving,
     }
