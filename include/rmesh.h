@@ -301,7 +301,7 @@ int rm_field_scatter_bwd(const float* raw_dev, const int32_t* tet_ids_dev, int64
  * vertex index in increasing order, at most tet_capacity of them. info_dev [8] i64 receives:
  *   0: n (the full count, which may exceed tet_capacity: run again with a larger capacity),
  *   1: Voronoi vertices found over all cells (4 n when the cells agree),
- *   2: cells over the per-cell capacity (256 vertices, 128 planes; non-zero = failed),
+ *   2: cells over the per-cell capacity (960 vertices, 1024 planes; non-zero = failed),
  *   3: duplicate points dropped (the copy with the lowest index keeps the cell),
  *   4: predicate evaluations that needed exact arithmetic,
  *   5: of those, inputs outside its range (coordinate exponents spanning more than 99 bits),

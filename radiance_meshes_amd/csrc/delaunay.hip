@@ -432,12 +432,13 @@ __global__ __launch_bounds__(256) void dt_ranges_kernel(const int64_t* __restric
 // The cell lives in LDS: plane table pl (point position and index), triangles tri (three 8-bit
 // plane slots and a flag for ill-conditioned vertices) with their vertex positions relative to p.
 // The main pass holds up to 256 vertices and 128 planes; cells that outgrow it are queued and
-// recomputed by a retry pass holding up to 1024 vertices and 256 planes.
+// recomputed by a retry pass holding up to 960 vertices and 1024 planes (a workgroup's LDS stays
+// within 64 KB).
 template <int MAXT, int MAXP>
 struct __align__(16) CellSmem {
-    static_assert(MAXP <= 256, "plane slots are 8-bit");
+    static_assert(MAXP <= 1024, "plane slots are at most 10-bit");
     float4 pl[MAXP];           // xyz: neighbour position; w: its index (int bits; far points n..n+3)
-    uint32_t tri[MAXT];        // plane slots a | b << 8 | c << 16
+    uint32_t tri[MAXT];        // plane slots a | b << SB | c << 2 SB (SB = 8, or 10 beyond 256 planes)
     double4 vt[MAXT];          // vertex position relative to p (xyz), relative error bound of it (w)
     uint32_t newtri[MAXT];     // triangles created by one clip
     uint32_t ctri[MAXT];       // triangles removed by one clip
@@ -552,6 +553,8 @@ template <int MAXT, int MAXP>
 __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A) {
     constexpr int TPL = MAXT / kWave;
     constexpr int UW = MAXP / 32;
+    constexpr int SB = MAXP > 256 ? 10 : 8;   // bits per plane slot in a triangle word
+    constexpr uint32_t SM = (1u << SB) - 1u;
     const int lane = threadIdx.x;
     const GridParams& gpr = *A.gp;
     const int gg = gpr.g;
@@ -586,7 +589,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
         double x, y, z;
         float er;
         vertex_of(na, ra, nb, rb, nc, rc, x, y, z, er);
-        S.tri[lane] = s3[0] | s3[1] << 8 | s3[2] << 16;
+        S.tri[lane] = s3[0] | s3[1] << SB | s3[2] << (2 * SB);
         S.vt[lane] = make_double4(x, y, z, er);
     }
     int nt = 4;
@@ -736,7 +739,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                         if (er < kLost && fabs(sdist) > (kQuickMargin + 8.0 * er) * mag) {
                             c = sdist > 0.0;
                         } else {
-                            const float4 Pa = S.pl[tw & 255u], Pb = S.pl[(tw >> 8) & 255u], Pc = S.pl[(tw >> 16) & 255u];
+                            const float4 Pa = S.pl[tw & SM], Pb = S.pl[(tw >> SB) & SM], Pc = S.pl[(tw >> (2 * SB)) & SM];
                             const int ids[5] = {p, __float_as_int(Pa.w), __float_as_int(Pb.w), __float_as_int(Pc.w), cq};
                             const float xs[5][3] = {{pfx, pfy, pfz}, {Pa.x, Pa.y, Pa.z}, {Pb.x, Pb.y, Pb.z},
                                                     {Pc.x, Pc.y, Pc.z}, {Q.x, Q.y, Q.z}};
@@ -776,27 +779,27 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                 // (i, j, slot); the neighbour is removed iff some removed triangle holds j -> i
                 for (int i = lane; i < nclip; i += kWave) {
                     const uint32_t tw = S.ctri[i];
-                    const uint32_t a = tw & 255u, b = (tw >> 8) & 255u, c = (tw >> 16) & 255u;
+                    const uint32_t a = tw & SM, b = (tw >> SB) & SM, c = (tw >> (2 * SB)) & SM;
                     bool f0 = false, f1 = false, f2 = false;
                     for (int u = 0; u < nclip; ++u) {
                         const uint32_t uw = S.ctri[u];
-                        const uint32_t u0 = uw & 255u, u1 = (uw >> 8) & 255u, u2 = (uw >> 16) & 255u;
+                        const uint32_t u0 = uw & SM, u1 = (uw >> SB) & SM, u2 = (uw >> (2 * SB)) & SM;
                         f0 |= (u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a);
                         f1 |= (u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b);
                         f2 |= (u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c);
                     }
-                    const uint32_t sl = (uint32_t)slot << 16;
+                    const uint32_t sl = (uint32_t)slot << (2 * SB);
                     if (!f0) {
                         const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = a | b << 8 | sl;
+                        if (pos < MAXT) S.newtri[pos] = a | b << SB | sl;
                     }
                     if (!f1) {
                         const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = b | c << 8 | sl;
+                        if (pos < MAXT) S.newtri[pos] = b | c << SB | sl;
                     }
                     if (!f2) {
                         const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = c | a << 8 | sl;
+                        if (pos < MAXT) S.newtri[pos] = c | a << SB | sl;
                     }
                 }
                 __syncthreads();
@@ -830,9 +833,9 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                 for (int t = lane; t < n_new; t += kWave) {
                     const uint32_t tw = S.newtri[t];
                     double na[3], nb[3], nc[3], ra, rb, rc;
-                    plane_of(S.pl, tw & 255u, px, py, pz, na, ra);
-                    plane_of(S.pl, (tw >> 8) & 255u, px, py, pz, nb, rb);
-                    plane_of(S.pl, (tw >> 16) & 255u, px, py, pz, nc, rc);
+                    plane_of(S.pl, tw & SM, px, py, pz, na, ra);
+                    plane_of(S.pl, (tw >> SB) & SM, px, py, pz, nb, rb);
+                    plane_of(S.pl, (tw >> (2 * SB)) & SM, px, py, pz, nc, rc);
                     double x, y, z;
                     float er;
                     vertex_of(na, ra, nb, rb, nc, rc, x, y, z, er);
@@ -851,7 +854,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                     const int t = lane + kWave * j;
                     if (t < nt) {
                         const uint32_t tw = S.tri[t];
-                        const uint32_t sa = tw & 255u, sb = (tw >> 8) & 255u, sc = (tw >> 16) & 255u;
+                        const uint32_t sa = tw & SM, sb = (tw >> SB) & SM, sc = (tw >> (2 * SB)) & SM;
 #pragma unroll
                         for (int w = 0; w < UW; ++w) {
                             uu[w] |= (sa >> 5) == (uint32_t)w ? 1u << (sa & 31) : 0u;
@@ -1006,8 +1009,8 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
             bool real = false, em = false;
             if (t < nt) {
                 const uint32_t tw = S.tri[t];
-                const int a = __float_as_int(S.pl[tw & 255u].w), b = __float_as_int(S.pl[(tw >> 8) & 255u].w),
-                          c = __float_as_int(S.pl[(tw >> 16) & 255u].w);
+                const int a = __float_as_int(S.pl[tw & SM].w), b = __float_as_int(S.pl[(tw >> SB) & SM].w),
+                          c = __float_as_int(S.pl[(tw >> (2 * SB)) & SM].w);
                 real = a < n && b < n && c < n;   // no far point
                 em = real && p < a && p < b && p < c;
             }
@@ -1041,8 +1044,8 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
             int4 tet = make_int4(0, 0, 0, 0);
             if (t < nt) {
                 const uint32_t tw = S.tri[t];
-                const int a = __float_as_int(S.pl[tw & 255u].w), b = __float_as_int(S.pl[(tw >> 8) & 255u].w),
-                          c = __float_as_int(S.pl[(tw >> 16) & 255u].w);
+                const int a = __float_as_int(S.pl[tw & SM].w), b = __float_as_int(S.pl[(tw >> SB) & SM].w),
+                          c = __float_as_int(S.pl[(tw >> (2 * SB)) & SM].w);
                 em = a < n && b < n && c < n && p < a && p < b && p < c;
                 // rotate (a, b, c) cyclically to put the smallest first: orientation is kept
                 if (a < b && a < c) tet = make_int4(p, a, b, c);
@@ -1067,10 +1070,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dt
     cell_run(S, (int)blockIdx.x, A);
 }
 
-// Cells that outgrew the main pass, with four times the room; a fixed grid walks the queue.
+// Cells that outgrew the main pass, with 3.75 times the vertices and eight times the planes; a
+// fixed grid walks the queue.
+static_assert(sizeof(CellSmem<256, 128>) <= 65536 && sizeof(CellSmem<960, 1024>) <= 65536,
+              "a cell's LDS must fit one workgroup's 64 KB");
 __global__ __launch_bounds__(64) void dt_cell_retry_kernel(CellArgs A, const int32_t* __restrict__ queue,
                                                            const unsigned long long* __restrict__ queue_len) {
-    __shared__ CellSmem<1024, 256> S;
+    __shared__ CellSmem<960, 1024> S;
     const unsigned long long len = *queue_len;
     for (unsigned long long i = blockIdx.x; i < len; i += gridDim.x) cell_run(S, queue[i], A);
 }

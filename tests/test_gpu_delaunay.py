@@ -169,3 +169,16 @@ def test_capacity_retry_and_del_dropin():
     assert (ind < pts.shape[0]).all()
     ref = O.delaunay(pts.numpy().astype(np.float32).astype(np.float64))
     assert (O.canonical_tets(ind) == O.canonical_tets(ref)).all()
+
+
+def test_cell_beyond_the_main_pass_capacity():
+    """The centre of 300 points on a sphere has a 596-vertex Voronoi cell with 300 faces, beyond
+    the main pass (256 vertices, 128 planes): the retry pass (960 vertices, 1024 planes) computes it."""
+    rng = np.random.default_rng(6)
+    d = rng.normal(0, 1, (300, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts = np.concatenate([np.zeros((1, 3)), d]).astype(np.float32)
+    print("sphere 300 + centre", flush=True)
+    t, info = _same_as_scipy(pts, "sphere+centre")
+    assert info["max_cell_vertices"] > 256
+    assert (t == 0).any(axis=1).sum() == 2 * 300 - 4
