@@ -59,6 +59,9 @@ def parse():
                    help="skip the BF16-vs-FP64 field MLP report on the benchmark mesh")
     p.add_argument("--no-layout", action="store_true", help="keep the Delaunay order of vertices and tets")
     p.add_argument("--no-delaunay", action="store_true", help="skip the GPU Delaunay leg")
+    p.add_argument("--delaunay-only", action="store_true",
+                   help="only triangulate the benchmark's GMM points (float32) on the GPU, 3 timed calls "
+                        "after one warm-up, and print one JSON line (for profiling; no CPU mesh needed)")
     p.add_argument("--delaunay-cpu-points", type=int, default=100_000,
                    help="points of the scipy (Qhull) CPU reference sample in the Delaunay leg")
     p.add_argument("--mesh-cache", default="", help="npz file to load the mesh from / save it to")
@@ -480,6 +483,22 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.delaunay_only:
+        import torch
+        from radiance_meshes_amd import scenes
+        from radiance_meshes_amd.delaunay import delaunay_tets
+        P = torch.from_numpy(scenes.gmm_points(args.points, 0).astype(np.float32)).cuda()
+        delaunay_tets(P)
+        torch.cuda.synchronize()
+        wall = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            t, info = delaunay_tets(P, return_info=True)
+            torch.cuda.synchronize()
+            wall.append(time.perf_counter() - t0)
+        print(json.dumps({"delaunay_points": int(P.shape[0]), "tets": int(info["n_tets"]),
+                          "ms_per_call": round(1e3 * sum(wall) / 3, 2), "info": info}), flush=True)
         return
 
     # the serving leg's other meshes build on host cores while the GPU runs the training leg
