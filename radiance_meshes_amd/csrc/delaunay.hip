@@ -431,7 +431,8 @@ __global__ __launch_bounds__(256) void dt_ranges_kernel(const int64_t* __restric
 //
 // The cell lives in LDS: plane table pl (point position and index), triangles tri (three 8-bit
 // plane slots and a flag for ill-conditioned vertices) with their vertex positions relative to p.
-// The main pass holds up to 256 vertices and 128 planes; cells that outgrow it are queued and
+// The main pass holds up to 128 vertices and 128 planes (9.9 KB of LDS: 3 waves per SIMD; a
+// typical cell has 26 vertices, 1 in 1000 more than 60); cells that outgrow it are queued and
 // recomputed by a retry pass holding up to 960 vertices and 1024 planes (a workgroup's LDS stays
 // within 64 KB).
 template <int MAXT, int MAXP>
@@ -1066,13 +1067,13 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dt_cell_kernel(CellArgs A) {
-    __shared__ CellSmem<256, 128> S;
+    __shared__ CellSmem<128, 128> S;
     cell_run(S, (int)blockIdx.x, A);
 }
 
 // Cells that outgrew the main pass, with 3.75 times the vertices and eight times the planes; a
 // fixed grid walks the queue.
-static_assert(sizeof(CellSmem<256, 128>) <= 65536 && sizeof(CellSmem<960, 1024>) <= 65536,
+static_assert(sizeof(CellSmem<128, 128>) <= 65536 && sizeof(CellSmem<960, 1024>) <= 65536,
               "a cell's LDS must fit one workgroup's 64 KB");
 __global__ __launch_bounds__(64) void dt_cell_retry_kernel(CellArgs A, const int32_t* __restrict__ queue,
                                                            const unsigned long long* __restrict__ queue_len) {

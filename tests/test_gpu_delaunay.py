@@ -173,7 +173,7 @@ def test_capacity_retry_and_del_dropin():
 
 def test_cell_beyond_the_main_pass_capacity():
     """The centre of 300 points on a sphere has a 596-vertex Voronoi cell with 300 faces, beyond
-    the main pass (256 vertices, 128 planes): the retry pass (960 vertices, 1024 planes) computes it."""
+    the main pass (128 vertices, 128 planes): the retry pass (960 vertices, 1024 planes) computes it."""
     rng = np.random.default_rng(6)
     d = rng.normal(0, 1, (300, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
