@@ -451,18 +451,37 @@ struct __align__(16) CellSmem {
     float4 spt[kWave];
     int bpre[kWave];           // a batch of grid cells: first flat index of each lane's cell
     int bst[kWave];            // and its first position in the sorted point list
-    int n_new;
 };
 
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+// Whole-wave reductions: DPP within each 16-lane row (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4,
+// row_ror 8), then the four rows' results by readlane. Every lane must be active.
+template <int CTRL>
+__device__ inline int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)), dpp_i32<CTRL>(__double2loint(v)));
+}
+__device__ inline double lane_f64(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ inline double wave_max(double v) {   // of non-negative values
+    v = fmax(v, dpp_f64<0xB1>(v));
+    v = fmax(v, dpp_f64<0x4E>(v));
+    v = fmax(v, dpp_f64<0x124>(v));
+    v = fmax(v, dpp_f64<0x128>(v));
+    return fmax(fmax(lane_f64(v, 0), lane_f64(v, 16)), fmax(lane_f64(v, 32), lane_f64(v, 48)));
 }
 __device__ inline uint32_t wave_or(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
-    return v;
+    int x = (int)v;
+    x |= dpp_i32<0xB1>(x);
+    x |= dpp_i32<0x4E>(x);
+    x |= dpp_i32<0x124>(x);
+    x |= dpp_i32<0x128>(x);
+    return (uint32_t)(__builtin_amdgcn_readlane(x, 0) | __builtin_amdgcn_readlane(x, 16) |
+                      __builtin_amdgcn_readlane(x, 32) | __builtin_amdgcn_readlane(x, 48));
 }
 __device__ inline int lanes_below(uint64_t m, int lane) {
     return __popcll(m & ((1ull << lane) - 1ull));
@@ -487,10 +506,9 @@ __device__ inline void vertex_of(const double a[3], double ra, const double b[3]
     const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
     const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
     const double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
-    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]),
-                 nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    const double kappa = fabs(det) / (na * nb * nc);
-    err = kappa > 1e-15 ? (float)fmin(1e-15 / kappa, 1.0) : 1.0f;
+    const double k2 = det * det / ((a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) *
+                                   (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]));   // kappa^2
+    err = k2 > 1e-30 ? fminf(1e-15f / sqrtf((float)k2), 1.0f) : 1.0f;
     if (!(fabs(det) > 0.0)) {
         x = y = 0.0;
         z = 1e150;
@@ -600,17 +618,24 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
     bool dead = false, fail = false;
     int exact = 0, range = 0;
     __syncthreads();
-    double maxr2;
-    {
+    // squared reach of the cell: the largest |v - p|^2 over its vertices (widened by their error
+    // bounds); a point farther than twice the reach cannot cut it. Recomputed once per round of
+    // candidates, so between two recomputations it only overestimates (the cell only shrinks).
+    auto cell_reach2 = [&]() -> double {
         double r2 = 0.0;
-        if (lane < 4) {
-            const double4 V = S.vt[lane];
-            const double rr = V.x * V.x + V.y * V.y + V.z * V.z;
-            const double er = V.w;
-            r2 = er < kLost ? rr * (1.0 + 16.0 * er) : 4.0 * rr;
+#pragma unroll
+        for (int j = 0; j < TPL; ++j) {
+            const int t = lane + kWave * j;
+            if (t < nt) {
+                const double4 V = S.vt[t];
+                const double rr = V.x * V.x + V.y * V.y + V.z * V.z;
+                const double er = V.w;
+                r2 = fmax(r2, er < kLost ? rr * (1.0 + 16.0 * er) : 4.0 * rr);
+            }
         }
-        maxr2 = wave_max(r2);
-    }
+        return wave_max(r2);
+    };
+    double maxr2 = cell_reach2();
 
     // a box (relative to p) can hold a point that cuts the cell only if it meets some ball
     // B(v, |v - p|) over the cell's vertices v
@@ -659,6 +684,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
         S.bst[lane] = s;
         __syncthreads();
         for (int c0 = 0; c0 < total; c0 += kWave) {
+            maxr2 = cell_reach2();
             const int kf = c0 + lane;
             const bool have = kf < total;
             int q = -1;
@@ -770,41 +796,47 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                     if (c) S.ctri[nclip + lanes_below(cm, lane)] = S.tri[t];
                     nclip += __popcll(cm);
                 }
-                if (lane == 0) {
-                    S.pl[slot] = make_float4(Q.x, Q.y, Q.z, __int_as_float(cq));
-                    S.n_new = 0;
-                }
+                if (lane == 0) S.pl[slot] = make_float4(Q.x, Q.y, Q.z, __int_as_float(cq));
                 __syncthreads();
                 // 3. boundary of the removed region: edge (i -> j) of a removed triangle whose
                 // neighbour (the triangle holding j -> i) is not removed gives the new triangle
                 // (i, j, slot); the neighbour is removed iff some removed triangle holds j -> i
-                for (int i = lane; i < nclip; i += kWave) {
-                    const uint32_t tw = S.ctri[i];
-                    const uint32_t a = tw & SM, b = (tw >> SB) & SM, c = (tw >> (2 * SB)) & SM;
-                    bool f0 = false, f1 = false, f2 = false;
-                    for (int u = 0; u < nclip; ++u) {
-                        const uint32_t uw = S.ctri[u];
-                        const uint32_t u0 = uw & SM, u1 = (uw >> SB) & SM, u2 = (uw >> (2 * SB)) & SM;
-                        f0 |= (u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a);
-                        f1 |= (u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b);
-                        f2 |= (u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c);
+                int n_new = 0;
+                for (int i0 = 0; i0 < nclip; i0 += kWave) {
+                    const int i = i0 + lane;
+                    uint32_t a = 0, b = 0, c = 0;
+                    bool e0 = false, e1 = false, e2 = false;
+                    if (i < nclip) {
+                        const uint32_t tw = S.ctri[i];
+                        a = tw & SM;
+                        b = (tw >> SB) & SM;
+                        c = (tw >> (2 * SB)) & SM;
+                        bool f0 = false, f1 = false, f2 = false;
+                        for (int u = 0; u < nclip; ++u) {
+                            const uint32_t uw = S.ctri[u];
+                            const uint32_t u0 = uw & SM, u1 = (uw >> SB) & SM, u2 = (uw >> (2 * SB)) & SM;
+                            f0 |= (u0 == b && u1 == a) || (u1 == b && u2 == a) || (u2 == b && u0 == a);
+                            f1 |= (u0 == c && u1 == b) || (u1 == c && u2 == b) || (u2 == c && u0 == b);
+                            f2 |= (u0 == a && u1 == c) || (u1 == a && u2 == c) || (u2 == a && u0 == c);
+                        }
+                        e0 = !f0;
+                        e1 = !f1;
+                        e2 = !f2;
                     }
+                    // positions in the new-triangle list from ballots, edge by edge
                     const uint32_t sl = (uint32_t)slot << (2 * SB);
-                    if (!f0) {
-                        const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = a | b << SB | sl;
-                    }
-                    if (!f1) {
-                        const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = b | c << SB | sl;
-                    }
-                    if (!f2) {
-                        const int pos = atomicAdd(&S.n_new, 1);
-                        if (pos < MAXT) S.newtri[pos] = c | a << SB | sl;
-                    }
+                    const uint64_t m0 = __ballot(e0), m1 = __ballot(e1), m2 = __ballot(e2);
+                    int pos = n_new + lanes_below(m0, lane);
+                    if (e0 && pos < MAXT) S.newtri[pos] = a | b << SB | sl;
+                    n_new += __popcll(m0);
+                    pos = n_new + lanes_below(m1, lane);
+                    if (e1 && pos < MAXT) S.newtri[pos] = b | c << SB | sl;
+                    n_new += __popcll(m1);
+                    pos = n_new + lanes_below(m2, lane);
+                    if (e2 && pos < MAXT) S.newtri[pos] = c | a << SB | sl;
+                    n_new += __popcll(m2);
                 }
                 __syncthreads();
-                const int n_new = S.n_new;
                 const int kept = nt - nclip;
                 if (kept + n_new > MAXT) {
                     fail = true;
@@ -849,7 +881,6 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                 uint32_t uu[UW];
 #pragma unroll
                 for (int w = 0; w < UW; ++w) uu[w] = 0u;
-                double r2 = 0.0;
 #pragma unroll
                 for (int j = 0; j < TPL; ++j) {
                     const int t = lane + kWave * j;
@@ -862,15 +893,10 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                             uu[w] |= (sb >> 5) == (uint32_t)w ? 1u << (sb & 31) : 0u;
                             uu[w] |= (sc >> 5) == (uint32_t)w ? 1u << (sc & 31) : 0u;
                         }
-                        const double4 V = S.vt[t];
-                        const double rr = V.x * V.x + V.y * V.y + V.z * V.z;
-                        const double er = V.w;
-                        r2 = fmax(r2, er < kLost ? rr * (1.0 + 16.0 * er) : 4.0 * rr);
                     }
                 }
 #pragma unroll
                 for (int w = 0; w < UW; ++w) used[w] = wave_or(uu[w]);
-                maxr2 = wave_max(r2);
             }
             __syncthreads();
             if (fail) break;
@@ -889,6 +915,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
     constexpr int kFineRings = 3;
     bool done = false;
     for (int r = 0; r <= kFineRings && !dead && !fail; ++r) {
+        maxr2 = cell_reach2();
         if (r > 0) {
             // lower bound on |q - p| for every point in a cell at Chebyshev distance >= r
             double L = DBL_MAX;
@@ -940,6 +967,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
         const int gb = (gg + 3) >> 2;
         const int bc[3] = {cx >> 2, cy >> 2, cz >> 2};
         for (int rb = 0; !dead && !fail; ++rb) {
+            maxr2 = cell_reach2();
             if (rb > 0) {
                 double L = DBL_MAX;
                 for (int c = 0; c < 3; ++c) {
