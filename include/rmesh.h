@@ -186,16 +186,20 @@ int rm_activate_bwd(const float* camera_center_dev, const float* density_dev, co
  * by render_err (delaunay_rasterization/internal/render_err.py:138-150). Pixel-centre rays walk
  * the same binned lists as rm_composite_fwd, starting from log-transmittance 1 (that kernel's
  * initial state) and stopping after a blend below -5.5; every blended tet receives
- * w = exp(logT) * sigma * dt. Adds into tet_err_dev [T,16] f32 (caller-zeroed):
+ * w = exp(logT) * sigma * dt. Adds into tet_err_dev [n_tets,16] f32 (caller-zeroed):
  *   0: w, 1: w*pixel_err, 2: w*pixel_err^2, 3, 4: untouched, 5: w*ssim_err,
- *   6-8: w*entry point, 9-11: w*exit point, 12: w, 13-15: w*segment midpoint,
- * and into tet_count_dev [T,2] i32 (caller-zeroed): 0: blended pixels, 1: max int(65535 w).
+ *   6-8: w*entry point, 9-11: w*exit point,
+ * then writes 12: channel 0 and 13-15: (channels 6-8 + channels 9-11) / 2 -- the sums of the
+ * reference's per-vote w and w*segment midpoint, so they stay consistent when calls accumulate --
+ * and adds into tet_count_dev [n_tets,2] i32 (caller-zeroed): 0: blended pixels, 1: max int(65535 w).
+ * tet_err_dev must be 16-byte aligned.
  * pixel_err_dev / ssim_err_dev are [H,W] f32. The reference's gt, output_img and n_contributors
  * arguments are read by neither kernel and do not cross this boundary. */
 int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
                const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                const float* features_dev, int32_t n_features, const float* pixel_err_dev,
-               const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev, rm_stream_t stream);
+               const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev, int64_t n_tets,
+               rm_stream_t stream);
 
 /* ---- batched forward rendering (serving) -----------------------------------------------
  * No counterpart in the reference (which renders one view per call, evaluate.py:50-63 /

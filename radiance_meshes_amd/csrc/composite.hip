@@ -1164,17 +1164,26 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
                     case 9: val = v[7]; break;
                     case 10: val = v[8]; break;
                     case 11: val = v[9]; break;
-                    case 13: val = 0.5f * (v[4] + v[7]); break;
-                    case 14: val = 0.5f * (v[5] + v[8]); break;
-                    case 15: val = 0.5f * (v[6] + v[9]); break;
                     default: val = 0.f; break;   // channels 3 and 4 carry no vote
                 }
-                if (gl != 3 && gl != 4) unsafeAtomicAdd(tet_err + 16 * (int64_t)tet + gl, val);
+                // channels 12-15 (w again, w * segment midpoint) follow from 0 and 6-11: written
+                // once per tet by tet_err_finish_kernel instead of four more atomics per vote row
+                if (gl != 3 && gl != 4 && gl < 12) unsafeAtomicAdd(tet_err + 16 * (int64_t)tet + gl, val);
                 if (gl == 0) atomicAdd(tet_count + 2 * (int64_t)tet, n_hit);
                 if (gl == 1) atomicMax(tet_count + 2 * (int64_t)tet + 1, wmax);
             }
         }
     }
+}
+
+// Channels 12-15 of the vote rows from the accumulated 0 and 6-11: 12 = sum w, 13-15 = sum of
+// w * (entry + exit) / 2, the reference's per-vote midpoint terms summed.
+__global__ __launch_bounds__(256) void tet_err_finish_kernel(float4* __restrict__ err, int64_t n_tets) {
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= n_tets) return;
+    const float4 r0 = err[4 * t], r1 = err[4 * t + 1], r2 = err[4 * t + 2];
+    // r1 = channels 4-7, r2 = 8-11: entry (6, 7, 8) and exit (9, 10, 11)
+    err[4 * t + 3] = make_float4(r0.x, 0.5f * (r1.z + r2.y), 0.5f * (r1.w + r2.z), 0.5f * (r2.x + r2.w));
 }
 
 template <int TS, int F, bool P64>
@@ -1370,8 +1379,9 @@ extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, i
                           const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                           const float* features_dev, int32_t n_features, const float* pixel_err_dev,
                           const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev,
-                          rm_stream_t stream) {
+                          int64_t n_tets, rm_stream_t stream) {
     RM_CHECK_ARG(n_sorted >= 0, "rm_tet_err: negative n_sorted");
+    RM_CHECK_ARG(n_tets >= 0, "rm_tet_err: negative n_tets");
     if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
     // the votes use pixel-centre rays: any non-null, aligned pointer stands in for the jitter check
     int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
@@ -1391,6 +1401,9 @@ extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, i
     }
 #undef RM_ERRV_P
 #undef RM_ERRV
+    if (n_tets > 0)
+        hipLaunchKernelGGL(tet_err_finish_kernel, dim3((unsigned)ceil_div(n_tets, 256)), dim3(256), 0, st,
+                           (float4*)tet_err_dev, n_tets);
     RM_CHECK_LAUNCH("tet_err_kernel");
     return RM_OK;
 }

@@ -84,7 +84,7 @@ def calc_tet_err(sorted_tetra_idx, tile_ranges, indices, vertices, tet_density, 
         check(lib().rm_tet_err(_cam(tcam, render_grid), ptr(sorted_tetra_idx), sorted_tetra_idx.numel(),
                                ptr(tile_ranges), ptr(indices), ptr(vertices), ptr(tet_density),
                                tet_density.shape[1], ptr(pixel_err), ptr(ssim_err), ptr(tet_err), ptr(tet_count),
-                               stream_of(dev)), "rm_tet_err")
+                               T, stream_of(dev)), "rm_tet_err")
     return tet_err, tet_count
 
 
