@@ -564,6 +564,7 @@ struct CellArgs {
     int64_t* pt_off;
     int64_t* info;
     const int32_t* ext;                // kExt extreme points
+    const int32_t* order;              // main pass: block b computes point order[b]
     int32_t* retry;                    // cells that outgrew the main pass (null in the retry pass)
     unsigned long long* retry_count;
 };
@@ -736,7 +737,9 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
                 for (int j = 0; j < kWave; ++j) {
                     const double dj = S.cd2[j];
                     const int ij = S.cid[j];
-                    rank += (dj < d2) || (dj == d2 && ij < q);
+                    // ties (the same point offered by two lanes, as the extreme-point seeds can)
+                    // broken by lane, so the ranks are a permutation and every slot gets written
+                    rank += (dj < d2) || (dj == d2 && (ij < q || (ij == q && j < lane)));
                 }
                 S.sd2[rank] = d2;
                 S.sid[rank] = q;
@@ -1096,7 +1099,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dt_cell_kernel(CellArgs A) {
     __shared__ CellSmem<128, 128> S;
-    cell_run(S, (int)blockIdx.x, A);
+    cell_run(S, A.order[blockIdx.x], A);
 }
 
 // Cells that outgrew the main pass, with 3.75 times the vertices and eight times the planes; a
@@ -1132,7 +1135,8 @@ inline int grid_dim(int64_t n) {
 }
 
 struct DtLayout {
-    size_t keys, keys2, vals, vals2, sortws, ranges, partial, gp, app, cnt, incl, off, scanws, appcount, retry, ext, total;
+    size_t keys, keys2, vals, vals2, sortws, ranges, partial, gp, app, cnt, incl, off, scanws, appcount, retry, ext, order,
+        total;
     int g, nb;
 };
 
@@ -1158,8 +1162,43 @@ inline DtLayout dt_layout(int64_t n, int64_t cap) {
     L.appcount = o; o += al(16);   // append counter, retry-queue length
     L.retry = o; o += al(4 * (size_t)n);
     L.ext = o; o += al(8 * kExt + 4 * kExt);   // keys, then ids
+    L.order = o; o += al(4 * (size_t)n);
     L.total = o;
     return L;
+}
+
+// Work estimate per point for the dispatch order: the number of points in its 3x3x3 block of grid
+// cells (sparse neighbourhoods make large, slow cells; points outside the grid's range get 0).
+// Sorting by it starts the slow cells first, so no long cell runs alone at the end of the grid.
+__global__ __launch_bounds__(256) void dt_weight_kernel(const float* __restrict__ pts, int64_t n,
+                                                        const GridParams* __restrict__ gp,
+                                                        const int2* __restrict__ cell_rng, int64_t* keys,
+                                                        int32_t* vals) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n) return;
+    const GridParams g = *gp;
+    const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    bool outside = false;
+    int c[3];
+    for (int k = 0; k < 3; ++k) {
+        outside |= x[k] < g.lo[k] || x[k] > g.lo[k] + g.g * g.h[k];
+        c[k] = cell_coord(x[k], g, k);
+    }
+    int64_t w = 0;
+    if (!outside)
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int X = c[0] + dx, Y = c[1] + dy, Z = c[2] + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= g.g || Y >= g.g || Z >= g.g) continue;
+                    const int2 r = cell_rng[((int64_t)Z * g.g + Y) * g.g + X];
+                    w += r.y - r.x;
+                }
+    // equal estimates in a hashed order: inputs sorted along a space-filling curve would otherwise
+    // start the cells of one region together
+    const uint32_t h = ((uint32_t)i * 2654435761u) >> 20;   // 12 bits
+    keys[i] = ((w < (1 << 20) - 1 ? w : (1 << 20) - 1) << 12) | h;
+    vals[i] = (int32_t)i;
 }
 
 __global__ void dt_zero_u64_kernel(unsigned long long* p, unsigned long long* ext_keys) {
@@ -1252,6 +1291,14 @@ extern "C" int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* t
     ca.pt_off = off;
     ca.info = info_dev;
     ca.ext = ext_ids;
+    // dispatch order: most expensive cells first (the sorted-by-cell keys and the input key/value
+    // buffers are free once the cell ranges exist)
+    int32_t* order = (int32_t*)(ws + L.order);
+    hipLaunchKernelGGL(dt_weight_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, points_dev, n,
+                       (const GridParams*)gp, (const int2*)ranges, keys, vals);
+    rc = rm_sort_pairs(keys, vals, keys2, order, n, 32, ws + L.sortws, rm_sort_pairs_workspace_size(n), stream);
+    if (rc != RM_OK) return rc;
+    ca.order = order;
     ca.retry = retry;
     ca.retry_count = retrycount;
     hipLaunchKernelGGL(dt_cell_kernel, dim3((unsigned)n), dim3(kWave), 0, st, ca);
