@@ -309,7 +309,7 @@ int rm_field_scatter_bwd(const float* raw_dev, const int32_t* tet_ids_dev, int64
  *   3: duplicate points dropped (the copy with the lowest index keeps the cell),
  *   4: predicate evaluations that needed exact arithmetic,
  *   5: of those, inputs outside its range (coordinate exponents spanning more than 99 bits),
- *   6: cell topology errors (non-zero = failed), 7: largest cell, in vertices.
+ *   6: cells recomputed by the larger second pass, 7: largest cell (in vertices) of the first pass.
  * Exact ties (co-spherical or coplanar points) are resolved by a symbolic perturbation, so the
  * result is one valid Delaunay tetrahedralisation also for degenerate input. The point at infinity
  * is modelled by four far points (a tetrahedron 1e8 times the point-set extent): a hull tet whose

@@ -39,7 +39,7 @@ constexpr double kLost = 1e-3;              // vertex error estimate beyond whic
 constexpr double kReach = 1.0 + 1e-6;       // safety factor on the 2|v-p| reach bound
 
 // info_dev slots (int64)
-enum { kInfoTets = 0, kInfoRealVerts, kInfoOverflow, kInfoDup, kInfoExact, kInfoRange, kInfoTopo, kInfoMaxT,
+enum { kInfoTets = 0, kInfoRealVerts, kInfoOverflow, kInfoDup, kInfoExact, kInfoRange, kInfoRetried, kInfoMaxT,
        kInfoCount = 8 };
 
 struct GridParams {
@@ -409,7 +409,12 @@ __global__ __launch_bounds__(256) void dt_keys_kernel(const float* __restrict__ 
     const GridParams g = *gp;
     const int cx = cell_coord(pts[3 * i], g, 0), cy = cell_coord(pts[3 * i + 1], g, 1),
               cz = cell_coord(pts[3 * i + 2], g, 2);
-    keys[i] = ((int64_t)cz * g.g + cy) * g.g + cx;
+    // cell id above 20 bits of an index hash: within a cell the points come in a scrambled order.
+    // A cell's points are clipped 64 at a time; in index order an input sorted along a space-
+    // filling curve would hand over 64 points of one corner of a dense cell at once, each cutting
+    // a sliver off a still-large cell, and the cell would outgrow its capacity.
+    const uint64_t h = ((uint32_t)i * 2654435761u) >> 12;   // 20 bits
+    keys[i] = (int64_t)(((uint64_t)(((int64_t)cz * g.g + cy) * g.g + cx) << 20) | h);
     vals[i] = (int32_t)i;
 }
 
@@ -421,9 +426,9 @@ __global__ __launch_bounds__(256) void dt_zero_ranges_kernel(int2* r, int64_t m)
 __global__ __launch_bounds__(256) void dt_ranges_kernel(const int64_t* __restrict__ keys, int64_t n, int2* r) {
     const int64_t i = blockIdx.x * 256ll + threadIdx.x;
     if (i >= n) return;
-    const int64_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) r[k].x = (int)i;
-    if (i == n - 1 || keys[i + 1] != k) r[k].y = (int)(i + 1);
+    const int64_t k = keys[i] >> 20;   // the cell id (the low bits order points within a cell)
+    if (i == 0 || (keys[i - 1] >> 20) != k) r[k].x = (int)i;
+    if (i == n - 1 || (keys[i + 1] >> 20) != k) r[k].y = (int)(i + 1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1059,6 +1064,7 @@ __device__ void cell_run(CellSmem<MAXT, MAXP>& S, const int p, const CellArgs& A
         if (nreal) atomicAdd((unsigned long long*)&A.info[kInfoRealVerts], (unsigned long long)nreal);
         if (dead) atomicAdd((unsigned long long*)&A.info[kInfoDup], 1ull);
         if (requeue) {
+            atomicAdd((unsigned long long*)&A.info[kInfoRetried], 1ull);
             const unsigned long long slot = atomicAdd(A.retry_count, 1ull);
             A.retry[slot] = p;   // at most one entry per point: the list holds n
         } else if (fail) {
@@ -1268,6 +1274,7 @@ extern "C" int rm_delaunay(const float* points_dev, int64_t n_points, int32_t* t
     RM_CHECK_LAUNCH("rm_delaunay grid");
     int end_bit = 1;
     while ((1ll << end_bit) < cells) ++end_bit;
+    end_bit += 20;
     int rc = rm_sort_pairs(keys, vals, keys2, vals2, n, end_bit, ws + L.sortws, rm_sort_pairs_workspace_size(n),
                            stream);
     if (rc != RM_OK) return rc;

@@ -19,7 +19,7 @@ import torch
 from ._capi import check, lib, ptr, stream_of
 
 INFO_FIELDS = ("n_tets", "voronoi_vertices", "overflow_cells", "duplicates", "exact_evaluations",
-               "exact_out_of_range", "topology_errors", "max_cell_vertices")
+               "exact_out_of_range", "retried_cells", "max_cell_vertices")
 
 
 def delaunay_tets(points: torch.Tensor, capacity: int | None = None, return_info: bool = False):
@@ -47,7 +47,7 @@ def delaunay_tets(points: torch.Tensor, capacity: int | None = None, return_info
             break
         cap = int(vals[0])
     d = dict(zip(INFO_FIELDS, vals))
-    if d["overflow_cells"] or d["topology_errors"] or d["voronoi_vertices"] != 4 * d["n_tets"]:
+    if d["overflow_cells"] or d["voronoi_vertices"] != 4 * d["n_tets"]:
         raise RuntimeError(f"rm_delaunay: cells disagree or overflowed: {d}")
     out = tets[:d["n_tets"]]
     return (out, d) if return_info else out
