@@ -151,3 +151,27 @@ def test_predicate_wide_exponent_range(hostlib):
         x = _oriented(x)
         ids = list(rng.permutation(20)[:5])
         assert _host(hostlib, ids, x) == int(O.perturbed_inside(x, ids)), (k, x, ids)
+
+
+def test_host_layer_rejects_bad_input_before_any_gpu_call():
+    """delaunay_tets / Del validate shapes, finiteness and the point count on the host."""
+    import torch
+    from radiance_meshes_amd.delaunay import Del, delaunay_tets
+    with pytest.raises(ValueError):
+        delaunay_tets(torch.zeros((5, 2)))
+    with pytest.raises(ValueError):
+        Del(3).compute(torch.zeros((4, 3)))
+    if not torch.cuda.is_available():
+        return
+    with pytest.raises(ValueError):
+        delaunay_tets(torch.tensor([[0.0, 0.0, float("nan")]] * 5).cuda())
+
+
+def test_gdel3d_alias_resolves_to_the_gpu_module():
+    import importlib
+    import sys
+    import radiance_meshes_amd
+    radiance_meshes_amd.install_reference_aliases()
+    mod = importlib.import_module("gdel3d")
+    assert mod is sys.modules["radiance_meshes_amd.delaunay"]
+    assert hasattr(mod, "Del")
