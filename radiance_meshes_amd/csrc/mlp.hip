@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(const float* __restri
 }
 
 template <int P>
-__global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void mlp_bwd_kernel(
     const float* __restrict__ x, int64_t n, const float* w1, const float* b1, const float* w2, const float* b2,
     const float* w3, const float* b3, int n_out, const float* __restrict__ gy, float* __restrict__ gx,
     float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3) {
@@ -219,12 +219,13 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
     // weight-gradient tiles (D layout): gW3 [16 x 64], gW2 [64 x 64], gW1 [64 x 32]; bias partials
     // per lane for the units it holds in the chain layout
     f4 g3[4], g2[4][4], g1[4][2];
-    f4 gb1p[4], gb2p[4], gb3p = {0.f, 0.f, 0.f, 0.f};
+    // bias partials from the point fragments: lane (q, c) sums unit 16*ot + c over its points 4q + s
+    float gb1s[4], gb2s[4], gb3s = 0.f;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         g3[a] = f4{0.f, 0.f, 0.f, 0.f};
-        gb1p[a] = f4{0.f, 0.f, 0.f, 0.f};
-        gb2p[a] = f4{0.f, 0.f, 0.f, 0.f};
+        gb1s[a] = 0.f;
+        gb2s[a] = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) g2[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
             const int64_t pp = t * 16 + 4 * q + r;
             dyp[r] = (pp < n && c < n_out) ? gy[pp * n_out + c] : 0.f;
         }
-        gb3p += dyc[0];
+        gb3s += (dyp[0] + dyp[1]) + (dyp[2] + dyp[3]);
         // gW3 += dY A2^T
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) put_block(tb + kt * kTBlock, a2[kt]);
@@ -265,7 +266,6 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
             f4 d = chain<P, 1>(dyc, [&](int i, int k) { return S.w3[k * SW3 + 16 * ot + i]; });
 #pragma unroll
             for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * dselu_of_act(a2[ot][r]);
-            gb2p[ot] += dh2[ot];
         }
         wave_sync();
         // gW2 += dH2 A1^T
@@ -276,16 +276,18 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
         }
         wave_sync();
         {
-            float dh2p[4][4], a1p[4][4];
+            // the dH2 fragments are read one block at a time, to keep the live registers down
+            float a1p[4][4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                get_block(tb + b * kTBlock, dh2p[b]);
-                get_block(tb + (4 + b) * kTBlock, a1p[b]);
+            for (int b = 0; b < 4; ++b) get_block(tb + (4 + b) * kTBlock, a1p[b]);
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot) {
+                float dh2p[4];
+                get_block(tb + ot * kTBlock, dh2p);
+                gb2s[ot] += (dh2p[0] + dh2p[1]) + (dh2p[2] + dh2p[3]);
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt) g2[ot][kt] = outer<P>(dh2p, a1p[kt], g2[ot][kt]);
             }
-#pragma unroll
-            for (int ot = 0; ot < 4; ++ot)
-#pragma unroll
-                for (int kt = 0; kt < 4; ++kt) g2[ot][kt] = outer<P>(dh2p[ot], a1p[kt], g2[ot][kt]);
         }
         // dH1 = (W2^T dH2) * selu'(H1)
         f4 dh1[4];
@@ -294,7 +296,6 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
             f4 d = chain<P, 4>(dh2, [&](int i, int k) { return S.w2[k * SW2 + 16 * ot + i]; });
 #pragma unroll
             for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * dselu_of_act(a1[ot][r]);
-            gb1p[ot] += dh1[ot];
         }
         // dX = W1^T dH1
         if (gx) {
@@ -322,6 +323,7 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
             for (int ot = 0; ot < 4; ++ot) {
                 float dh1p[4];
                 get_block(tb + ot * kTBlock, dh1p);
+                gb1s[ot] += (dh1p[0] + dh1p[1]) + (dh1p[2] + dh1p[3]);
 #pragma unroll
                 for (int kt = 0; kt < 2; ++kt) g1[ot][kt] = outer<P>(dh1p, xp[kt], g1[ot][kt]);
             }
@@ -349,11 +351,14 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_kernel(
             for (int kt = 0; kt < 4; ++kt) atomicAdd(&G2[(16 * ot + o) * kHid + 16 * kt + c], g2[ot][kt][r]);
 #pragma unroll
             for (int kt = 0; kt < 2; ++kt) atomicAdd(&G1[(16 * ot + o) * kIn + 16 * kt + c], g1[ot][kt][r]);
-            atomicAdd(&B1[16 * ot + o], gb1p[ot][r]);
-            atomicAdd(&B2[16 * ot + o], gb2p[ot][r]);
         }
-        atomicAdd(&B3[o], gb3p[r]);
     }
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+        atomicAdd(&B1[16 * ot + c], gb1s[ot]);
+        atomicAdd(&B2[16 * ot + c], gb2s[ot]);
+    }
+    atomicAdd(&B3[c], gb3s);
     __syncthreads();
     for (int i = threadIdx.x; i < kNGrad; i += blockDim.x) {
         const float v = TB[i];
