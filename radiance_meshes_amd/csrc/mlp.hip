@@ -24,6 +24,8 @@
 // the lanes' k, so dH and act blocks are transposed through a 16x17 LDS block per 16 units.
 // Weight-gradient tiles accumulate in registers across all tiles of a wave, are summed across the
 // workgroup in LDS and added to global memory with one float atomic per element.
+#include <atomic>
+
 #include "common.h"
 
 namespace rmesh {
@@ -33,6 +35,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 32, kHid = 64, kOutMax = 16;
+constexpr int kMaxDevices = 64;
 constexpr int kWaves = 4, kThreads = 64 * kWaves;
 constexpr int SW1 = kIn + 1, SW2 = kHid + 1, SW3 = kHid + 1;   // LDS row strides (floats)
 constexpr int kTS = 17, kTBlock = 16 * kTS;                    // transpose block (16 x 17 floats)
@@ -378,11 +381,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
+// Workgroups of `fn` that are resident on the current device at once (CUs x blocks per CU), cached
+// per kernel and device; `fallback` if the runtime cannot say. The kernels loop over the tiles in
+// grid strides, so a grid larger than this would run its last workgroups as a second, partly
+// occupied pass over the same amount of work per workgroup.
+inline int resident_blocks(const void* fn, std::atomic<int>* cache, int fallback) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return fallback;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0) != hipSuccess || cus <= 0 ||
+        per_cu <= 0) {
+        (void)hipGetLastError();
+        return fallback;
+    }
+    v = cus * per_cu;
+    cache[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
 inline unsigned grid_for(int64_t n, int cap) {
     int64_t tiles = (n + 15) / 16;
     int64_t blocks = (tiles + kWaves - 1) / kWaves;
     return (unsigned)(blocks < cap ? blocks : cap);
 }
+
+std::atomic<int> g_resident[4][kMaxDevices];   // mlp_fwd<0>, mlp_fwd<1>, mlp_bwd<0>, mlp_bwd<1>
 
 }  // namespace
 }  // namespace rmesh
@@ -398,7 +424,9 @@ extern "C" int rm_mlp_fwd(const float* x_dev, int64_t n, const float* w1, const 
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && w1 && b1 && w2 && b2 && w3 && b3 && y_dev, "rm_mlp_fwd: null buffer");
     RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0, "rm_mlp_fwd: x must be 16-byte aligned");
-    dim3 grid(grid_for(n, 1280)), block(kThreads);
+    const void* fn = precision == 0 ? reinterpret_cast<const void*>(&mlp_fwd_kernel<0>)
+                                    : reinterpret_cast<const void*>(&mlp_fwd_kernel<1>);
+    dim3 grid(grid_for(n, resident_blocks(fn, g_resident[precision], 1024))), block(kThreads);
     if (precision == 0)
         hipLaunchKernelGGL(mlp_fwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
                            n_out, y_dev);
@@ -421,7 +449,9 @@ extern "C" int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const 
                  "rm_mlp_bwd: null buffer");
     RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0 && ((uintptr_t)grad_x_dev & 15) == 0,
                  "rm_mlp_bwd: x and grad_x must be 16-byte aligned");
-    dim3 grid(grid_for(n, 512)), block(kThreads);
+    const void* fn = precision == 0 ? reinterpret_cast<const void*>(&mlp_bwd_kernel<0>)
+                                    : reinterpret_cast<const void*>(&mlp_bwd_kernel<1>);
+    dim3 grid(grid_for(n, resident_blocks(fn, g_resident[2 + precision], 512))), block(kThreads);
     if (precision == 0)
         hipLaunchKernelGGL(mlp_bwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
                            n_out, grad_y_dev, grad_x_dev, gw1, gb1, gw2, gb2, gw3, gb3);
