@@ -56,7 +56,7 @@ def _mlp_params(seed, n_out=13):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 2e-5), (1, 5e-2)])
-@pytest.mark.parametrize("n", [16, 1000, 40001])
+@pytest.mark.parametrize("n", [16, 1000, 40001, 200003])   # the last: several grid-stride passes
 def test_mlp_forward_matches_fp64(oracle_lib, precision, tol, n):
     import torch
     from radiance_meshes_amd.field import _MLP
@@ -72,11 +72,14 @@ def test_mlp_forward_matches_fp64(oracle_lib, precision, tol, n):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 2e-5), (1, 5e-2)])
-def test_mlp_backward_matches_fp64(precision, tol):
+@pytest.mark.parametrize("n", [3001, 100003])
+def test_mlp_backward_matches_fp64(precision, tol, n):
+    """n = 100003 exceeds one grid-stride pass of the resident workgroups (at most 5 per CU, 16
+    queries per wave), so waves carry their weight-gradient tiles over several query tiles; the
+    last tile is ragged."""
     import torch
     from radiance_meshes_amd.field import _MLP
     p = _mlp_params(5)
-    n = 3001
     x = np.random.default_rng(6).standard_normal((n, 32)).astype(np.float32)
     gy = np.random.default_rng(7).standard_normal((n, 13))
     print(f"mlp bwd n={n} precision={precision}", flush=True)
