@@ -64,17 +64,13 @@ __device__ inline void cell(const float* __restrict__ x, int64_t n, const GridAr
     cell_of(v, g, l, bx, by, bz, fx, fy, fz);
 }
 
+// Encoding of one clamped point at level l: the 8 corner rows gathered, trilinear in x -> y -> z.
 template <int F>
-__global__ __launch_bounds__(256) void hashgrid_fwd_kernel(const float* __restrict__ x, int64_t N,
-                                                           const float* __restrict__ table, GridArgs g,
-                                                           float* __restrict__ out) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N * g.L) return;
-    int64_t n = t / g.L;
-    int l = (int)(t - n * g.L);
+__device__ inline void encode_level(const float v[3], const GridArgs& g, int l, const float* __restrict__ table,
+                                    float (&res)[F]) {
     int bx, by, bz;
     float fx, fy, fz;
-    cell(x, n, g, l, bx, by, bz, fx, fy, fz);
+    cell_of(v, g, l, bx, by, bz, fx, fy, fz);
     const float* tb = table + g.offs[l] * F;
     float e[8][F];
 #pragma unroll
@@ -92,7 +88,61 @@ __global__ __launch_bounds__(256) void hashgrid_fwd_kernel(const float* __restri
         float c3 = e[3][f] * wx0 + e[7][f] * fx;
         c0 = c0 * wy0 + c2 * fy;
         c1 = c1 * wy0 + c3 * fy;
-        out[(n * g.L + l) * F + f] = c0 * wz0 + c1 * fz;
+        res[f] = c0 * wz0 + c1 * fz;
+    }
+}
+
+// One thread per (point, level), point-major (any L * F).
+template <int F>
+__global__ __launch_bounds__(256) void hashgrid_fwd_kernel(const float* __restrict__ x, int64_t N,
+                                                           const float* __restrict__ table, GridArgs g,
+                                                           float* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * g.L) return;
+    int64_t n = t / g.L;
+    int l = (int)(t - n * g.L);
+    float v[3];
+    load_point(x, n, v);
+    float r[F];
+    encode_level<F>(v, g, l, table, r);
+#pragma unroll
+    for (int f = 0; f < F; ++f) out[(n * g.L + l) * F + f] = r[f];
+}
+
+// Level-major forward for L * F <= kLmMaxRow: a workgroup takes 64 consecutive points, wave w
+// levels w, w + 4, ..., one point per lane, so each gather instruction serves 64 neighbouring
+// points at one level (spatially ordered points share corner rows at the coarser levels, and
+// the level's parameters are wave-uniform). The rows are assembled in LDS and written out as one
+// contiguous block.
+constexpr int kLmPoints = 64, kLmWaves = 4, kLmMaxRow = 64;
+
+template <int F>
+__global__ __launch_bounds__(kLmPoints * kLmWaves) void hashgrid_fwd_lm_kernel(const float* __restrict__ x,
+                                                                               int64_t N,
+                                                                               const float* __restrict__ table,
+                                                                               GridArgs g, float* __restrict__ out) {
+    __shared__ float rows[kLmPoints * (kLmMaxRow + 1)];
+    const int LF = g.L * F, stride = LF + 1;   // odd stride: a wave's 64 row writes hit distinct banks
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t n0 = (int64_t)blockIdx.x * kLmPoints;
+    const int64_t n = n0 + lane;
+    if (n < N) {
+        float v[3];
+        load_point(x, n, v);
+        for (int l = wave; l < g.L; l += kLmWaves) {
+            float r[F];
+            encode_level<F>(v, g, l, table, r);
+#pragma unroll
+            for (int f = 0; f < F; ++f) rows[lane * stride + l * F + f] = r[f];
+        }
+    }
+    __syncthreads();
+    const int64_t left = N - n0;
+    const int total = (int)(left < kLmPoints ? left : kLmPoints) * LF;
+    float* __restrict__ o = out + n0 * LF;
+    for (int i = threadIdx.x; i < total; i += kLmPoints * kLmWaves) {
+        const int pt = i / LF;
+        o[i] = rows[pt * stride + (i - pt * LF)];
     }
 }
 
@@ -578,6 +628,13 @@ extern "C" int rm_hashgrid_fwd(const float* x_dev, int64_t n, const float* table
     if (e) return e;
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(x_dev && table_dev && out_dev, "rm_hashgrid_fwd: null buffer");
+    if (n_levels * n_feat <= kLmMaxRow) {
+        dim3 grid((unsigned)ceil_div(n, kLmPoints)), block(kLmPoints * kLmWaves);
+        RM_FEAT_DISPATCH(n_feat, hashgrid_fwd_lm_kernel, grid, block, 0, as_stream(stream), x_dev, n, table_dev, g,
+                         out_dev);
+        RM_CHECK_LAUNCH("hashgrid_fwd_lm_kernel");
+        return RM_OK;
+    }
     int64_t total = n * n_levels;
     dim3 grid((unsigned)ceil_div(total, 256)), block(256);
     RM_FEAT_DISPATCH(n_feat, hashgrid_fwd_kernel, grid, block, 0, as_stream(stream), x_dev, n, table_dev, g, out_dev);

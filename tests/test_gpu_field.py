@@ -14,11 +14,14 @@ def _grid(seed=0, n=20000, L=16, F=2, log2=19, base=16, growth=1.38):
     return x, table, res, offs, L, F, log2
 
 
-def test_hashgrid_forward_matches_fp64(oracle_lib):
+@pytest.mark.parametrize("L,F,log2", [(16, 2, 19), (8, 8, 19), (3, 4, 19), (16, 8, 17)])
+def test_hashgrid_forward_matches_fp64(oracle_lib, L, F, log2):
+    """L * F <= 64 runs the level-major kernel (64 points per workgroup, 20000 = 312 x 64 + 32
+    leaves a ragged last group; L = 3 leaves a wave without a level), L * F = 128 the point-major one."""
     import torch
     from radiance_meshes_amd.field import _HashEncode
     O = oracle_lib
-    x, table, res, offs, L, F, log2 = _grid()
+    x, table, res, offs, L, F, log2 = _grid(L=L, F=F, log2=log2)
     print(f"hashgrid fwd n={len(x)} levels={L} dense={[int(r) ** 3 <= 2 ** log2 for r in res]}", flush=True)
     out = _HashEncode.apply(torch.from_numpy(x).cuda(), torch.from_numpy(table).cuda(), res, offs, L, F, log2)
     ref = O.hashgrid_fwd(x, table.astype(np.float64), res, offs, F, log2)
