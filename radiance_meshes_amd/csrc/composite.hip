@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
                 const Hit h = intersect(p, ray, cam.min_t);
                 const float x = S.fa[s].x * h.dt;
                 if (h.hit && x > 0.f) {
-                    w = expf(logT) * x;
+                    w = __expf(logT) * x;   // hardware exponential: ~4e-7 relative for logT in [-5.5, 1]
                     se = h.se;
                     sx = h.sx;
                     hit = 1;
