@@ -238,7 +238,7 @@ def run_config(args, pts, tets, cam, tile):
     target = torch.rand((H, W, 3), generator=torch.Generator().manual_seed(1)).to(dev)
     jitter = torch.full((H, W, 2), 0.5, device=dev)
     opt = torch.optim.Adam([{"params": field.parameters(), "lr": 1e-3},
-                            {"params": [vertices], "lr": 1e-6}], foreach=True)
+                            {"params": [vertices], "lr": 1e-6}], fused=True)
     set_intersect_precision(args.intersect == "fp64")
     stats = {}
 
