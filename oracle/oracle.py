@@ -62,6 +62,10 @@ def lib():
         L.orc_composite_fwd.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, P, P, P]
         L.orc_composite_bwd.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, P, P,
                                         P, P, P, P]
+        L.orc_composite_fwd_tiles.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, C.c_int64,
+                                              P, P, P, P]
+        L.orc_composite_bwd_tiles.argtypes = [C.POINTER(OrcCamera), P, P, P, P, P, C.c_int, P, P, C.c_int64,
+                                              P, P, P, P, P, P, P]
         L.orc_quadrature.argtypes = [C.POINTER(OrcCamera), P, P, C.c_int64, C.c_double, C.c_int,
                                      P, P, P, C.c_int, C.c_int64, P]
         L.orc_get_ray.argtypes = [C.POINTER(OrcCamera), C.c_double, C.c_double, P, P]
@@ -102,7 +106,9 @@ def bin_tiles(cam: OrcCamera, tiles, rect, vs):
     return keys[:P], vals[:P], ranges
 
 
-def composite_fwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter):
+def composite_fwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter, tiles=None):
+    """FP64 forward (img [H,W,4], dist [H,W,5], n_contrib [H,W], margin [H,W]); with `tiles` (tile ids)
+    only those tiles' pixels are computed and every other pixel stays zero (margin 0)."""
     H, W = cam.H, cam.W
     F = feats.shape[1]
     img = np.zeros((H, W, 4))
@@ -112,10 +118,13 @@ def composite_fwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, 
     v = np.ascontiguousarray(vertices, np.float64)
     f = np.ascontiguousarray(feats, np.float64)
     j = np.ascontiguousarray(jitter, np.float32)
-    lib().orc_composite_fwd(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
-                            _p(np.ascontiguousarray(ranges, np.int32)),
-                            _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F, _p(j),
-                            _p(img), _p(dist), _p(nc), _p(margin))
+    args = [_p(np.ascontiguousarray(sorted_idx, np.int32)), _p(np.ascontiguousarray(ranges, np.int32)),
+            _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F, _p(j)]
+    if tiles is None:
+        lib().orc_composite_fwd(C.byref(cam), *args, _p(img), _p(dist), _p(nc), _p(margin))
+    else:
+        tl = np.ascontiguousarray(tiles, np.int32)
+        lib().orc_composite_fwd_tiles(C.byref(cam), *args, _p(tl), len(tl), _p(img), _p(dist), _p(nc), _p(margin))
     return img, dist, nc, margin
 
 
@@ -137,12 +146,26 @@ def tet_err(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, pixel_
 
 
 def composite_bwd(cam: OrcCamera, sorted_idx, ranges, indices, vertices, feats, jitter, img, dist,
-                  nc, gimg, gdist):
+                  nc, gimg, gdist, tiles=None):
+    """FP64 adjoint (grad_vertices [V,3], grad_features [T,F]); with `tiles` only those tiles' pixels
+    are walked (the other pixels' incoming gradients are ignored)."""
     F = feats.shape[1]
     v = np.ascontiguousarray(vertices, np.float64)
     f = np.ascontiguousarray(feats, np.float64)
     gv = np.zeros_like(v)
     gf = np.zeros_like(f)
+    if tiles is not None:
+        tl = np.ascontiguousarray(tiles, np.int32)
+        lib().orc_composite_bwd_tiles(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
+                                      _p(np.ascontiguousarray(ranges, np.int32)),
+                                      _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F,
+                                      _p(np.ascontiguousarray(jitter, np.float32)), _p(tl), len(tl),
+                                      _p(np.ascontiguousarray(img, np.float64)),
+                                      _p(np.ascontiguousarray(dist, np.float64)),
+                                      _p(np.ascontiguousarray(nc, np.int32)),
+                                      _p(np.ascontiguousarray(gimg, np.float64)),
+                                      _p(np.ascontiguousarray(gdist, np.float64)), _p(gv), _p(gf))
+        return gv, gf
     lib().orc_composite_bwd(C.byref(cam), _p(np.ascontiguousarray(sorted_idx, np.int32)),
                             _p(np.ascontiguousarray(ranges, np.int32)),
                             _p(np.ascontiguousarray(indices, np.int32)), _p(v), _p(f), F,

@@ -16,6 +16,7 @@
  *                      integrate_tetra_segment (interp_version.slang:21-98), update_alpha_out
  *                      and the termination rule (alphablend_shader_interp.slang:97-126,153-183,
  *                      194-271), evaluated in double on the same per-tile lists.
+ *   (the *_tiles variants restrict either pass to a list of tiles)
  *   orc_composite_bwd  FP64 adjoint of orc_composite_fwd with the reference's backward structure
  *                      (bwd_alpha_blend, alphablend_shader_interp.slang:273-369): back-to-front
  *                      walk from the stored final state, offsets >= n_contrib skipped.
@@ -230,6 +231,7 @@ double orc_power_of_circumsphere(const double* A, const double* B, const double*
 
 void orc_project(const orc_camera* c, const int32_t* idx, const float* verts, int64_t T,
                  int32_t* tiles, int32_t* rect, float* vs, float* cc) {
+#pragma omp parallel for schedule(static, 4096)
     for (int64_t t = 0; t < T; ++t) {
         tiles[t] = 0;
         for (int k = 0; k < 4; ++k) rect[4 * t + k] = 0;
@@ -448,14 +450,42 @@ static const double* grd_row(const double* f, int F, int ch) { return F == 7 ? f
 /* Forward. out_img [H,W,4], out_dist [H,W,5], ncontrib [H,W], margin [H,W] (min |logT+5.5|
  * after any blend before termination; marks pixels whose termination decision is within the
  * margin of the threshold). */
-void orc_composite_fwd(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
-                       const int32_t* idx, const double* verts, const double* feats, int F,
-                       const float* jitter, double* out_img, double* out_dist, int32_t* ncontrib,
-                       double* margin) {
-    int H = c->H, W = c->W;
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int y = 0; y < H; ++y) {
-        for (int x = 0; x < W; ++x) {
+/* The pixels of the listed tiles (every pixel when tile_list is NULL) as (x, y) pairs in raster
+ * order within each tile; tiles are visited in list order. Caller frees *out. */
+static int64_t pixel_list(const orc_camera* c, const int32_t* tile_list, int64_t n_list, int32_t** out) {
+    int64_t n = 0;
+    if (!tile_list) {
+        n = (int64_t)c->H * c->W;
+        int32_t* p = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)(n > 0 ? n : 1));
+        int64_t q = 0;
+        for (int y = 0; y < c->H; ++y)
+            for (int x = 0; x < c->W; ++x) { p[2 * q] = x; p[2 * q + 1] = y; q++; }
+        *out = p;
+        return n;
+    }
+    int32_t* p = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)(n_list * c->th * c->tw + 1));
+    for (int64_t i = 0; i < n_list; ++i) {
+        int ty = tile_list[i] / c->gw, tx = tile_list[i] % c->gw;
+        for (int y = ty * c->th; y < (ty + 1) * c->th && y < c->H; ++y)
+            for (int x = tx * c->tw; x < (tx + 1) * c->tw && x < c->W; ++x) { p[2 * n] = x; p[2 * n + 1] = y; n++; }
+    }
+    *out = p;
+    return n;
+}
+
+/* The forward over the pixels of tile_list [n_list] only (all tiles when tile_list is NULL);
+ * other pixels' outputs are left untouched. */
+void orc_composite_fwd_tiles(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
+                             const int32_t* idx, const double* verts, const double* feats, int F,
+                             const float* jitter, const int32_t* tile_list, int64_t n_list,
+                             double* out_img, double* out_dist, int32_t* ncontrib, double* margin) {
+    int W = c->W;
+    int32_t* pl;
+    int64_t n_pix = pixel_list(c, tile_list, n_list, &pl);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < n_pix; ++q) {
+        {
+            int x = pl[2 * q], y = pl[2 * q + 1];
             int tile = (y / c->th) * c->gw + (x / c->tw);
             int s0 = ranges[2 * tile], s1 = ranges[2 * tile + 1];
             int64_t pix = (int64_t)y * W + x;
@@ -508,6 +538,15 @@ void orc_composite_fwd(const orc_camera* c, const int32_t* sorted, const int32_t
             if (margin) margin[pix] = mg;
         }
     }
+    free(pl);
+}
+
+void orc_composite_fwd(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
+                       const int32_t* idx, const double* verts, const double* feats, int F,
+                       const float* jitter, double* out_img, double* out_dist, int32_t* ncontrib,
+                       double* margin) {
+    orc_composite_fwd_tiles(c, sorted, ranges, idx, verts, feats, F, jitter, NULL, 0, out_img, out_dist,
+                            ncontrib, margin);
 }
 
 static void atomic_addd(double* p, double v) {
@@ -528,15 +567,19 @@ static void cross_bwd(const double* e1, const double* e2, const double* gn, doub
 static const int kFace[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 
 /* Backward. Gradients are accumulated (+=) into grad_verts [V,3], grad_feats [T,F]. */
-void orc_composite_bwd(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
-                       const int32_t* idx, const double* verts, const double* feats, int F,
-                       const float* jitter, const double* img, const double* dist,
-                       const int32_t* ncontrib, const double* gimg, const double* gdist,
-                       double* grad_verts, double* grad_feats) {
-    int H = c->H, W = c->W;
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int y = 0; y < H; ++y) {
-        for (int x = 0; x < W; ++x) {
+/* The backward over the pixels of tile_list [n_list] only (all tiles when tile_list is NULL). */
+void orc_composite_bwd_tiles(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
+                             const int32_t* idx, const double* verts, const double* feats, int F,
+                             const float* jitter, const int32_t* tile_list, int64_t n_list,
+                             const double* img, const double* dist, const int32_t* ncontrib,
+                             const double* gimg, const double* gdist, double* grad_verts, double* grad_feats) {
+    int W = c->W;
+    int32_t* pl;
+    int64_t n_pix = pixel_list(c, tile_list, n_list, &pl);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < n_pix; ++q) {
+        {
+            int x = pl[2 * q], y = pl[2 * q + 1];
             int tile = (y / c->th) * c->gw + (x / c->tw);
             int s0 = ranges[2 * tile];
             int64_t pix = (int64_t)y * W + x;
@@ -660,6 +703,16 @@ void orc_composite_bwd(const orc_camera* c, const int32_t* sorted, const int32_t
             }
         }
     }
+    free(pl);
+}
+
+void orc_composite_bwd(const orc_camera* c, const int32_t* sorted, const int32_t* ranges,
+                       const int32_t* idx, const double* verts, const double* feats, int F,
+                       const float* jitter, const double* img, const double* dist,
+                       const int32_t* ncontrib, const double* gimg, const double* gdist,
+                       double* grad_verts, double* grad_feats) {
+    orc_composite_bwd_tiles(c, sorted, ranges, idx, verts, feats, F, jitter, NULL, 0, img, dist, ncontrib,
+                            gimg, gdist, grad_verts, grad_feats);
 }
 
 /* ------------------------------------------------------------------ FP64 error votes ----- */
@@ -822,7 +875,41 @@ void orc_power_bwd(const double* V12, const double* P, double g_power, const dou
     double bc[3], ca[3], ab[3];
     cross3d(b, c, bc); cross3d(c, a, ca); cross3d(a, b, ab);
     double den = 2.0 * dot3d(a, bc);
-    if (fabs(den) < 1e-10) return;
+    if (fabs(den) < 1e-10) {
+        /* stable_power.slang:41-58: a very small tet (largest edge below 1e-7) uses the centroid
+         * and its smallest corner distance; otherwise the power is the constant -1e20. */
+        const double* pts[4] = {V12, V12 + 3, V12 + 6, V12 + 9};
+        double md = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) {
+                double d[3] = {pts[j][0] - pts[i][0], pts[j][1] - pts[i][1], pts[j][2] - pts[i][2]};
+                double l = len3d(d);
+                if (l > md) md = l;
+            }
+        if (md >= 1e-7) return;
+        double ctr[3];
+        for (int i = 0; i < 3; ++i) ctr[i] = 0.25 * (pts[0][i] + pts[1][i] + pts[2][i] + pts[3][i]);
+        int km = 0;
+        double r = 1e300;
+        for (int k = 0; k < 4; ++k) {
+            double d[3] = {pts[k][0] - ctr[0], pts[k][1] - ctr[1], pts[k][2] - ctr[2]};
+            double l = len3d(d);
+            if (l < r) { r = l; km = k; }
+        }
+        /* power = |ctr - P|^2 - r^2, r = |V_km - ctr| */
+        double gct[3], u[3];
+        for (int i = 0; i < 3; ++i) {
+            gct[i] = g_center[i] + 2.0 * (ctr[i] - P[i]) * g_power;
+            u[i] = r > 0 ? (pts[km][i] - ctr[i]) / r : 0.0;
+        }
+        double gr = -2.0 * r * g_power;
+        for (int i = 0; i < 3; ++i) {
+            g12[3 * km + i] += gr * u[i];
+            double gc = gct[i] - gr * u[i];
+            for (int k = 0; k < 4; ++k) g12[3 * k + i] += 0.25 * gc;
+        }
+        return;
+    }
     double num[3], rel[3], ctr[3];
     for (int i = 0; i < 3; ++i) { num[i] = aa * bc[i] + bb * ca[i] + cc * ab[i]; rel[i] = num[i] / den; ctr[i] = A[i] + rel[i]; }
     double radius = len3d(rel);

@@ -93,3 +93,60 @@ def test_render_mesh_loaded_from_ply(oracle_lib, tmp_path):
     err = np.abs(img.cpu().numpy()[..., :3] - o_img[..., :3]).max()
     print(f"PLY-loaded mesh: image err vs FP64 oracle {err:.2e}")
     assert err <= 1e-5
+
+
+class _CircumModel(_Model):
+    """Cell values that depend on the circumcenters, as the reference's field does when
+    ablate_circumsphere is off (models/ingp_color.py:320-325: the circumcenter reaches the field
+    through the query point and offset_normalize): values = features + cc @ A."""
+
+    def __init__(self, sc, A):
+        import torch
+        super().__init__(sc)
+        self.A = torch.from_numpy(A).float().cuda()
+
+    def get_cell_values(self, camera, mask=None, all_circumcenters=None):
+        import torch
+        return torch.zeros((int(mask.sum()), 0, 3), device="cuda"), self.feats[mask] + all_circumcenters @ self.A
+
+
+def test_render_circumcenter_gradient_chain(oracle_lib):
+    """render() with circumcenter-dependent cell values: the vertex gradient is the compositing
+    adjoint plus the projection adjoint of d loss / d circumcenter (VertexShader.backward);
+    checked against the FP64 oracle chain (orc_composite_bwd, then orc_power_bwd per tet)."""
+    import torch
+    from radiance_meshes_amd.render import render
+    from radiance_meshes_amd import scenes
+    O = oracle_lib
+    sc = scenes.ball_scene(n_points=400, seed=4, size=48, tile_size=4, n_features=7, dtype=np.float32)
+    A = np.random.default_rng(5).normal(0, 0.05, (3, 7))
+    cam, model = _Camera(sc.camera), _CircumModel(sc, A)
+    H = W = 48
+    Wt = np.random.default_rng(6).standard_normal((3, H, W))
+    print("render() with circumcenter-dependent values", flush=True)
+    pkg = render(cam, model, tile_size=4, min_t=0.1)
+    (pkg["render"] * torch.from_numpy(Wt).float().cuda()).sum().backward()
+    torch.cuda.synchronize()
+    ocam = O.make_camera(sc.camera, 4, 0.1)
+    t, r, vs, cc = O.project(ocam, sc.indices, sc.vertices)
+    keys, vals, ranges = O.bin_tiles(ocam, t, r, vs)
+    mask = t > 0
+    f64 = (sc.features.astype(np.float64) + cc.astype(np.float64) @ A) * mask[:, None]
+    jit = np.full((H, W, 2), 0.5, np.float32)
+    img, dist, nc, margin = O.composite_fwd(ocam, vals, ranges, sc.indices, sc.vertices, f64, jit)
+    err = np.abs(pkg["render"].detach().cpu().numpy() - img[..., :3].transpose(2, 0, 1)).max()
+    print(f"image err vs FP64 oracle {err:.3e}")
+    assert err <= 1e-5
+    gimg = np.zeros((H, W, 4))
+    gimg[..., :3] = Wt.transpose(1, 2, 0)
+    gv, gf = O.composite_bwd(ocam, vals, ranges, sc.indices, sc.vertices, f64, jit, img, dist, nc, gimg,
+                             np.zeros((H, W, 5)))
+    gcc = (gf @ A.T) * mask[:, None]
+    gv = gv + O.project_bwd(ocam, sc.indices, sc.vertices.astype(np.float64), t, np.zeros((len(t), 3)), gcc)
+    g = model.vertices.grad.cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(g - gv) / np.linalg.norm(gv)
+    gproj = np.abs(gv - O.composite_bwd(ocam, vals, ranges, sc.indices, sc.vertices, f64, jit, img, dist, nc,
+                                        gimg, np.zeros((H, W, 5)))[0]).max()
+    print(f"vertex grad rel L2 err vs FP64 chain {rel:.2e} (projection part up to {gproj:.3e})")
+    assert gproj > 0
+    assert rel <= 1e-3

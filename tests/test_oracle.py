@@ -84,6 +84,85 @@ def test_power_adjoint_matches_finite_differences(O):
         np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-5 * np.abs(fd).max())
 
 
+def test_power_adjoint_small_tet_branch(O):
+    """The very-small-tet branch (stable_power.slang:41-58: |2 a.(b x c)| < 1e-10 and largest edge
+    below 1e-7): centroid and smallest corner distance, against FP64 central differences with a
+    step far below the tet's size; a large flat tet instead has the constant power -1e20 and a
+    zero adjoint."""
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        C0 = rng.uniform(-0.5, 0.5, 3)
+        V = C0 + rng.uniform(-1, 1, (4, 3)) * 2e-8
+        P = rng.uniform(-1, 1, 3)
+        gp, gc = rng.standard_normal(), rng.standard_normal(3)
+        pw, c = O.power_of_circumsphere(V[0], V[1], V[2], V[3], P)
+        np.testing.assert_allclose(c, V.mean(0), rtol=0, atol=1e-15)
+        r = np.linalg.norm(V - V.mean(0), axis=1).min()
+        np.testing.assert_allclose(pw, ((V.mean(0) - P) ** 2).sum() - r * r, rtol=1e-12)
+
+        def f(Vx):
+            p_, c_ = O.power_of_circumsphere(Vx[0], Vx[1], Vx[2], Vx[3], P)
+            return gp * p_ + gc @ c_
+
+        g = O.power_bwd(V.reshape(12), P, gp, gc)
+        fd = np.zeros((4, 3))
+        h = 1e-9   # the power is quadratic in V on this branch: differences are exact up to rounding
+        for i in range(4):
+            for k in range(3):
+                Vp, Vm = V.copy(), V.copy()
+                Vp[i, k] += h
+                Vm[i, k] -= h
+                fd[i, k] = (f(Vp) - f(Vm)) / (2 * h)
+        np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-4 * np.abs(fd).max())
+    flat = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 1e-12]], float)
+    assert O.power_of_circumsphere(*flat, np.array([0, 0, 3.0]))[0] == -1e20
+    assert not O.power_bwd(flat.reshape(12), np.array([0, 0, 3.0]), 1.0, np.ones(3)).any()
+
+
+def test_project_bwd_matches_finite_differences(O, small):
+    """oracle.project_bwd (the adjoint of vertex_shader's vs[:,2] and circumcenter outputs w.r.t.
+    the vertices, tile_shader_slang.py:127-150) against FP64 central differences of
+    L = sum_t w_t power_t + u_t . circumcenter_t over the visible tets, on 40 vertex coordinates
+    (each shared by several tets, so the per-vertex accumulation is covered)."""
+    sc = small
+    cam = O.make_camera(sc.camera, sc.tile_size, sc.min_t)
+    tiles, rect, vs, cc = O.project(cam, sc.indices, sc.vertices)
+    rng = np.random.default_rng(22)
+    T = len(sc.indices)
+    gvs = np.zeros((T, 3))
+    gvs[:, 2] = rng.standard_normal(T)
+    gcc = rng.standard_normal((T, 3))
+    V = sc.vertices.astype(np.float64)
+    g = O.project_bwd(cam, sc.indices, V, tiles, gvs, gcc)
+    P = np.array(cam.cam_pos[:], np.float64)
+    vis = tiles > 0
+
+    def L(Vx, tets):
+        s = 0.0
+        for t in tets:
+            q = sc.indices[t]
+            pw, c = O.power_of_circumsphere(Vx[q[0]], Vx[q[1]], Vx[q[2]], Vx[q[3]], P)
+            s += gvs[t, 2] * pw + gcc[t] @ c
+        return s
+
+    used = np.unique(sc.indices[vis])
+    h = 1e-6
+    errs = []
+    for v in rng.choice(used, 40, replace=False):
+        tets = np.nonzero(vis & (sc.indices == v).any(1))[0]
+        k = int(rng.integers(0, 3))
+        Vp, Vm = V.copy(), V.copy()
+        Vp[v, k] += h
+        Vm[v, k] -= h
+        fd = (L(Vp, tets) - L(Vm, tets)) / (2 * h)
+        errs.append(abs(g[v, k] - fd) / max(abs(fd), 1e-3))
+    print(f"project_bwd vs central differences: max rel err {max(errs):.2e}")
+    assert max(errs) < 5e-5
+    # culled tets contribute nothing
+    g2 = O.project_bwd(cam, sc.indices, V, np.where(vis, 0, tiles), gvs, gcc)
+    assert not g2.any()
+
+
 # ----------------------------------------------------------------------------- projection
 
 def test_projection_invariants_and_brute_force_cover(O, small):
