@@ -17,8 +17,8 @@
 // Backward: the reference's structure (bwd_alpha_blend, alphablend_shader_interp.slang:273-369):
 // each pixel walks its list back to front from the stored final state, undoing log-transmittance
 // and the distortion moments, skipping entries at or beyond its n_contrib. Per-tet gradients
-// are accumulated as adjoints of the plane records (per plane: sum of ds/den and of
-// (ds*s/den)*d), summed across the wavefront (or 16-lane tile group) with cross-lane shuffles,
+// are accumulated as adjoints of the plane records (per plane: sum of u = ds/den and of
+// u (x - v0), x the crossing point), summed across the wavefront (or 16-lane tile group) with cross-lane shuffles,
 // and converted once per (wave, tet) to vertex and feature gradients, which lanes of the
 // wave then add with one float atomic each.
 #include "common.h"
@@ -118,13 +118,15 @@ __device__ inline float3 load_v(const float* __restrict__ v, int i) {
     return f3(p[0], p[1], p[2]);
 }
 
-struct Hit {
+template <typename T>
+struct HitT {
     bool hit;
     float se, sx;      // clamped entry / exit distance from the camera along the ray
     float dt;          // sx - se, formed in the plane precision before rounding
     float se_raw;      // entering plane distance before clamping at min_t
     float den_e, den_x;
     int ie, ix;
+    T se_p, sx_p;      // entering / exiting plane distances in the plane precision, unclamped
 };
 
 __device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -138,8 +140,8 @@ __device__ inline double rcp_(double x) {
 
 // ray_tetrahedron_intersect_fused on plane records (n_i, e_i) for a ray from cam along d.
 template <typename PL, typename T>
-__device__ inline Hit intersect(const PL pl[4], RayT<T> r, float mtf) {
-    Hit h;
+__device__ inline HitT<T> intersect(const PL pl[4], RayT<T> r, float mtf) {
+    HitT<T> h;
     const T mt = (T)mtf;
     T se = (T)-1e30, sx = (T)1e30, de = 1, dx = 1;
     int ie = -1, ix = -1;
@@ -168,6 +170,8 @@ __device__ inline Hit intersect(const PL pl[4], RayT<T> r, float mtf) {
     h.den_x = (float)dx;
     h.ie = ie;
     h.ix = ix;
+    h.se_p = se;
+    h.sx_p = sx;
     return h;
 }
 
@@ -238,6 +242,7 @@ struct Stage {
     float4 fg[(NG_<F>::NGRD + 3) / 4][WG];
     float4 vtx[VTX ? 3 : 1][VTX ? WG : 1];  // v0..v3 packed (backward epilogue)
     int meta[BWD ? (VTX ? 2 : 1) : 1][BWD ? WG : 1];  // tet id, face sign bits
+    typename Prec<P64>::R cv[BWD ? 3 : 1][BWD ? WG : 1];  // cam - v0 in the plane precision (backward)
 };
 
 template <typename T>
@@ -302,7 +307,12 @@ __device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int t
         S.fg[k][slot] = make_float4(g[4 * k], 4 * k + 1 < NGRD ? g[(4 * k + 1) % NGRD] : 0.f,
                                     4 * k + 2 < NGRD ? g[(4 * k + 2) % NGRD] : 0.f,
                                     4 * k + 3 < NGRD ? g[(4 * k + 3) % NGRD] : 0.f);
-    if constexpr (BWD) S.meta[0][slot] = tet;
+    if constexpr (BWD) {
+        S.meta[0][slot] = tet;
+        S.cv[0][slot] = (T)cp.x - (T)v0.x;
+        S.cv[1][slot] = (T)cp.y - (T)v0.y;
+        S.cv[2][slot] = (T)cp.z - (T)v0.z;
+    }
     if constexpr (VTX) {
         S.vtx[0][slot] = make_float4(v0.x, v0.y, v0.z, v1.x);
         S.vtx[1][slot] = make_float4(v1.y, v1.z, v2.x, v2.y);
@@ -328,8 +338,8 @@ __device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot
 
 // Blend of one hit into the pixel state (integrate_tetra_segment and the distortion moments);
 // returns true when the pixel terminates (log-transmittance below -5.5) after this tet.
-template <int WG, int F, typename St>
-__device__ __forceinline__ bool blend_step(const St& S, int slot, const Hit& h, float rdx, float rdy, float rdz,
+template <int WG, int F, typename St, typename H>
+__device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, float rdx, float rdy, float rdz,
                                            float mt, float& cr, float& cg, float& cb, float& logT, float& X,
                                            float& Y, float& Z, float& W, float& V) {
     constexpr int NGRD = NG_<F>::NGRD;
@@ -421,8 +431,8 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
                 const int s0 = base + jj, s1 = two ? s0 + 1 : s0;
                 const PL p0[4] = {S.pl[0][s0], S.pl[1][s0], S.pl[2][s0], S.pl[3][s0]};
                 const PL p1[4] = {S.pl[0][s1], S.pl[1][s1], S.pl[2][s1], S.pl[3][s1]};
-                const Hit h0 = intersect(p0, ray, mt);
-                const Hit h1 = intersect(p1, ray, mt);
+                const auto h0 = intersect(p0, ray, mt);
+                const auto h1 = intersect(p1, ray, mt);
                 ++nc;
                 if (blend_step<C::WG, F>(S, s0, h0, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V)) {
                     alive = false;
@@ -510,8 +520,10 @@ __device__ inline void group_sum(float (&v)[N], int width) {
 }
 
 // Per-tet gradients from the reduced per-slot sums acc = [dsig, dbase[3], Sq[NCH], Rg[3*NCH],
-// u[4], r[12]]: outputs 0..11 are the four vertices' gradients, 12.. the feature gradients.
-// Colour gradients enter through the base colour folded at v0 (cv0 = cam - v0); each face's
+// u[4], q[12]]: Sq / Rg are the sums of the colour adjoints at the entry and exit points and of
+// those adjoints times the points' offsets from v0; per plane u is the sum of ds/den and q the
+// sum of u (x - v0) over its crossing points x. Outputs 0..11 are the four vertices' gradients,
+// 12.. the feature gradients. Colour gradients enter through the base colour at v0; each face's
 // plane record n = +-cross(b - a, c - a), e = n . (cam - a) passes its adjoint to a, b, c.
 // The face normals are recomputed from the vertices in the plane precision (the same cross
 // product plane_rec formed, flipped by the recorded sign), so no plane record is read here.
@@ -524,7 +536,6 @@ __device__ inline void tet_grads(const float3 (&V4)[4], int signs, const float (
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;
     float3 gv[4] = {f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0), f3(0, 0, 0)};
-    const float3 cv0 = cp - V4[0];
     outv[12] = acc[0];
     outv[13] = acc[1]; outv[14] = acc[2]; outv[15] = acc[3];
 #pragma unroll
@@ -532,9 +543,9 @@ __device__ inline void tet_grads(const float3 (&V4)[4], int signs, const float (
         float Sq = acc[4 + ch];
         float3 grow = f3(gr[3 * ch], gr[(3 * ch + 1) % NGRD], gr[(3 * ch + 2) % NGRD]);
         gv[0] = gv[0] - Sq * grow;
-        outv[16 + 3 * ch] = Sq * cv0.x + acc[4 + NCH + 3 * ch];
-        outv[17 + 3 * ch] = Sq * cv0.y + acc[5 + NCH + 3 * ch];
-        outv[18 + 3 * ch] = Sq * cv0.z + acc[6 + NCH + 3 * ch];
+        outv[16 + 3 * ch] = acc[4 + NCH + 3 * ch];
+        outv[17 + 3 * ch] = acc[5 + NCH + 3 * ch];
+        outv[18 + 3 * ch] = acc[6 + NCH + 3 * ch];
     }
     const int PU = 4 + NCH + 3 * NCH;
     const int fa_[4] = {0, 0, 0, 1}, fb_[4] = {1, 1, 2, 2}, fc_[4] = {2, 3, 3, 3};
@@ -548,7 +559,8 @@ __device__ inline void tet_grads(const float3 (&V4)[4], int signs, const float (
         const V3<T> vc = {(T)V4[cc].x, (T)V4[cc].y, (T)V4[cc].z};
         const V3<T> n = v3cross(v3sub(vb, va), v3sub(vc, va));
         const float3 nn = sgn * f3((float)n.x, (float)n.y, (float)n.z);
-        const float3 gn = (-u) * (cp - V4[a]) - rr;
+        // d/dn of -(e / den) summed over pixels: -sum u (x - v_a) = -(sum u (x - v0) + u (v0 - v_a))
+        const float3 gn = a == 0 ? f3(-rr.x, -rr.y, -rr.z) : (-u) * (V4[0] - V4[a]) - rr;
         gv[a] = gv[a] + u * nn;
         const float3 gcr = sgn * gn;
         const float3 e1 = V4[b] - V4[a], e2 = V4[cc] - V4[a];
@@ -631,7 +643,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     using C = Cfg<TS>;
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;        // independent colour-gradient rows
-    // reduced per-tet values: dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], r[12]
+    // reduced per-tet values: dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], q[12]
     constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;
     constexpr int NOUT = 12 + F;                // vertex + feature gradients per tet
     using T = typename Prec<P64>::R;
@@ -699,7 +711,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
         cnt = cnt < C::GROUP ? cnt : C::GROUP;
         const int base = g * C::GROUP;
         // slot body for slot jj with its intersection h (computed ahead, see below)
-        auto slot_body = [&](int jj, const Hit& h) {
+        auto slot_body = [&](int jj, const HitT<typename Prec<P64>::R>& h) {
+            using TP = typename Prec<P64>::R;
             const int slot = base + jj;
             const int kk = c * C::GROUP + jj;
             float acc[NV];
@@ -757,6 +770,15 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     float dsig = dm0 + dm1 * (te + tx_) * 0.5f + dV * 2.0f * sig * dt * (1.0f / 3.0f);
                     float ddt = dV * sig * sig * (1.0f / 3.0f);
                     float dse = dm1 * sig * 0.5f, dsx = dm1 * sig * 0.5f;
+                    // entry / exit points relative to v0, formed in the plane precision from
+                    // cam - v0 and s d (each ~ the camera distance, far larger than the tet), so
+                    // the sums below carry no cancellation between the two
+                    const TP c0 = S.cv[0][slot], c1 = S.cv[1][slot], c2 = S.cv[2][slot];
+                    const TP sec = h.se_p > (TP)mt ? h.se_p : (TP)mt;
+                    const float qex = (float)(c0 + sec * ray.dx), qey = (float)(c1 + sec * ray.dy),
+                                qez = (float)(c2 + sec * ray.dz);
+                    const float qxx = (float)(c0 + h.sx_p * ray.dx), qxy = (float)(c1 + h.sx_p * ray.dy),
+                                qxz = (float)(c2 + h.sx_p * ray.dz);
                     float dw0 = 0.f, dw1 = 0.f;
                     float dqe[3], dqx[3];
 #pragma unroll
@@ -776,35 +798,35 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     if (F == 7) {
                         float qe = dqe[0] + dqe[1] + dqe[2], qx = dqx[0] + dqx[1] + dqx[2];
                         acc[4] = qe + qx;
-                        float rs = qe * h.se + qx * h.sx;
-                        acc[5] = rs * rdx; acc[6] = rs * rdy; acc[7] = rs * rdz;
+                        acc[5] = qe * qex + qx * qxx;
+                        acc[6] = qe * qey + qx * qxy;
+                        acc[7] = qe * qez + qx * qxz;
                         dse += qe * gd[0];
                         dsx += qx * gd[0];
                     } else {
 #pragma unroll
                         for (int ch = 0; ch < 3; ++ch) {
                             acc[4 + ch] = dqe[ch] + dqx[ch];
-                            float rs = dqe[ch] * h.se + dqx[ch] * h.sx;
-                            acc[7 + 3 * ch] = rs * rdx;
-                            acc[8 + 3 * ch] = rs * rdy;
-                            acc[9 + 3 * ch] = rs * rdz;
+                            acc[7 + 3 * ch] = dqe[ch] * qex + dqx[ch] * qxx;
+                            acc[8 + 3 * ch] = dqe[ch] * qey + dqx[ch] * qxy;
+                            acc[9 + 3 * ch] = dqe[ch] * qez + dqx[ch] * qxz;
                             dse += dqe[ch] * gd[ch];
                             dsx += dqx[ch] * gd[ch];
                         }
                     }
-                    // plane adjoints: s_i = -e_i / den_i
+                    // plane adjoints: s_i = -e_i / den_i; per plane the sums of u = ds / den and of
+                    // u (x - v0) over its crossing points x (the entering plane only counts when
+                    // its crossing lies beyond min_t, where the clamped entry is the crossing)
                     const int PU = 4 + NCH + 3 * NCH;
                     float ue = (h.se_raw > mt) ? dse * rcp_(h.den_e) : 0.f;
                     float ux = dsx * rcp_(h.den_x);
-                    float re = ue * h.se_raw, rx = ux * h.sx;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        float u = (h.ie == i ? ue : 0.f) + (h.ix == i ? ux : 0.f);
-                        float rr = (h.ie == i ? re : 0.f) + (h.ix == i ? rx : 0.f);
-                        acc[PU + i] = u;
-                        acc[PU + 4 + 3 * i] = rr * rdx;
-                        acc[PU + 5 + 3 * i] = rr * rdy;
-                        acc[PU + 6 + 3 * i] = rr * rdz;
+                        const bool en = h.ie == i, ex = h.ix == i;
+                        acc[PU + i] = (en ? ue : 0.f) + (ex ? ux : 0.f);
+                        acc[PU + 4 + 3 * i] = (en ? ue * qex : 0.f) + (ex ? ux * qxx : 0.f);
+                        acc[PU + 5 + 3 * i] = (en ? ue * qey : 0.f) + (ex ? ux * qxy : 0.f);
+                        acc[PU + 6 + 3 * i] = (en ? ue * qez : 0.f) + (ex ? ux * qxz : 0.f);
                     }
                     // step the pixel state back over this tet
                     A = Ap; Z = zp; W = wp;
@@ -883,8 +905,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
             const int s0 = base + jj, s1 = jj > 0 ? s0 - 1 : s0;
             const PL p0[4] = {S.pl[0][s0], S.pl[1][s0], S.pl[2][s0], S.pl[3][s0]};
             const PL p1[4] = {S.pl[0][s1], S.pl[1][s1], S.pl[2][s1], S.pl[3][s1]};
-            const Hit h0 = intersect(p0, ray, mt);
-            const Hit h1 = intersect(p1, ray, mt);
+            const auto h0 = intersect(p0, ray, mt);
+            const auto h1 = intersect(p1, ray, mt);
             slot_body(jj, h0);
             if (jj > 0) slot_body(jj - 1, h1);
         }
@@ -1141,7 +1163,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
             int hit = 0;
             if (alive) {
                 const PL p[4] = {S.pl[0][s], S.pl[1][s], S.pl[2][s], S.pl[3][s]};
-                const Hit h = intersect(p, ray, cam.min_t);
+                const auto h = intersect(p, ray, cam.min_t);
                 const float x = S.fa[s].x * h.dt;
                 if (h.hit && x > 0.f) {
                     w = __expf(logT) * x;   // hardware exponential: ~4e-7 relative for logT in [-5.5, 1]
