@@ -162,6 +162,21 @@ int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int
                         float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
                         size_t workspace_bytes, rm_stream_t stream);
 
+/* No counterpart in the reference: rm_composite_bwd_ws without a workspace argument, for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). The per-tet rows live in a static device array of
+ * rm_composite_bwd_noalloc_max_tets() rows (524,288 tets, 64 MiB), one copy per GPU held from
+ * library load until the process ends; n_tets above that returns RM_ERR_CAPACITY. No allocation
+ * inside. All calls on one GPU share its array: calls that overlap on one GPU (different streams
+ * or host threads) race and must be serialised by the caller. Same gradients as rm_composite_bwd_ws. */
+int64_t rm_composite_bwd_noalloc_max_tets(void);
+int rm_composite_bwd_noalloc(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                             const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                             const float* vertices_dev, const float* features_dev, int32_t n_features,
+                             const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                             const int32_t* ncontrib_dev, const float* grad_img_dev, const float* grad_dist_dev,
+                             float* grad_vertices_dev, float* grad_features_dev, rm_stream_t stream);
+
 /* ---- per-tet feature activation of the reference's field model ------------------------
  * Replaces activate_output (utils/model_util.py:93-105) with offset_normalize (:81-91),
  * compute_vertex_colors_from_field (:62-78), eval_sh (utils/eval_sh_py.py:137-195) and safe_div

@@ -65,6 +65,9 @@ SIGNATURES = {
     "rm_composite_bwd_workspace_size": (_SZ, [_I64]),
     "rm_composite_bwd_ws": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _I64, _P, _P, _I32, _P, _P, _P, _P,
                                       _P, _P, _P, _P, _P, _SZ, _P]),
+    "rm_composite_bwd_noalloc_max_tets": (_I64, []),
+    "rm_composite_bwd_noalloc": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _I64, _P, _P, _I32, _P, _P, _P,
+                                           _P, _P, _P, _P, _P, _P]),
     "rm_activate_fwd": (C.c_int, [_P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _P, _P]),
     "rm_activate_bwd": (C.c_int, [_P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "rm_tet_err": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _I64, _P]),

@@ -21,6 +21,9 @@
 // u (x - v0), x the crossing point), summed across the wavefront (or 16-lane tile group) with cross-lane shuffles,
 // and converted once per (wave, tet) to vertex and feature gradients, which lanes of the
 // wave then add with one float atomic each.
+#include <mutex>
+#include <string>
+
 #include "common.h"
 
 namespace rmesh {
@@ -1405,6 +1408,69 @@ extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_t
 #undef RM_BWD
     RM_CHECK_LAUNCH("composite_bwd_kernel");
     return RM_OK;
+}
+
+// ---- no-allocation variant of rm_composite_bwd_ws -------------------------------------------
+// The per-tet rows live in a file-scope device array instead of a caller workspace, for callers
+// that cannot allocate (code under graph capture, serving loops). Each GPU holds its own copy of
+// the array from module load until the process ends; its device address is looked up once per
+// device with hipGetSymbolAddress and only that address is handed to kernels (the array's name in
+// host code is a host-side stand-in, not a device pointer).
+constexpr int64_t kNoAllocMaxTets = int64_t(1) << 19;   // 524,288 tets: 64 MiB of rows per GPU
+static __device__ __align__(256) float scratch[kNoAllocMaxTets * kRowFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(float** out, const char* who) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error(std::string(who) + ": no current HIP device (" + hipGetErrorString(e) + ")");
+        return RM_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error(std::string(who) + ": hipGetSymbolAddress(scratch) failed: " + hipGetErrorString(e));
+            return RM_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return RM_OK;
+}
+}  // namespace
+
+extern "C" int64_t rm_composite_bwd_noalloc_max_tets(void) { return kNoAllocMaxTets; }
+
+extern "C" int rm_composite_bwd_noalloc(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                        const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                                        const float* vertices_dev, const float* features_dev, int32_t n_features,
+                                        const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                                        const int32_t* ncontrib_dev, const float* grad_img_dev,
+                                        const float* grad_dist_dev, float* grad_vertices_dev,
+                                        float* grad_features_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(n_tets >= 0, "rm_composite_bwd_noalloc: negative n_tets");
+    if (n_tets > kNoAllocMaxTets) {
+        set_error("rm_composite_bwd_noalloc: n_tets " + std::to_string(n_tets) + " exceeds the static scratch (" +
+                  std::to_string(kNoAllocMaxTets) + " tets); use rm_composite_bwd_ws");
+        return RM_ERR_CAPACITY;
+    }
+    float* rows = nullptr;
+    if (n_tets > 0) {
+        int e = scratch_of_current_device(&rows, "rm_composite_bwd_noalloc");
+        if (e) return e;
+    }
+    // same kernels as rm_composite_bwd_ws; the rows are cleared by a fill kernel before use
+    return rm_composite_bwd_ws(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, n_tets, vertices_dev,
+                               features_dev, n_features, jitter_dev, img_dev, dist_dev, ncontrib_dev, grad_img_dev,
+                               grad_dist_dev, grad_vertices_dev, grad_features_dev, rows,
+                               rm_composite_bwd_workspace_size(n_tets), stream);
 }
 
 extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
