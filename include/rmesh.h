@@ -127,6 +127,18 @@ int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_
                      float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
                      rm_stream_t stream);
 
+/* rm_composite_fwd that also records, for the backward, which list entries matter: for every entry
+ * k (an index into sorted_tet_dev) that the pixels of its tile walked, pair_flags_dev[k] = 1 when some
+ * pixel of the tile blended that tet (a hit with sigma * dt > 0) and 0 otherwise; entries no pixel
+ * reached are left unwritten. pair_flags_dev holds n_sorted bytes. Square tiles 4 and 8 (a tile
+ * within one wavefront); with tile 16 nothing is written. Same image outputs as rm_composite_fwd.
+ * No counterpart in the reference (its backward revisits every entry). */
+int rm_composite_fwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                           const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
+                           const float* features_dev, int32_t n_features, const float* jitter_dev,
+                           float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
+                           uint8_t* pair_flags_dev, rm_stream_t stream);
+
 /* Precision of the per-pixel ray/plane intersection in rm_composite_fwd/bwd (process-wide):
  * 1 (default) = FP64 plane records and crossing distances, rounded to FP32 once the segment
  * [entry, exit] is known; 0 = FP32 throughout (faster; rays that graze a face then carry the FP32
@@ -161,6 +173,19 @@ int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int
                         const int32_t* ncontrib_dev, const float* grad_img_dev, const float* grad_dist_dev,
                         float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
                         size_t workspace_bytes, rm_stream_t stream);
+
+/* rm_composite_bwd_ws (the same backward, alphablend_shader_interp.slang:273-369) that walks only the
+ * list entries rm_composite_fwd_flags flagged for the same forward: an entry no pixel of its tile
+ * blended contributes no gradient, so staging, intersecting and reducing it is skipped. Same
+ * gradients as rm_composite_bwd_ws up to the order of float additions. With tile 16 the flags are
+ * not read and every entry is walked. */
+int rm_composite_bwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                           const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                           const float* vertices_dev, const float* features_dev, int32_t n_features,
+                           const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                           const int32_t* ncontrib_dev, const uint8_t* pair_flags_dev, const float* grad_img_dev,
+                           const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
+                           void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
 
 /* No counterpart in the reference: rm_composite_bwd_ws without a workspace argument, for callers
  * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that

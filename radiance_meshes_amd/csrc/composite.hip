@@ -75,6 +75,8 @@ __device__ inline float3 cross3(float3 a, float3 b) {
 // camera.slang:241-296 undistort (Newton, 10 iterations) and 298-346 get_ray, evaluated in T.
 template <typename T>
 __device__ inline RayT<T> make_ray(const CamArgs& c, int ix, int iy, float jx, float jy) {
+    // no implicit FMA contraction: every kernel that casts this pixel's ray gets the same bits
+#pragma clang fp contract(off)
     const T px = (T)ix + (T)jx, py = (T)iy + (T)jy;
     T x = (px - (T)c.K[2]) / (T)c.K[0];
     T y = (py - (T)c.K[5]) / (T)c.K[4];
@@ -144,6 +146,9 @@ __device__ inline double rcp_(double x) {
 // ray_tetrahedron_intersect_fused on plane records (n_i, e_i) for a ray from cam along d.
 template <typename PL, typename T>
 __device__ inline HitT<T> intersect(const PL pl[4], RayT<T> r, float mtf) {
+    // explicit FMAs and no implicit contraction: the forward, the backward and the error votes
+    // reach bit-identical hit decisions for the same ray and plane records
+#pragma clang fp contract(off)
     HitT<T> h;
     const T mt = (T)mtf;
     T se = (T)-1e30, sx = (T)1e30, de = 1, dx = 1;
@@ -153,9 +158,9 @@ __device__ inline HitT<T> intersect(const PL pl[4], RayT<T> r, float mtf) {
     // others are entering (den < 0) or exiting candidates (s of a parallel plane is not used)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const T den = pl[i].x * r.dx + pl[i].y * r.dy + pl[i].z * r.dz;
+        const T den = fma(pl[i].z, r.dz, fma(pl[i].y, r.dy, pl[i].x * r.dx));
         const bool par = fabs(den) < (T)1e-10;
-        miss = miss || (par && pl[i].w + mt * den > (T)0);
+        miss = miss || (par && fma(mt, den, pl[i].w) > (T)0);
         const T s = -pl[i].w * rcp_(den);
         const bool ent = !par && den < (T)0 && s > se;
         const bool ext = !par && !(den < (T)0) && s < sx;
@@ -202,6 +207,14 @@ __device__ inline float dphi_of(float x, float em1, float alpha) {
 // quotient through the hardware reciprocal (1 ulp).
 __device__ inline float phi_of_rcp(float x, float em1) { return x < 1e-6f ? 1.0f - 0.5f * x : -em1 * rcp_(x); }
 
+// DPP lane permute of a 64-bit bit mask within a row of 16 lanes.
+template <int CTRL>
+__device__ inline uint64_t bits_dpp(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
 // XCD-aware workgroup order: consecutive dispatch slots go round-robin over the 8 XCDs, so
 // slot b is remapped to make each XCD own a contiguous band of tiles (shared tets stay in its L2).
 __device__ inline int remap_block(int b, int nb) {
@@ -245,6 +258,7 @@ struct Stage {
     float4 fg[(NG_<F>::NGRD + 3) / 4][WG];
     float4 vtx[VTX ? 3 : 1][VTX ? WG : 1];  // v0..v3 packed (backward epilogue)
     int meta[BWD ? (VTX ? 2 : 1) : 1][BWD ? WG : 1];  // tet id, face sign bits
+    int kidx[BWD ? WG : 1];            // position of the staged entry in its tile's list (backward)
     typename Prec<P64>::R cv[BWD ? 3 : 1][BWD ? WG : 1];  // cam - v0 in the plane precision (backward)
 };
 
@@ -265,6 +279,7 @@ __device__ inline V3<T> v3cross(V3<T> a, V3<T> b) {
 // n = +-cross(b - a, c - a) flipped away from o (intersect.slang:455-479), e = n . (cam - a).
 template <typename T, typename PL>
 __device__ inline int plane_rec(float3 af, float3 bf, float3 cf, float3 of, float3 camf, PL& out) {
+#pragma clang fp contract(off)
     V3<T> a = {(T)af.x, (T)af.y, (T)af.z}, b = {(T)bf.x, (T)bf.y, (T)bf.z};
     V3<T> c = {(T)cf.x, (T)cf.y, (T)cf.z}, o = {(T)of.x, (T)of.y, (T)of.z};
     V3<T> cam = {(T)camf.x, (T)camf.y, (T)camf.z};
@@ -344,12 +359,13 @@ __device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot
 template <int WG, int F, typename St, typename H>
 __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, float rdx, float rdy, float rdz,
                                            float mt, float& cr, float& cg, float& cb, float& logT, float& X,
-                                           float& Y, float& Z, float& W, float& V) {
+                                           float& Y, float& Z, float& W, float& V, bool& blended) {
     constexpr int NGRD = NG_<F>::NGRD;
     const float4 fa = S.fa[slot];
     const float dt = h.dt;
     const float x = fa.x * dt;
-    if (!(h.hit && x > 0.f)) return false;
+    blended = h.hit && x > 0.f;
+    if (!blended) return false;
     float gr[NGRD];
     read_grads(S, slot, gr);
     const float em1 = expm1_neg(x);
@@ -389,7 +405,8 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
                                           const int4* __restrict__ indices, const float* __restrict__ verts,
                                           const float* __restrict__ feats, const float2* __restrict__ jitter,
                                           float4* __restrict__ out_img, float* __restrict__ out_dist,
-                                          int* __restrict__ out_nc, int64_t pix_base) {
+                                          int* __restrict__ out_nc, int64_t pix_base,
+                                          uint8_t* __restrict__ flags = nullptr) {
     using C = Cfg<TS>;
     constexpr int NGRD = NG_<F>::NGRD;
     using T = typename Prec<P64>::R;
@@ -419,6 +436,7 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
     const int nchunks = s_chunks;
     for (int c = 0; c < nchunks; ++c) {
         if (!__syncthreads_or(alive)) break;
+        uint64_t blend_bits = 0;
         int k = c * C::GROUP + r;
         if (k < n)
             stage_tet(S, threadIdx.x, (int)(sorted[rg.x + k] - tet_base), cam, indices, verts, feats);
@@ -427,6 +445,8 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
             int cnt = n - c * C::GROUP;
             cnt = cnt < C::GROUP ? cnt : C::GROUP;
             const int base = g * C::GROUP;
+            // this lane's blends of the chunk's entries, one bit per entry (see the flags below)
+            auto record = [&](int jj, bool bl) { blend_bits |= (uint64_t)bl << jj; };
             // two slots per step: both intersections (independent of the pixel state) first,
             // then the two blends in list order
             for (int jj = 0; jj < cnt; jj += 2) {
@@ -437,16 +457,39 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
                 const auto h0 = intersect(p0, ray, mt);
                 const auto h1 = intersect(p1, ray, mt);
                 ++nc;
-                if (blend_step<C::WG, F>(S, s0, h0, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V)) {
+                bool bl;
+                const bool t0 = blend_step<C::WG, F>(S, s0, h0, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V, bl);
+                record(jj, bl);
+                if (t0) {
                     alive = false;
                     break;
                 }
                 if (!two) break;
                 ++nc;
-                if (blend_step<C::WG, F>(S, s1, h1, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V)) {
+                const bool t1 = blend_step<C::WG, F>(S, s1, h1, rdx, rdy, rdz, mt, cr, cg, cb, logT, X, Y, Z, W, V, bl);
+                record(jj + 1, bl);
+                if (t1) {
                     alive = false;
                     break;
                 }
+            }
+        }
+        if constexpr (C::GROUP <= 64) {
+            if (flags) {
+                // entry r of the chunk: 1 when some pixel of the tile blended it; written by lane r of
+                // the tile group for every entry of the chunk (entries past every pixel's last one are
+                // never read back)
+                uint64_t any = blend_bits;
+                any |= bits_dpp<0xB1>(any);      // quad_perm [1,0,3,2]
+                any |= bits_dpp<0x4E>(any);      // quad_perm [2,3,0,1]
+                any |= bits_dpp<0x124>(any);     // row_ror:4
+                any |= bits_dpp<0x128>(any);     // row_ror:8
+                if (C::GROUP == 64) {
+                    any |= __shfl_xor(any, 16, 64);
+                    any |= __shfl_xor(any, 32, 64);
+                }
+                const int cntc = n - c * C::GROUP;
+                if (r < cntc) flags[rg.x + c * C::GROUP + r] = (uint8_t)((any >> r) & 1);
             }
         }
     }
@@ -464,11 +507,11 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_fwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
-    int* __restrict__ out_nc) {
+    int* __restrict__ out_nc, uint8_t* __restrict__ flags) {
     __shared__ Stage<Cfg<TS>::WG, F, false, P64> S;
     __shared__ int s_chunks;
     fwd_tiles<TS, F, P64>(S, s_chunks, cam, remap_block(blockIdx.x, gridDim.x), sorted, n_sorted, ranges, 0, 0,
-                          indices, verts, feats, jitter, out_img, out_dist, out_nc, 0);
+                          indices, verts, feats, jitter, out_img, out_dist, out_nc, 0, flags);
 }
 
 // Batched forward: the workgroups of all views of a plan in one launch (pixel-centre rays).
@@ -636,14 +679,18 @@ __device__ inline void group16_transpose_sum(const float (&a)[32], int r, float&
 // ROWS: instead of converting each (tile, tet) sum to vertex and feature gradients, add the sums
 // themselves to the tet's row of `rows` (kRowFloats floats per tet); composite_tet_grads_kernel
 // converts the rows once per tet. Otherwise gradients go straight to gverts / gfeats.
-template <int TS, int F, bool P64, bool ROWS>
+// FLAGS (tile groups within one wave): only list entries whose forward blend flag is set are
+// staged and walked; a chunk's flagged entries are compacted in list order.
+template <int TS, int F, bool P64, bool ROWS, bool FLAGS = false>
 __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
     const int* __restrict__ ncontrib, const float4* __restrict__ gimg, const float* __restrict__ gdist,
-    float* __restrict__ gverts, float* __restrict__ gfeats, float* __restrict__ rows) {
+    float* __restrict__ gverts, float* __restrict__ gfeats, float* __restrict__ rows,
+    const uint8_t* __restrict__ flags) {
     using C = Cfg<TS>;
+    static_assert(!FLAGS || C::GROUP <= 64, "blend flags need the tile group within one wave");
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;        // independent colour-gradient rows
     // reduced per-tet values: dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], q[12]
@@ -706,18 +753,33 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     for (int c = nchunks_wg - 1; c >= 0; --c) {
         __syncthreads();
         int k = c * C::GROUP + r;
-        if (c < nchunks_g && k < nmax_g)
-            stage_tet(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
-        __syncthreads();
-        // tile groups past their last chunk run no slots (cnt = 0) but still reach the barriers below
-        int cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
-        cnt = cnt < C::GROUP ? cnt : C::GROUP;
+        const bool valid = c < nchunks_g && k < nmax_g;
         const int base = g * C::GROUP;
+        int cnt;
+        if constexpr (FLAGS) {
+            // compact the chunk to the entries some pixel of the tile blended in the forward
+            const bool f = valid && flags[rg.x + k] != 0;
+            const int gb = lane & ~(C::GROUP - 1);
+            const uint64_t gm = (C::GROUP == 64 ? ~0ull : ((1ull << C::GROUP) - 1ull)) << gb;
+            const uint64_t m = __ballot(f) & gm;
+            if (f) {
+                const int ci = __popcll(m & ((1ull << lane) - 1ull));
+                stage_tet(S, base + ci, sorted[rg.x + k], cam, indices, verts, feats);
+                S.kidx[base + ci] = k;
+            }
+            cnt = __popcll(m);
+        } else {
+            if (valid) stage_tet(S, threadIdx.x, sorted[rg.x + k], cam, indices, verts, feats);
+            // tile groups past their last chunk run no slots (cnt = 0) but still reach the barriers below
+            cnt = c < nchunks_g ? nmax_g - c * C::GROUP : 0;
+            cnt = cnt < C::GROUP ? cnt : C::GROUP;
+        }
+        __syncthreads();
         // slot body for slot jj with its intersection h (computed ahead, see below)
         auto slot_body = [&](int jj, const HitT<typename Prec<P64>::R>& h) {
             using TP = typename Prec<P64>::R;
             const int slot = base + jj;
-            const int kk = c * C::GROUP + jj;
+            const int kk = FLAGS ? S.kidx[slot] : c * C::GROUP + jj;
             float acc[NV];
 #pragma unroll
             for (int i = 0; i < NV; ++i) acc[i] = 0.f;
@@ -957,12 +1019,13 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
 
 template <int TS, int F, bool P64>
 int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
-               const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st) {
+               const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st,
+               uint8_t* flags = nullptr) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
     hipLaunchKernelGGL((composite_fwd_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
                        n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
-                       (float4*)img, dist, nc);
+                       (float4*)img, dist, nc, flags);
     return 0;
 }
 
@@ -1066,16 +1129,31 @@ template <int TS, int F, bool P64, bool ROWS>
 int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, const float* img, const float* dist, const int* nc,
                const float* gimg, const float* gdist, float* gverts, float* gfeats, float* rows, int64_t n_tets,
-               hipStream_t st) {
+               hipStream_t st, const uint8_t* flags = nullptr) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
     if (ROWS && n_tets > 0) {
         const int64_t n4 = n_tets * (kRowFloats / 4);
         hipLaunchKernelGGL(zero_f4_kernel, dim3((unsigned)ceil_div(n4, 256)), dim3(256), 0, st, (float4*)rows, n4);
     }
-    hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
-                       n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
-                       (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats, rows);
+    if constexpr (ROWS && C::GROUP <= 64) {
+        if (flags) {
+            hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS, true>), dim3(sgw * sgh), dim3(C::WG), 0, st,
+                               a, sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
+                               (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist,
+                               gverts, gfeats, rows, flags);
+        } else {
+            hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS>), dim3(sgw * sgh), dim3(C::WG), 0, st, a,
+                               sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
+                               (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist,
+                               gverts, gfeats, rows, (const uint8_t*)nullptr);
+        }
+    } else {
+        hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
+                           n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, (const float2*)jitter,
+                           (const float4*)img, dist, nc, (const float4*)gimg, gdist, gverts, gfeats, rows,
+                           (const uint8_t*)nullptr);
+    }
     if (ROWS && n_tets > 0)
         hipLaunchKernelGGL((composite_tet_grads_kernel<F, P64>), dim3((unsigned)ceil_div(n_tets, kTetGradTets)),
                            dim3(kTetGradTets), 0,
@@ -1299,23 +1377,26 @@ extern "C" int rm_set_intersect_precision(int fp64) {
     return RM_OK;
 }
 
-extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
-                                const int32_t* ranges_dev,
-                                const int32_t* indices_dev, const float* vertices_dev, const float* features_dev,
-                                int32_t n_features, const float* jitter_dev, float* out_img_dev,
-                                float* out_dist_dev, int32_t* out_ncontrib_dev, rm_stream_t stream) {
-    RM_CHECK_ARG(n_sorted >= 0, "rm_composite_fwd: negative n_sorted");
+namespace {
+int composite_fwd_entry(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                        const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
+                        const float* features_dev, int32_t n_features, const float* jitter_dev, float* out_img_dev,
+                        float* out_dist_dev, int32_t* out_ncontrib_dev, uint8_t* flags_dev, rm_stream_t stream,
+                        const char* who) {
+    const std::string w(who);
+    RM_CHECK_ARG(n_sorted >= 0, w + ": negative n_sorted");
     if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
     int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
-                         jitter_dev, "rm_composite_fwd");
+                         jitter_dev, who);
     if (e) return e;
-    RM_CHECK_ARG(out_img_dev && out_dist_dev && out_ncontrib_dev, "rm_composite_fwd: null output");
-    RM_CHECK_ARG(((uintptr_t)out_img_dev & 15) == 0, "rm_composite_fwd: out_img must be 16-byte aligned");
+    RM_CHECK_ARG(out_img_dev && out_dist_dev && out_ncontrib_dev, w + ": null output");
+    RM_CHECK_ARG(((uintptr_t)out_img_dev & 15) == 0, w + ": out_img must be 16-byte aligned");
     CamArgs a = make_cam_args(*cam);
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
 #define RM_FWD(TS, F, P) launch_fwd<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
-                                        features_dev, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st)
+                                        features_dev, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st, \
+                                        flags_dev)
 #define RM_FWD_P(TS, F) (g_fp64_planes ? RM_FWD(TS, F, true) : RM_FWD(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_FWD_P(4, 7); else if (ts == 8) RM_FWD_P(8, 7); else RM_FWD_P(16, 7);
@@ -1326,6 +1407,28 @@ extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_
 #undef RM_FWD
     RM_CHECK_LAUNCH("composite_fwd_kernel");
     return RM_OK;
+}
+}  // namespace
+
+extern "C" int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                const int32_t* ranges_dev,
+                                const int32_t* indices_dev, const float* vertices_dev, const float* features_dev,
+                                int32_t n_features, const float* jitter_dev, float* out_img_dev,
+                                float* out_dist_dev, int32_t* out_ncontrib_dev, rm_stream_t stream) {
+    return composite_fwd_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev,
+                               n_features, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, nullptr, stream,
+                               "rm_composite_fwd");
+}
+
+extern "C" int rm_composite_fwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                      const int32_t* ranges_dev, const int32_t* indices_dev,
+                                      const float* vertices_dev, const float* features_dev, int32_t n_features,
+                                      const float* jitter_dev, float* out_img_dev, float* out_dist_dev,
+                                      int32_t* out_ncontrib_dev, uint8_t* pair_flags_dev, rm_stream_t stream) {
+    RM_CHECK_ARG(pair_flags_dev != nullptr || n_sorted == 0, "rm_composite_fwd_flags: null pair_flags");
+    return composite_fwd_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev,
+                               n_features, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, pair_flags_dev,
+                               stream, "rm_composite_fwd_flags");
 }
 
 extern "C" int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
@@ -1368,28 +1471,30 @@ extern "C" size_t rm_composite_bwd_workspace_size(int64_t n_tets) {
     return n_tets > 0 ? (size_t)n_tets * kRowFloats * sizeof(float) : 0;
 }
 
-extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
-                                   const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
-                                   const float* vertices_dev, const float* features_dev, int32_t n_features,
-                                   const float* jitter_dev, const float* img_dev, const float* dist_dev,
-                                   const int32_t* ncontrib_dev, const float* grad_img_dev,
-                                   const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
-                                   void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
-    RM_CHECK_ARG(n_sorted >= 0, "rm_composite_bwd_ws: negative n_sorted");
-    RM_CHECK_ARG(n_tets >= 0, "rm_composite_bwd_ws: negative n_tets");
+namespace {
+int composite_bwd_rows_entry(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                             const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                             const float* vertices_dev, const float* features_dev, int32_t n_features,
+                             const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                             const int32_t* ncontrib_dev, const float* grad_img_dev, const float* grad_dist_dev,
+                             float* grad_vertices_dev, float* grad_features_dev, const uint8_t* flags_dev,
+                             void* workspace_dev, size_t workspace_bytes, rm_stream_t stream, const char* who) {
+    const std::string w(who);
+    RM_CHECK_ARG(n_sorted >= 0, w + ": negative n_sorted");
+    RM_CHECK_ARG(n_tets >= 0, w + ": negative n_tets");
     if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
     int e = check_common(cam, n_features, sorted_tet_dev, ranges_dev, indices_dev, vertices_dev, features_dev,
-                         jitter_dev, "rm_composite_bwd_ws");
+                         jitter_dev, who);
     if (e) return e;
     RM_CHECK_ARG(img_dev && dist_dev && ncontrib_dev && grad_img_dev && grad_dist_dev && grad_vertices_dev &&
                      grad_features_dev,
-                 "rm_composite_bwd_ws: null buffer");
+                 w + ": null buffer");
     RM_CHECK_ARG(((uintptr_t)img_dev & 15) == 0 && ((uintptr_t)grad_img_dev & 15) == 0,
-                 "rm_composite_bwd_ws: img and grad_img must be 16-byte aligned");
+                 w + ": img and grad_img must be 16-byte aligned");
     RM_CHECK_ARG(workspace_bytes >= rm_composite_bwd_workspace_size(n_tets),
-                 "rm_composite_bwd_ws: workspace smaller than rm_composite_bwd_workspace_size(n_tets)");
+                 w + ": workspace smaller than rm_composite_bwd_workspace_size(n_tets)");
     RM_CHECK_ARG(n_tets == 0 || (workspace_dev && ((uintptr_t)workspace_dev & 127) == 0),
-                 "rm_composite_bwd_ws: workspace must be 128-byte aligned");
+                 w + ": workspace must be 128-byte aligned");
     CamArgs a = make_cam_args(*cam);
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
@@ -1397,7 +1502,7 @@ extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_t
 #define RM_BWD(TS, F, P) launch_bwd<TS, F, P, true>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, \
                                         vertices_dev, features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, \
                                         grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev, rows, \
-                                        n_tets, st)
+                                        n_tets, st, flags_dev)
 #define RM_BWD_P(TS, F) (g_fp64_planes ? RM_BWD(TS, F, true) : RM_BWD(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_BWD_P(4, 7); else if (ts == 8) RM_BWD_P(8, 7); else RM_BWD_P(16, 7);
@@ -1408,6 +1513,36 @@ extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_t
 #undef RM_BWD
     RM_CHECK_LAUNCH("composite_bwd_kernel");
     return RM_OK;
+}
+}  // namespace
+
+extern "C" int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                   const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                                   const float* vertices_dev, const float* features_dev, int32_t n_features,
+                                   const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                                   const int32_t* ncontrib_dev, const float* grad_img_dev,
+                                   const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
+                                   void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
+    return composite_bwd_rows_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, n_tets, vertices_dev,
+                                    features_dev, n_features, jitter_dev, img_dev, dist_dev, ncontrib_dev,
+                                    grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev, nullptr,
+                                    workspace_dev, workspace_bytes, stream, "rm_composite_bwd_ws");
+}
+
+extern "C" int rm_composite_bwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
+                                      const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
+                                      const float* vertices_dev, const float* features_dev, int32_t n_features,
+                                      const float* jitter_dev, const float* img_dev, const float* dist_dev,
+                                      const int32_t* ncontrib_dev, const uint8_t* pair_flags_dev,
+                                      const float* grad_img_dev, const float* grad_dist_dev,
+                                      float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
+                                      size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(pair_flags_dev != nullptr || n_sorted == 0, "rm_composite_bwd_flags: null pair_flags");
+    return composite_bwd_rows_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, n_tets, vertices_dev,
+                                    features_dev, n_features, jitter_dev, img_dev, dist_dev, ncontrib_dev,
+                                    grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev,
+                                    pair_flags_dev, workspace_dev, workspace_bytes, stream,
+                                    "rm_composite_bwd_flags");
 }
 
 // ---- no-allocation variant of rm_composite_bwd_ws -------------------------------------------

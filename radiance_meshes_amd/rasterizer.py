@@ -250,13 +250,24 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         tet_alive = torch.zeros((indices.shape[0],), dtype=torch.bool, device=dev)
         cam = _cam(tcam, render_grid)
         F = tet_density.shape[1]
+        # per list entry: did some pixel of its tile blend the tet (4x4 and 8x8 tiles); the backward
+        # then skips the entries no pixel blended
+        P = sorted_tetra_idx.numel()
+        flags = torch.empty((P if ts[0] <= 8 else 0,), device=dev, dtype=torch.uint8)
         with timing.span("composite_fwd"):
-            check(lib().rm_composite_fwd(cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges), ptr(indices), ptr(vertices),
-                                     ptr(tet_density), F, ptr(ray_jitter), ptr(output_img), ptr(distortion_img),
-                                     ptr(n_contributors), stream_of(dev)), "rm_composite_fwd")
+            if flags.numel():
+                check(lib().rm_composite_fwd_flags(cam, ptr(sorted_tetra_idx), P, ptr(tile_ranges), ptr(indices),
+                                                   ptr(vertices), ptr(tet_density), F, ptr(ray_jitter), ptr(output_img),
+                                                   ptr(distortion_img), ptr(n_contributors), ptr(flags),
+                                                   stream_of(dev)), "rm_composite_fwd_flags")
+            else:
+                check(lib().rm_composite_fwd(cam, ptr(sorted_tetra_idx), P, ptr(tile_ranges), ptr(indices),
+                                             ptr(vertices), ptr(tet_density), F, ptr(ray_jitter), ptr(output_img),
+                                             ptr(distortion_img), ptr(n_contributors), stream_of(dev)),
+                      "rm_composite_fwd")
         non_tensor, tensor_data = split_tensors(tcam)
         ctx.save_for_backward(sorted_tetra_idx, tile_ranges, indices, vertices, tet_density, output_img,
-                              distortion_img, n_contributors, ray_jitter, *tensor_data)
+                              distortion_img, n_contributors, ray_jitter, flags, *tensor_data)
         ctx.non_tensor_data = non_tensor
         ctx.render_grid = render_grid
         ctx.cam = cam
@@ -266,7 +277,7 @@ class AlphaBlendTiledRender(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output_img, grad_distortion_img, grad_tet_alive):
         (sorted_tetra_idx, tile_ranges, indices, vertices, tet_density, output_img, distortion_img,
-         n_contributors, ray_jitter) = ctx.saved_tensors[:9]
+         n_contributors, ray_jitter, flags) = ctx.saved_tensors[:10]
         if grad_output_img is None:
             grad_output_img = torch.zeros_like(output_img)
         if grad_distortion_img is None:
@@ -280,12 +291,21 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         wsb = L.rm_composite_bwd_workspace_size(n_tets)
         ws = torch.empty((max(wsb, 1),), device=vertices.device, dtype=torch.uint8)
         with timing.span("composite_bwd"):
-            check(L.rm_composite_bwd_ws(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(), ptr(tile_ranges),
-                                        ptr(indices), n_tets, ptr(vertices), ptr(tet_density), tet_density.shape[1],
-                                        ptr(ray_jitter), ptr(output_img), ptr(distortion_img), ptr(n_contributors),
-                                        ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),
-                                        ptr(tet_density_grad), ptr(ws), wsb, stream_of(vertices.device)),
-                  "rm_composite_bwd_ws")
+            if flags.numel():
+                check(L.rm_composite_bwd_flags(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(),
+                                               ptr(tile_ranges), ptr(indices), n_tets, ptr(vertices), ptr(tet_density),
+                                               tet_density.shape[1], ptr(ray_jitter), ptr(output_img),
+                                               ptr(distortion_img), ptr(n_contributors), ptr(flags),
+                                               ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),
+                                               ptr(tet_density_grad), ptr(ws), wsb, stream_of(vertices.device)),
+                      "rm_composite_bwd_flags")
+            else:
+                check(L.rm_composite_bwd_ws(ctx.cam, ptr(sorted_tetra_idx), sorted_tetra_idx.numel(),
+                                            ptr(tile_ranges), ptr(indices), n_tets, ptr(vertices), ptr(tet_density),
+                                            tet_density.shape[1], ptr(ray_jitter), ptr(output_img),
+                                            ptr(distortion_img), ptr(n_contributors), ptr(grad_output_img),
+                                            ptr(grad_distortion_img), ptr(vertices_grad), ptr(tet_density_grad),
+                                            ptr(ws), wsb, stream_of(vertices.device)), "rm_composite_bwd_ws")
         return None, None, None, vertices_grad, tet_density_grad, None, None, None
 
 

@@ -325,3 +325,67 @@ def test_projection_backward_fp64(oracle_lib, kind, tile):
           f"(median {np.median(rel):.2e}, |g|max {np.abs(o).max():.3e})")
     assert np.abs(o).max() > 0
     assert rel.max() <= 1e-3
+
+
+@pytest.mark.parametrize("kind,tile,nf", [("plain", 4, 7), ("plain", 8, 13), ("ball", 4, 13), ("ball", 8, 7)])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_flagged_pair_matches_workspace_path(kind, tile, nf, fp64):
+    """rm_composite_fwd_flags writes the same image, distortion and n_contrib as rm_composite_fwd
+    (bit for bit); rm_composite_bwd_flags, which walks only the entries some pixel of the tile
+    blended, gives the gradients of rm_composite_bwd_ws (per element within 1e-5 of the largest:
+    the two differ only in the order of float additions); on the scene an entry is skipped for."""
+    import torch
+    from radiance_meshes_amd.rasterizer import _cam, set_intersect_precision, vertex_and_tile_shader
+    from radiance_meshes_amd._capi import check, lib, ptr, stream_of
+    sc = scene(kind, tile, nf)
+    print(f"flagged pair {kind} tile={tile} nf={nf} fp64={fp64}", flush=True)
+    verts, idx, feats, _ = device_inputs(sc)
+    jit = torch.from_numpy(jitter(sc, 5)).cuda()
+    tcam, rg = tcam_of(sc), grid_of(sc)
+    H, W = sc.camera.height, sc.camera.width
+    sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(idx, verts, tcam, rg)
+    cam = _cam(tcam, rg)
+    L = lib()
+    st = stream_of(verts.device)
+    P, T = sidx.numel(), idx.shape[0]
+    outs = []
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    gimg = torch.randn((H, W, 4), device="cuda", generator=gen)
+    gdist = torch.randn((H, W, 5), device="cuda", generator=gen) * 0.1
+    wsb = L.rm_composite_bwd_workspace_size(T)
+    set_intersect_precision(fp64)
+    try:
+        for flagged in (False, True):
+            img = torch.empty((H, W, 4), device="cuda")
+            dist = torch.empty((H, W, 5), device="cuda")
+            nc = torch.empty((H, W), device="cuda", dtype=torch.int32)
+            flags = torch.full((P,), 7, device="cuda", dtype=torch.uint8)
+            ws = torch.empty((wsb,), device="cuda", dtype=torch.uint8)
+            gv = torch.zeros_like(verts)
+            gf = torch.zeros_like(feats)
+            if flagged:
+                check(L.rm_composite_fwd_flags(cam, ptr(sidx), P, ptr(ranges), ptr(idx), ptr(verts), ptr(feats), nf,
+                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), st), "fwd_flags")
+                check(L.rm_composite_bwd_flags(cam, ptr(sidx), P, ptr(ranges), ptr(idx), T, ptr(verts), ptr(feats), nf,
+                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), ptr(gimg),
+                                               ptr(gdist), ptr(gv), ptr(gf), ptr(ws), wsb, st), "bwd_flags")
+            else:
+                check(L.rm_composite_fwd(cam, ptr(sidx), P, ptr(ranges), ptr(idx), ptr(verts), ptr(feats), nf,
+                                         ptr(jit), ptr(img), ptr(dist), ptr(nc), st), "fwd")
+                check(L.rm_composite_bwd_ws(cam, ptr(sidx), P, ptr(ranges), ptr(idx), T, ptr(verts), ptr(feats), nf,
+                                            ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(gimg), ptr(gdist), ptr(gv),
+                                            ptr(gf), ptr(ws), wsb, st), "bwd_ws")
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy() for t in (img, dist, nc, gv, gf, flags)])
+    finally:
+        set_intersect_precision(True)
+    for a, b in zip(outs[0][:3], outs[1][:3]):
+        np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+    fl = outs[1][5]
+    n_ent = int((fl <= 1).sum())
+    print(f"{n_ent} of {P} entries walked, {int((fl == 1).sum())} flagged")
+    assert int((fl == 0).sum()) > 0 and int((fl == 1).sum()) > 0 and set(np.unique(fl)) <= {0, 1, 7}
+    for name, a, b in (("vertex", outs[1][3], outs[0][3]), ("feature", outs[1][4], outs[0][4])):
+        err = np.abs(a.astype(np.float64) - b).max() / np.abs(b).max()
+        print(f"{name}: flagged vs workspace path, max abs diff / max |grad| = {err:.2e}")
+        assert err <= 1e-5, name
