@@ -9,10 +9,14 @@ vertices. The mesh is 1M points from a seeded 32-component Gaussian mixture plus
 Delaunay-tetrahedralised on the CPU before timing starts.
 
 Prints one JSON line (rank 0): metric fwd+bwd Mrays/s (rays = pixels per step / step time), the
-roofline of the dominant kernel (HIP events around its launches inside the timed region), and a
-CPU baseline: the reference's CPU method (per-ray midpoint quadrature testing every tet at every
-sample, jaxutil/tetra_quad.py) restated in C, timed on a bounded sample of rays of the same view.
-The line also carries the per-tet error-vote timing, the BF16-vs-FP64 field report, the GPU
+roofline of the dominant kernel (HIP events around its launches inside the timed region; HBM and
+vector-ALU issue figures), and a CPU baseline: the same rasterizer restated in C (oracle/, FP64
+compositing, OpenMP) on the host's cores, projection and binning of the whole mesh plus the
+compositing forward and backward of a sample of tiles, next to the reference's CPU method (per-ray
+midpoint quadrature testing every tet at every sample, jaxutil/tetra_quad.py) on a few rays; the
+GPU image of the step is compared with the FP64 one on the sampled tiles (image_err_vs_fp64).
+The line also carries forward-only frame rates at 1080p/2k/4k on the same mesh, the per-tet
+error-vote timing, the BF16-vs-FP64 field report, the GPU
 Delaunay tetrahedralisation of the benchmark's points (wall time per call, set difference with the
 benchmark mesh, scipy/Qhull on a 100k-point sample as the CPU reference) and the serving leg: three device-resident meshes (250k, 1M, 3M points) served a Poisson trace of
 mixed-resolution views (256x256 .. 1920x1080, 200/s for --serve-seconds, then a saturating
@@ -32,6 +36,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+N_CU, CLOCK_HZ = 256, 2.4e9    # MI355X compute units and peak engine clock
 FP32_VALU_PEAK_TFLOPS = 78.6   # non-packed FP32 vector FMA rate (packed FP32 ops are not used)
 
 
@@ -52,9 +57,13 @@ def parse():
                    "(one JSON line each on stderr)")
     p.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
     p.add_argument("--intersect", default="fp64", choices=["fp64", "fp32"])
-    p.add_argument("--cpu-baseline-rays", type=int, default=32)
-    p.add_argument("--cpu-baseline-samples", type=int, default=128)
+    p.add_argument("--cpu-baseline-tiles", type=int, default=4096,
+                   help="tiles of the view composited (forward + backward) by the CPU restatement")
+    p.add_argument("--cpu-baseline-rays", type=int, default=16, help="rays of the quadrature (reference method)")
+    p.add_argument("--cpu-baseline-samples", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-forward-fps", action="store_true",
+                   help="skip the forward-only 1080p/2k/4k frame rates on the benchmark mesh")
     p.add_argument("--no-precision-report", action="store_true",
                    help="skip the BF16-vs-FP64 field MLP report on the benchmark mesh")
     p.add_argument("--no-layout", action="store_true", help="keep the Delaunay order of vertices and tets")
@@ -118,8 +127,9 @@ def build_workload(args):
     return pts, tets, cam
 
 
-def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats):
-    """Sum over pixels of n_contrib: (pixel, tet) evaluations of one forward pass."""
+def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats, out=None):
+    """Sum over pixels of n_contrib: (pixel, tet) evaluations of one forward pass (the image is
+    left in out["img"] when out is given)."""
     import torch
     from radiance_meshes_amd._capi import check, lib, ptr, stream_of
     from radiance_meshes_amd.rasterizer import _cam
@@ -130,6 +140,8 @@ def count_evaluations(indices, vertices, tcam, rg, jitter, sidx, ranges, feats):
     check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(indices), ptr(vertices),
                                  ptr(feats), feats.shape[1], ptr(jitter), ptr(img), ptr(dist), ptr(nc),
                                  stream_of()), "rm_composite_fwd")
+    if out is not None:
+        out["img"] = img
     return int(nc.sum().item())
 
 
@@ -206,9 +218,10 @@ def delaunay_leg(args, pts, tets, reps=3):
                              "points_per_s": round(n_cpu / cpu_s)}}
 
 
-def measured_traffic(workload, tile, precision, span):
+def measured_traffic(workload, tile, precision, span, key="traffic"):
     """HBM bytes per launch of `span`'s kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
-    recorded for this workload (profiles/traffic_*.json), or None when none was recorded."""
+    recorded for this workload (profiles/traffic_*.json), or None when none was recorded; with
+    key="valu_insts", the span's VALU wave-instructions per launch (SQ_INSTS_VALU pass)."""
     here = os.path.dirname(os.path.abspath(__file__))
     path = os.path.join(here, "profiles", "traffic_gmm1M_1280x840.json")
     try:
@@ -218,7 +231,7 @@ def measured_traffic(workload, tile, precision, span):
         return None
     if t.get("workload") != workload or t.get("tile") != tile or t.get("precision") != precision:
         return None
-    return t.get("traffic", {}).get(span)
+    return t.get(key, {}).get(span)
 
 
 def run_config(args, pts, tets, cam, tile):
@@ -274,11 +287,17 @@ def run_config(args, pts, tets, cam, tile):
     ms = dt / steps * 1e3
     mrays = H * W / (dt / steps) / 1e6
     n_vis = int(stats["mask"].sum().item())
-    n_eval = count_evaluations(indices, vertices.detach(), tcam, rg, jitter, *stats["last"])
+    # one more forward, without gradients, on the parameters as the timed steps left them: the
+    # evaluation count and what the CPU leg checks (binning, features, image) all come from it
+    with torch.no_grad():
+        sidx_l, ranges_l, _, _, mask_l, _ = vertex_and_tile_shader(indices, vertices, tcam, rg)
+        feats_l = field.features(vertices, indices, mask_l)
+    last_img = {}
+    n_eval = count_evaluations(indices, vertices.detach(), tcam, rg, jitter, sidx_l, ranges_l, feats_l, out=last_img)
     P = int(stats["P"])
     G = rg.grid_height * rg.grid_width
     N = H * W
-    votes = time_tet_err(indices, vertices.detach(), tcam, rg, *stats["last"], reps=3)
+    votes = time_tet_err(indices, vertices.detach(), tcam, rg, sidx_l, ranges_l, feats_l, reps=3)
     V = vertices.shape[0]
 
     # algorithmic bytes per launch (FP32 values, int32 indices)
@@ -303,10 +322,22 @@ def run_config(args, pts, tets, cam, tile):
                 "traffic": measured_traffic(workload, tile, args.precision, dominant),
                 "mean_ms": round(d["mean_ms"], 4),
                 "algorithmic_bytes": int(algo.get(dominant, 0))}
+    # the compositing kernels are bound by vector-ALU issue, not HBM: wave-instructions per launch
+    # (rocprofv3 SQ_INSTS_VALU pass of this workload) over the issue peak of one VALU
+    # wave-instruction per CU per cycle at the peak engine clock
+    insts = measured_traffic(workload, tile, args.precision, dominant, key="valu_insts")
+    if insts:
+        peak = N_CU * CLOCK_HZ * d["mean_ms"] * 1e-3
+        roofline["valu"] = {"insts_per_launch": insts, "issue_peak_per_s": N_CU * CLOCK_HZ,
+                            "frac": round(insts / peak, 4)}
     total_algo = sum(algo[k] * kern[k]["launches"] / steps for k in algo if k in kern)
     breakdown = {k: round(v["total_ms"] / steps, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])}
 
-    return {"value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
+    # what the CPU leg compares against: this step's binning, features, image and loss target
+    last = {"sidx": sidx_l.cpu().numpy(), "ranges": ranges_l.cpu().numpy(), "feats": feats_l.cpu().numpy(),
+            "img": last_img["img"].cpu().numpy(), "target": target.cpu().numpy(),
+            "vertices": vertices.detach().cpu().numpy()}
+    return {"last": last, "value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
             "hbm_frac_all_kernels": round(total_algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "breakdown_ms": breakdown,
             "tet_err": {**votes, "algorithmic_bytes": 96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"],
@@ -380,6 +411,18 @@ def precision_report(pts, tets, cam, tile):
             out[name] = {**params(raw), "pixel_abs_max": float(e.max()), "pixel_abs_mean": float(e.mean())}
     torch.cuda.synchronize()
     return out
+
+
+def full_features(pts, tets):
+    """13-channel features of every tet from the training step's field (default initialisation)."""
+    import torch
+    from radiance_meshes_amd import field as fieldmod
+    fld = fieldmod.Field(n_features=13, precision="fp32", scale=2.0, device="cuda")
+    with torch.no_grad():
+        v = torch.from_numpy(pts).cuda()
+        i = torch.from_numpy(tets).cuda()
+        f = fld.features(v, i, torch.ones((len(tets),), dtype=torch.bool, device="cuda"))
+    return f.cpu().numpy()
 
 
 def calibrate_copy():
@@ -521,6 +564,9 @@ def main():
     precision = None
     if not args.serve_only and not args.no_precision_report and not args.profile_steps:
         precision = precision_report(pts, tets, cam, args.tile)
+    fps = None
+    if res is not None and not args.no_forward_fps and not args.profile_steps:
+        fps = forward_fps(args, pts, tets, (6.0, 30.0, 20.0), full_features(pts, tets))
     tri = None
     if not args.serve_only and not args.no_delaunay and not args.profile_steps:
         try:
@@ -545,8 +591,8 @@ def main():
             log(f"[bench] serving leg failed: {type(e).__name__}: {e}")
             serving = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
-    if not args.no_cpu_baseline and not args.profile_steps:
-        cpu = cpu_baseline(args, pts, tets, cam)
+    if res is not None and not args.no_cpu_baseline and not args.profile_steps:
+        cpu = cpu_baseline(args, pts, tets, cam, res["last"])
     if res is None:
         print(json.dumps({"metric": "batched-served vs per-frame fps", "serving": serving}), flush=True)
         return
@@ -569,6 +615,8 @@ def main():
         "hbm_frac_all_kernels": res["hbm_frac_all_kernels"],
         "breakdown_ms": res["breakdown_ms"],
         "tet_err": res.get("tet_err"),
+        "image_err_vs_fp64": cpu.get("image_err_vs_fp64") if cpu else None,
+        "forward_fps": fps,
         "precision": precision,
         "delaunay": tri,
         "cpu_baseline": cpu,
@@ -577,37 +625,132 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(args, pts, tets, cam):
-    """The reference's CPU method (midpoint quadrature over all tets per sample) on host cores."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import oracle as O
-    from radiance_meshes_amd import scenes
-    import torch
-    feats = scenes.tet_features(len(tets), 1, 13, np.float64)
-    ocam = O.make_camera(cam, args.tile, 0.1)
-    rng = np.random.default_rng(0)
-    pix = rng.choice(args.width * args.height, args.cpu_baseline_rays, replace=False)
-    jit = np.full((args.height, args.width, 2), 0.5, np.float32)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"[bench] CPU baseline: {len(pix)} rays x {args.cpu_baseline_samples} samples x {len(tets)} tets "
-        f"on {cores} threads")
-    t0 = time.perf_counter()
-    O.quadrature(ocam, jit, pix, 12.0, args.cpu_baseline_samples, tets, pts.astype(np.float64), feats)
-    t = time.perf_counter() - t0
+def _cpu_model():
     cpu_model = platform.processor() or ""
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": round(len(pix) / t / 1e6, 9), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{len(pix)} random pixels of the {args.width}x{args.height} view, "
-                      f"{args.cpu_baseline_samples} midpoint samples per ray over t in [0, 12], every one of "
-                      f"the {len(tets)} tets tested at every sample (forward only), {t:.1f} s",
-            "cpu": cpu_model}
+    return cpu_model
+
+
+def cpu_baseline(args, pts, tets, cam, last):
+    """The same algorithm on the host's cores, as the reference's own CPU path (tetra_quad) is the
+    method the image is checked against but not a rasterizer: the CPU restatement (oracle/) of
+    projection, binning and the per-tile compositor forward + backward (FP64 segments, OpenMP over
+    tiles) on this step's features. Projection and binning run over the whole mesh; compositing
+    over a bounded sample of tiles, extrapolated to the frame by tile count (labelled as such).
+    The GPU image of the same step is compared with the FP64 image on those tiles
+    (image_err_vs_fp64). The reference method itself -- midpoint quadrature testing every tet at
+    every sample (jaxutil/tetra_quad.py), restated in C -- is timed on a few rays as well."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    H, W, tile = args.height, args.width, args.tile
+    ocam = O.make_camera(cam, tile, 0.1)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[bench] CPU baseline: oracle projection, binning and compositing on {cores} threads")
+    pts = last["vertices"]
+    t0 = time.perf_counter()
+    ot, orr, ovs, occ = O.project(ocam, tets, pts)
+    t_proj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    keys, vals, oranges = O.bin_tiles(ocam, ot, orr, ovs)
+    t_bin = time.perf_counter() - t0
+    same_bins = bool(np.array_equal(vals, last["sidx"]) and np.array_equal(oranges, last["ranges"]))
+    G = ocam.gh * ocam.gw
+    n_s = min(args.cpu_baseline_tiles, G)
+    sample = np.sort(np.random.default_rng(7).choice(G, n_s, replace=False)).astype(np.int32)
+    f64 = last["feats"].astype(np.float64)
+    jit = np.full((H, W, 2), 0.5, np.float32)
+    t0 = time.perf_counter()
+    o_img, o_dist, o_nc, margin = O.composite_fwd(ocam, vals, oranges, tets, pts, f64, jit, tiles=sample)
+    t_fwd = time.perf_counter() - t0
+    gimg = np.zeros((H, W, 4))
+    gimg[..., :3] = 2.0 * (o_img[..., :3] - last["target"]) / (3 * H * W)
+    t0 = time.perf_counter()
+    O.composite_bwd(ocam, vals, oranges, tets, pts, f64, jit, o_img, o_dist, o_nc, gimg, np.zeros((H, W, 5)),
+                    tiles=sample)
+    t_bwd = time.perf_counter() - t0
+    frame_s = t_proj + t_bin + (t_fwd + t_bwd) * G / n_s
+    pix = np.zeros((H, W), bool)
+    for t in sample:
+        ty, tx = divmod(int(t), ocam.gw)
+        pix[ty * tile:(ty + 1) * tile, tx * tile:(tx + 1) * tile] = True
+    near = pix & (margin < 1e-4)
+    err = np.abs(last["img"][..., :3].astype(np.float64) - o_img[..., :3])
+    image_err = {"max_abs_rgb": float(err[pix & ~near].max()), "mean_abs_rgb": float(err[pix & ~near].mean()),
+                 "pixels": int(pix.sum()), "tiles": n_s,
+                 "near_threshold_pixels": int(near.sum()),
+                 "near_threshold_max_abs_rgb": float(err[near].max()) if near.any() else 0.0,
+                 "reference": "oracle FP64 compositor on the same binning and features",
+                 "binning_bit_exact_vs_oracle": same_bins}
+    # the reference's CPU method (per-ray midpoint quadrature over every tet) on a few rays
+    feats = last["feats"].astype(np.float64)
+    pix_q = np.random.default_rng(0).choice(W * H, args.cpu_baseline_rays, replace=False)
+    t0 = time.perf_counter()
+    O.quadrature(ocam, jit, pix_q, 12.0, args.cpu_baseline_samples, tets, pts.astype(np.float64), feats)
+    t_q = time.perf_counter() - t0
+    log(f"[bench] CPU: projection {t_proj:.2f} s, binning {t_bin:.2f} s, compositing {n_s} tiles "
+        f"{t_fwd:.2f} + {t_bwd:.2f} s; quadrature {t_q:.1f} s")
+    return {"value": round(W * H / frame_s / 1e6, 6), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"fwd+bwd of the {W}x{H} step's rasterizer: projection and binning of all "
+                      f"{len(tets)} tets ({t_proj:.2f} + {t_bin:.2f} s), compositing forward + backward of "
+                      f"{n_s} random {tile}x{tile} tiles of {G} ({t_fwd:.2f} + {t_bwd:.2f} s) extrapolated "
+                      f"to the frame by tile count; the field is not included",
+            "frame_s_extrapolated": round(frame_s, 3), "cpu": _cpu_model(),
+            "reference_method": {"what": "midpoint quadrature testing every tet at every sample "
+                                         "(jaxutil/tetra_quad.py) restated in C, forward only",
+                                 "rays": int(len(pix_q)), "samples": args.cpu_baseline_samples,
+                                 "seconds": round(t_q, 2), "mrays_per_s": round(len(pix_q) / t_q / 1e6, 9)},
+            "image_err_vs_fp64": image_err}
+
+
+def forward_fps(args, pts, tets, cam_pose, feats13):
+    """Forward-only frame rate at 1920x1080, 2560x1440 and 3840x2160 (the resolutions of the
+    reference's FPS table, tile_benchmark.py:80-88, 4x4 tiles as there) on the benchmark mesh:
+    projection, binning (with its host read of the pair count) and compositing of fixed per-tet
+    features, timed wall-clock per frame with the device synchronised."""
+    import torch
+    from radiance_meshes_amd import scenes
+    from radiance_meshes_amd.rasterizer import AlphaBlendTiledRender, RenderGrid, vertex_and_tile_shader
+    dev = "cuda"
+    verts = torch.from_numpy(pts).to(dev)
+    idx = torch.from_numpy(tets).to(dev)
+    feats = torch.from_numpy(feats13).to(dev)
+    out = {}
+    for name, (H, W) in (("1080p", (1080, 1920)), ("2k", (1440, 2560)), ("4k", (2160, 3840))):
+        cam = scenes.orbit_camera(*cam_pose, W, H, 50.0)
+        tcam = {k: (v.to(dev) if hasattr(v, "to") else v) for k, v in cam.tcam(4, 0.1).items()}
+        rg = RenderGrid(H, W, 4, 4)
+        jit = torch.full((H, W, 2), 0.5, device=dev)
+
+        def frame():
+            with torch.no_grad():
+                sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(idx, verts, tcam, rg)
+                img, dist, _ = AlphaBlendTiledRender.apply(sidx, ranges, idx, verts, feats, rg, tcam, jit)
+            return img
+
+        for _ in range(2):
+            frame()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(10):
+            t0 = time.perf_counter()
+            frame()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(t))
+        out[name] = {"width": W, "height": H, "ms_per_frame": round(ms, 3), "fps": round(1e3 / ms, 1),
+                     "mrays_per_s": round(W * H / ms / 1e3, 1)}
+        log(f"[bench] forward {name}: {ms:.2f} ms/frame")
+    del verts, idx, feats
+    torch.cuda.empty_cache()
+    return {"tets": int(len(tets)), "tile": 4, "features": "13-channel field output of the training step",
+            "timed": "projection + binning + compositing forward, wall clock per frame, device synchronised",
+            **out}
 
 
 if __name__ == "__main__":

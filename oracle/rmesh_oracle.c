@@ -272,48 +272,60 @@ uint32_t orc_sortable(float value) { /* tile_shader.slang:18-47 */
     return bits;
 }
 
-typedef struct { uint64_t key; int64_t pos; int32_t val; } orc_pair;
+typedef struct { uint32_t depth; int32_t tet; } orc_entry;
 
-static int cmp_pair(const void* a, const void* b) {
-    const orc_pair* x = (const orc_pair*)a;
-    const orc_pair* y = (const orc_pair*)b;
-    if (x->key != y->key) return x->key < y->key ? -1 : 1;
-    return x->pos < y->pos ? -1 : (x->pos > y->pos);
+static int cmp_entry(const void* a, const void* b) {
+    const orc_entry* x = (const orc_entry*)a;
+    const orc_entry* y = (const orc_entry*)b;
+    if (x->depth != y->depth) return x->depth < y->depth ? -1 : 1;
+    return (x->tet > y->tet) - (x->tet < y->tet);
 }
 
-/* Generates keys in the reference's buffer order (tet order, row-major tiles per tet), sorts
- * them stably, and writes sorted keys/values and [G,2] ranges (empty tiles = [0,0]).
- * Returns P. keys_out/vals_out must hold sum(tiles) entries. */
+/* Generates keys in the reference's buffer order (tet order, row-major tiles per tet) and sorts
+ * them stably by the 64-bit key (tile << 32) | sortable(depth), writing sorted keys/values and
+ * [G,2] ranges (empty tiles = [0,0]). A stable sort by that key is the same as grouping the
+ * entries by tile in buffer order and sorting each tile's entries by depth key, ties kept in
+ * buffer order, i.e. by increasing tet index (a tet adds at most one entry per tile); the tiles
+ * are sorted in parallel. Returns P. keys_out/vals_out must hold sum(tiles) entries. */
 int64_t orc_bin(const int32_t* tiles, const int32_t* rect, const float* vs, int64_t T, int32_t gh,
                 int32_t gw, int64_t* keys_out, int32_t* vals_out, int32_t* ranges) {
-    int64_t P = 0;
-    for (int64_t t = 0; t < T; ++t) P += tiles[t];
-    orc_pair* pr = (orc_pair*)malloc(sizeof(orc_pair) * (size_t)(P > 0 ? P : 1));
-    int64_t o = 0;
+    int64_t G = (int64_t)gh * gw;
+    int64_t* off = (int64_t*)calloc((size_t)G + 1, sizeof(int64_t));
+    for (int64_t t = 0; t < T; ++t) {
+        if (tiles[t] <= 0) continue;
+        for (int32_t y = rect[4 * t + 1]; y < rect[4 * t + 3]; ++y)
+            for (int32_t x = rect[4 * t + 0]; x < rect[4 * t + 2]; ++x) off[(int64_t)y * gw + x + 1]++;
+    }
+    for (int64_t g = 0; g < G; ++g) off[g + 1] += off[g];
+    int64_t P = off[G];
+    orc_entry* e = (orc_entry*)malloc(sizeof(orc_entry) * (size_t)(P > 0 ? P : 1));
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(G > 0 ? G : 1));
+    for (int64_t g = 0; g < G; ++g) fill[g] = off[g];
     for (int64_t t = 0; t < T; ++t) {
         if (tiles[t] <= 0) continue;
         uint32_t fk = orc_sortable(vs[3 * t + 2]);
         for (int32_t y = rect[4 * t + 1]; y < rect[4 * t + 3]; ++y)
             for (int32_t x = rect[4 * t + 0]; x < rect[4 * t + 2]; ++x) {
-                uint64_t key = (uint64_t)(uint32_t)(y * gw + x);
-                key = (key << 32) | fk;
-                pr[o].key = key; pr[o].pos = o; pr[o].val = (int32_t)t; o++;
+                int64_t g = (int64_t)y * gw + x;
+                e[fill[g]].depth = fk;
+                e[fill[g]].tet = (int32_t)t;
+                fill[g]++;
             }
     }
-    qsort(pr, (size_t)P, sizeof(orc_pair), cmp_pair);
-    for (int64_t i = 0; i < (int64_t)gh * gw * 2; ++i) ranges[i] = 0;
-    for (int64_t i = 0; i < P; ++i) {
-        keys_out[i] = (int64_t)pr[i].key;
-        vals_out[i] = pr[i].val;
-        uint32_t cur = (uint32_t)(pr[i].key >> 32);
-        if (i == 0) ranges[2 * cur] = 0;
-        else {
-            uint32_t prev = (uint32_t)(pr[i - 1].key >> 32);
-            if (cur != prev) { ranges[2 * prev + 1] = (int32_t)i; ranges[2 * cur] = (int32_t)i; }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t g = 0; g < G; ++g) {
+        int64_t n = off[g + 1] - off[g];
+        if (n > 1) qsort(e + off[g], (size_t)n, sizeof(orc_entry), cmp_entry);
+        for (int64_t i = off[g]; i < off[g + 1]; ++i) {
+            keys_out[i] = (int64_t)(((uint64_t)(uint32_t)g << 32) | e[i].depth);
+            vals_out[i] = e[i].tet;
         }
-        if (i == P - 1) ranges[2 * cur + 1] = (int32_t)P;
+        ranges[2 * g] = n > 0 ? (int32_t)off[g] : 0;
+        ranges[2 * g + 1] = n > 0 ? (int32_t)off[g + 1] : 0;
     }
-    free(pr);
+    free(e);
+    free(fill);
+    free(off);
     return P;
 }
 

@@ -173,16 +173,19 @@ def rgb_to_sh(rgb):
     return (np.asarray(rgb) - 0.5) / SH_C0
 
 
-def save_mesh_ply(path, vertices, indices, density, base_rgb_raw, grd, sh_rest=None):
+def save_mesh_ply(path, vertices, indices, density, rgb, grd, sh_rest=None):
     """The reference's save2ply layout (models/base_model.py:107-151): vertex xyz; tetrahedron
-    indices, s (density), grd_x/y/z, sh_0_{r,g,b} = RGB2SH(raw base colour at vertex 0) and
-    sh_{i}_{r,g,b}, i = 1..K, from sh_rest [T, K, 3]."""
+    indices, s (density), grd_x/y/z (the normalised colour gradient), sh_0_{r,g,b} = RGB2SH(rgb)
+    with rgb the field's raw colour at the tet's anchor point (what compute_features(offset=True)
+    returns, models/ingp_color.py:487-512: the centroid when the model was trained with
+    ablate_circumsphere, the circumcenter otherwise), and sh_{i}_{r,g,b}, i = 1..K, from
+    sh_rest [T, K, 3]."""
     v = np.asarray(vertices, np.float32)
     tet = {"indices": np.asarray(indices, np.int32), "s": np.asarray(density, np.float32).reshape(-1)}
     g = np.asarray(grd, np.float32).reshape(-1, 3)
     for i, co in enumerate("xyz"):
         tet[f"grd_{co}"] = np.ascontiguousarray(g[:, i])
-    sh0 = rgb_to_sh(np.asarray(base_rgb_raw, np.float32).reshape(-1, 3)).astype(np.float32)
+    sh0 = rgb_to_sh(np.asarray(rgb, np.float32).reshape(-1, 3)).astype(np.float32)
     for i, ch in enumerate("rgb"):
         tet[f"sh_0_{ch}"] = np.ascontiguousarray(sh0[:, i])
     if sh_rest is not None:
@@ -251,17 +254,41 @@ def sh_basis(dirs, deg: int):
     return torch.stack(cols, 1) if cols else torch.zeros((dirs.shape[0], 0), dtype=dirs.dtype, device=dirs.device)
 
 
-def view_features(mesh, cam_pos, sh_deg: int = 3, device="cuda"):
+def tet_anchor(verts, idx, anchor: str):
+    """The point a tet's colour field is anchored at: "centroid" (the mean of its vertices) or
+    "circumcenter" (the textbook circumsphere centre, in FP64 as calculate_circumcenters_torch
+    evaluates it on double vertices, models/ingp_color.py:320)."""
+    import torch
+    tets = verts[idx]
+    if anchor == "centroid":
+        return tets.mean(dim=1)
+    if anchor != "circumcenter":
+        raise ValueError(f"anchor must be 'centroid' or 'circumcenter', got {anchor!r}")
+    t = tets.double()
+    a, b, c = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0], t[:, 3] - t[:, 0]
+    num = ((a * a).sum(1, keepdim=True) * torch.cross(b, c, dim=1) + (b * b).sum(1, keepdim=True)
+           * torch.cross(c, a, dim=1) + (c * c).sum(1, keepdim=True) * torch.cross(a, b, dim=1))
+    den = 2 * (a * torch.cross(b, c, dim=1)).sum(1, keepdim=True)
+    return (t[:, 0] + num / den).to(verts.dtype)
+
+
+def view_features(mesh, cam_pos, sh_deg: int = 3, device="cuda", anchor: str = "centroid"):
     """Per-tet feature rows [T,7] = [density, colour, gradient] for a camera at cam_pos, from a
-    mesh dict of load_mesh_ply (tensors or arrays). colour = softplus_10(0.5 + SH_C0 sh0 +
-    sum_k Y_k(dir) sh_k) with dir the unit vector from the camera to the tet's centroid."""
+    mesh dict of load_mesh_ply (tensors or arrays), as activate_output (utils/model_util.py:93-105)
+    forms them: the colour field is SH2RGB(sh0) + grd . (p - anchor) (offset_normalize,
+    :81-91), evaluated at vertex 0 and view-dependent through the higher SH bands:
+    colour = softplus_10(SH2RGB(sh0) + grd . (v0 - anchor) + sum_k Y_k(dir) sh_k), dir the unit
+    vector from the camera to the tet's centroid. anchor: "centroid" -- where the reference's
+    model anchors the field when trained with ablate_circumsphere (train.py:145, its default) --
+    or "circumcenter"."""
     import torch
     t = {k: torch.as_tensor(v, device=device) for k, v in mesh.items()}
     verts, idx = t["vertices"].float(), t["indices"].long()
     cen = verts[idx].mean(dim=1)
     cp = torch.as_tensor(cam_pos, device=device, dtype=torch.float32).reshape(1, 3)
     d = torch.nn.functional.normalize(cen - cp, dim=1, eps=1e-8)
-    col = SH_C0 * t["sh0"].float() + 0.5
+    off = verts[idx[:, 0]] - tet_anchor(verts, idx, anchor)
+    col = SH_C0 * t["sh0"].float() + 0.5 + (t["grd"].float() * off).sum(1, keepdim=True)
     K = t["sh_rest"].shape[1]
     deg = min(sh_deg, int(round(math.sqrt(K + 1))) - 1)
     if deg > 0:

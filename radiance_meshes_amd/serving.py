@@ -190,7 +190,8 @@ class Bucket:
                 self.launch(reqs, keep_aux, full=True)
                 res, _ = self.finish()
             else:
-                res = [(r, (self.server.render_single(r.scene, r.camera),)) for r in reqs]
+                res = [(r, self.server.render_single(r.scene, r.camera, aux=True) if keep_aux
+                        else (self.server.render_single(r.scene, r.camera),)) for r in reqs]
             return res, True
         hw = self.H * self.W
         shapes = ((self.H, self.W, 4), (self.H, self.W, 5), (self.H, self.W))
@@ -215,6 +216,14 @@ class RenderServer:
 
     # ------------------------------------------------------------------ scenes
     def add_scene(self, vertices, indices, features):
+        """Registers a device-resident scene and returns its id. Buckets sized for the scenes
+        registered before are dropped (their workspaces and per-scene pair estimates no longer
+        fit) and are created again on their next use; a batch must not be in flight."""
+        if any(b.busy is not None for b in self.buckets.values()):
+            raise RuntimeError("add_scene while a batch is in flight: finish the batches first")
+        if self.buckets:
+            torch.cuda.synchronize(self.device)
+            self.buckets = {}
         v = vertices.to(self.device, torch.float32).contiguous()
         i = indices.to(self.device, torch.int32).contiguous()
         f = features.to(self.device, torch.float32).contiguous()
@@ -237,18 +246,25 @@ class RenderServer:
         return camera.tcam(self.tile, self.min_t)
 
     # ------------------------------------------------------------------ single view (baseline)
-    def render_single(self, sid, camera):
-        """One view through the single-view path (host sync for the pair count, as the reference)."""
-        from .rasterizer import AlphaBlendTiledRender, RenderGrid, vertex_and_tile_shader
+    def render_single(self, sid, camera, aux=False):
+        """One view through the single-view path (host sync for the pair count, as the reference):
+        the [H,W,4] image, or with aux the tuple (image, [H,W,5] distortion, [H,W] n_contrib) that
+        batched views return."""
+        from .rasterizer import RenderGrid, _cam, vertex_and_tile_shader
         sc = self.scenes[sid]
-        rg = RenderGrid(camera.height, camera.width, self.tile, self.tile)
+        H, W = camera.height, camera.width
+        rg = RenderGrid(H, W, self.tile, self.tile)
         tcam = self._tcam(camera)
         with torch.no_grad():
             sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(sc.indices, sc.vertices, tcam, rg)
-            jit = torch.full((camera.height, camera.width, 2), 0.5, device=self.device)
-            img, dist, _ = AlphaBlendTiledRender.apply(sidx, ranges, sc.indices, sc.vertices, sc.features, rg,
-                                                       tcam, jit)
-        return img
+            jit = torch.full((H, W, 2), 0.5, device=self.device)
+            img = torch.empty((H, W, 4), device=self.device)
+            dist = torch.empty((H, W, 5), device=self.device)
+            nc = torch.empty((H, W), device=self.device, dtype=torch.int32)
+            check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(sc.indices),
+                                         ptr(sc.vertices), ptr(sc.features), sc.features.shape[1], ptr(jit),
+                                         ptr(img), ptr(dist), ptr(nc), stream_of(self.device)), "rm_composite_fwd")
+        return (img, dist, nc) if aux else img
 
     # ------------------------------------------------------------------ buckets
     def probe_camera(self, H, W):

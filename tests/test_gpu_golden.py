@@ -3,17 +3,30 @@ binning, image within 1e-5 of the FP64 oracle, gradients within 1e-3 relative.""
 import numpy as np
 import pytest
 
-from tests._golden import load
+from tests._golden import family_cases, load, load_case
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", ["single_tet", "multi_tet"])
 def test_gpu_matches_golden(name):
+    print(f"golden {name}", flush=True)
+    _check(load(name))
+
+
+@pytest.mark.parametrize("family,case", family_cases())
+def test_gpu_matches_edge_case_golden(family, case):
+    """The restaged edge cases of the reference's tests: camera on a tet face at offsets 0 to 10
+    (looking along the face normal as the reference does, and into the tet), tets in the frustum,
+    point clouds of radius 10 and 100 seen from a point of the cloud and from outside; per-element
+    gradient errors as well as relative L2."""
+    print(f"golden {family}/{case}", flush=True)
+    _check(load_case(family, case), per_element=True)
+
+
+def _check(z, per_element=False):
     import torch
     from radiance_meshes_amd.rasterizer import AlphaBlendTiledRender, RenderGrid, VertexShader, vertex_and_tile_shader
-    z = load(name)
-    print(f"golden {name}", flush=True)
     dev = "cuda"
     verts = torch.from_numpy(z["vertices"]).to(dev).requires_grad_(True)
     idx = torch.from_numpy(z["indices"]).to(dev)
@@ -33,10 +46,20 @@ def test_gpu_matches_golden(name):
     loss = (img * torch.from_numpy(z["w_img"]).float().to(dev)).sum() + \
         (dist * torch.from_numpy(z["w_dist"]).float().to(dev)).sum()
     loss.backward()
-    err = np.abs(img.detach().cpu().numpy() - z["img"]).max()
-    print(f"image max-abs err {err:.3e}")
+    far = z["margin"] >= 1e-4     # pixels whose termination is not within FP32 noise of -5.5
+    err = np.abs(img.detach().cpu().numpy() - z["img"])[far].max() if far.any() else 0.0
+    print(f"image max-abs err {err:.3e} ({int((~far).sum())} near-threshold pixels)")
     assert err <= 1e-5 * max(1.0, np.abs(z["img"]).max())
+    assert (~far).sum() <= max(2, 0.02 * far.size)
     for got, ref, what in [(verts.grad, z["grad_vertices"], "vertices"), (feats.grad, z["grad_features"], "features")]:
-        rel = np.linalg.norm(got.cpu().numpy() - ref) / np.linalg.norm(ref)
-        print(f"grad {what} rel err {rel:.3e}")
+        g = got.cpu().numpy().astype(np.float64)
+        if not np.abs(ref).max() > 0:
+            print(f"grad {what}: all zero in the oracle; GPU max |g| {np.abs(g).max():.3e}")
+            assert np.abs(g).max() == 0
+            continue
+        rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        pe = (np.abs(g - ref) / (np.abs(ref) + 1e-4 * np.abs(ref).max())).max()
+        print(f"grad {what} rel L2 err {rel:.3e}, max per-element {pe:.3e}")
         assert rel <= 1e-3
+        if per_element:
+            assert pe <= 1e-3

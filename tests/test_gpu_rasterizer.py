@@ -7,7 +7,8 @@ pytestmark = pytest.mark.gpu
 
 from tests._util import device_inputs, f32_bits, grid_of, jitter, scene, tcam_of
 
-CASES = [("plain", 16, 7), ("plain", 4, 7), ("plain", 8, 13), ("ball", 16, 7), ("ball", 4, 13)]
+# ("plain", 16, 13) is the 10k-point workload exactly: 256x256, 16x16 tiles, 3x3 colour gradient
+CASES = [("plain", 16, 7), ("plain", 16, 13), ("plain", 4, 7), ("plain", 8, 13), ("ball", 16, 7), ("ball", 4, 13)]
 
 
 def _project(sc):
@@ -109,12 +110,25 @@ def test_composite_forward_fp64(oracle_lib, kind, tile, nf):
           f"near-threshold pixels = {int(flip.sum())}/{flip.size}")
     assert rgb_err <= 1e-5
     assert logt_err <= 1e-5
+    # near-threshold pixels: few, and off by at most what is left past the threshold (transmittance
+    # exp(-5.5 + 1e-4) times the largest colour magnitude of the scene's tets)
+    assert flip.sum() <= 0.01 * flip.size
+    if flip.any():
+        f = sc.features.astype(np.float64)
+        v = sc.vertices.astype(np.float64)[sc.indices]
+        diam = np.linalg.norm(v[:, :, None] - v[:, None], axis=-1).max((1, 2))
+        grd = np.abs(f[:, 4:]).reshape(len(f), -1, 3).sum(2).max(1)
+        cmax = (np.abs(f[:, 1:4]).max(1) + grd * diam).max()
+        e_flip = np.abs(g_img - o_img)[flip][:, :3].max()
+        print(f"near-threshold pixels: max err {e_flip:.3e} (bound {np.exp(-5.5 + 1e-4) * cmax:.3e})")
+        assert e_flip <= np.exp(-5.5 + 1e-4) * cmax + 1e-5
     d_err = (np.abs(dist.cpu().numpy() - o_dist) / np.maximum(1, np.abs(o_dist)))[~flip].max()
     print(f"distortion rel err = {d_err:.3e}")
     assert d_err <= 1e-4
 
 
-@pytest.mark.parametrize("kind,tile,nf", [("ball", 16, 7), ("ball", 4, 7), ("ball", 8, 13), ("plain", 16, 7)])
+@pytest.mark.parametrize("kind,tile,nf", [("ball", 16, 7), ("ball", 4, 7), ("ball", 8, 13), ("plain", 16, 7),
+                                          ("plain", 16, 13), ("plain", 4, 13)])
 def test_composite_backward_fp64(oracle_lib, kind, tile, nf):
     import torch
     from radiance_meshes_amd.rasterizer import AlphaBlendTiledRender, vertex_and_tile_shader
@@ -144,8 +158,12 @@ def test_composite_backward_fp64(oracle_lib, kind, tile, nf):
               "gradient": (g_f[:, 4:], gf[:, 4:])}
     for name, (a, b) in groups.items():
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
-        print(f"{name}: rel L2 err vs FP64 adjoint = {rel:.3e} (|ref| = {np.linalg.norm(b):.3e})")
+        # per element, relative to the element with a floor of 1e-4 of the group's largest value
+        pe = np.abs(a - b) / (np.abs(b) + 1e-4 * np.abs(b).max())
+        print(f"{name}: rel L2 err vs FP64 adjoint = {rel:.3e}, max per-element {pe.max():.3e} "
+              f"(|ref| = {np.linalg.norm(b):.3e})")
         assert rel <= 1e-3, name
+        assert pe.max() <= 1e-3, name
 
 
 @pytest.mark.parametrize("kind,tile,nf", [("plain", 16, 7), ("ball", 8, 13)])

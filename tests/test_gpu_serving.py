@@ -154,3 +154,48 @@ def test_batch_right_after_single_view_work():
         torch.cuda.synchronize()
         for j, c in enumerate(cams):
             _assert_same(out[j], _single_full(srv, 0, c), f"{W}x{H} view {j}")
+
+
+def test_overflow_fallback_keeps_aux_outputs():
+    """A batch whose pairs exceed the bucket and whose workspace may not grow (HBM budget) is
+    rendered view by view; with keep_aux it still returns (image, distortion, n_contrib), equal to
+    the single-view path's outputs."""
+    import torch
+    from radiance_meshes_amd.serving import RenderServer
+    srv = RenderServer(tile=4, n_features=13, min_t=0.1, hbm_budget_bytes=4 << 30, max_slots=4,
+                       use_graphs=False, pair_margin=0.05, probe_radius=6.0, on_overflow="grow")
+    sc = scene("plain", 4, 13)
+    srv.add_scene(torch.from_numpy(np.ascontiguousarray(sc.vertices, np.float32)),
+                  torch.from_numpy(np.ascontiguousarray(sc.indices, np.int32)),
+                  torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)))
+    views = [(0, scenes.orbit_camera(2.5, 40.0 * k, 10.0, RES_A[1], RES_A[0], 50.0)) for k in range(3)]
+    b = srv.bucket(*RES_A)
+    srv.budget = sum(x.bytes for x in srv.buckets.values())   # no room to grow
+    print(f"overflow fallback: bucket pair capacity {b.pair_cap}", flush=True)
+    out = srv.render_views(views, keep_aux=True)
+    torch.cuda.synchronize()
+    assert len(out) == 3
+    for (s, c), o in zip(views, out):
+        assert len(o) == 3
+        _assert_same(o, _single_full(srv, s, c), "fallback view")
+
+
+def test_add_scene_after_buckets_exist():
+    """Scenes added after a bucket was created are served too: the bucket is rebuilt for the new
+    scene set, and its views match the single-view path."""
+    import torch
+    print("add_scene after a bucket exists", flush=True)
+    srv = _server(False, max_slots=2)
+    first = srv.render_views(_views(RES_B, 2, 7), keep_aux=True)
+    torch.cuda.synchronize()
+    assert len(first) == 2
+    sc = scene("ball", 4, 13, n_points=4000, seed=5)
+    sid = srv.add_scene(torch.from_numpy(np.ascontiguousarray(sc.vertices, np.float32)),
+                        torch.from_numpy(np.ascontiguousarray(sc.indices, np.int32)),
+                        torch.from_numpy(np.ascontiguousarray(sc.features, np.float32)))
+    views = [(sid, scenes.orbit_camera(3.0, 25.0, 15.0, RES_B[1], RES_B[0], 50.0)),
+             (0, scenes.orbit_camera(3.0, 70.0, -5.0, RES_B[1], RES_B[0], 50.0))]
+    out = srv.render_views(views, keep_aux=True)
+    torch.cuda.synchronize()
+    for (s, c), o in zip(views, out):
+        _assert_same(o, _single_full(srv, s, c), f"scene {s} after add_scene")

@@ -104,10 +104,55 @@ def test_view_features():
     d = cen - cam
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     B = meshio.sh_basis(torch.from_numpy(d), 3).numpy()
-    raw = m["base"].astype(np.float64) + np.einsum("tk,tkc->tc", B, m["sh_rest"].astype(np.float64))
+    v = m["vertices"][m["indices"]].astype(np.float64)
+    off = v[:, 0] - v.mean(1)                          # the field anchored at the centroid
+    v0col = m["base"].astype(np.float64) + (m["grd"].astype(np.float64) * off).sum(1, keepdims=True)
+    raw = v0col + np.einsum("tk,tkc->tc", B, m["sh_rest"].astype(np.float64))
     col = np.log1p(np.exp(10 * raw)) / 10
     np.testing.assert_allclose(f[:, 0], m["density"], rtol=1e-6)
     np.testing.assert_allclose(f[:, 1:4], col, atol=2e-6)
     np.testing.assert_allclose(f[:, 4:7], m["grd"], rtol=1e-6)
     f0 = meshio.view_features(mesh, cam, sh_deg=0, device="cpu").double().numpy()
-    np.testing.assert_allclose(f0[:, 1:4], np.log1p(np.exp(10 * m["base"].astype(np.float64))) / 10, atol=2e-6)
+    np.testing.assert_allclose(f0[:, 1:4], np.log1p(np.exp(10 * v0col)) / 10, atol=2e-6)
+
+
+@pytest.mark.parametrize("anchor", ["centroid", "circumcenter"])
+def test_view_features_of_reference_save2ply_fields(oracle_lib, tmp_path, anchor):
+    """A PLY holding exactly the fields the reference's save2ply writes (models/base_model.py:107-151:
+    s = density, grd = offset_normalize's normalised gradient, sh_0 = RGB2SH(rgb) with rgb the raw
+    colour at the anchor, the higher SH bands as they are) renders through view_features to the
+    features the reference's activate_output (utils/model_util.py:93-105) forms from the raw field
+    outputs with the same anchor -- here the FP64 oracle restatement of activate_output."""
+    O = oracle_lib
+    from radiance_meshes_amd import scenes
+    sc = scenes.ball_scene(n_points=120, seed=4, dtype=np.float32)
+    m = {"vertices": sc.vertices, "indices": sc.indices}
+    T = len(m["indices"])
+    rng = np.random.default_rng(5)
+    rgb = rng.uniform(0.1, 0.9, (T, 3))
+    grd_raw = rng.normal(0, 0.3, (T, 3))
+    sh = rng.normal(0, 0.1, (T, 15, 3))
+    dens = rng.uniform(0.5, 3.0, T)
+    cam = np.array([0.3, -2.0, 4.0])
+    V = m["vertices"].astype(np.float64)
+    tets = V[m["indices"]]
+    if anchor == "centroid":
+        cc = tets.mean(1)
+    else:
+        cc = np.stack([O.power_of_circumsphere(*tets[t], np.zeros(3))[1] for t in range(T)])
+    ref = O.activate_output(cam, dens, rgb, grd_raw, sh, m["indices"], cc, V, 3)
+    p = tmp_path / "ckpt.ply"
+    meshio.save_mesh_ply(p, m["vertices"], m["indices"], dens, rgb, ref[:, 4:7], sh)   # save2ply's fields
+    f = meshio.view_features(meshio.load_mesh_ply(p), cam.astype(np.float32), sh_deg=3, device="cpu",
+                             anchor=anchor).double().numpy()
+    # flat tets (zero volume) have no circumcenter; compare the others
+    vol6 = np.abs(np.einsum("ij,ij->i", np.cross(tets[:, 1] - tets[:, 0], tets[:, 2] - tets[:, 0]),
+                            tets[:, 3] - tets[:, 0]))
+    ok = vol6 > 1e-9
+    assert ok.mean() > 0.99
+    err = np.abs(f - ref)[ok].max(0)
+    print(f"{anchor}: max abs err per channel {err} over {int(ok.sum())} of {T} tets")
+    assert err.max() <= 2e-5
+    # the former reading (colour at vertex 0 taken as SH2RGB(sh_0), no gradient term) differs
+    shifted = np.abs((ref[ok, 4:7] * (tets[ok, 0] - cc[ok])).sum(1)).max()
+    assert shifted > 1e-2
