@@ -319,6 +319,51 @@ int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const float* b1, 
                const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1, float* gw2,
                float* gb2, float* gw3, float* gb3, rm_stream_t stream);
 
+/* One head of the reference model's field, n_in -> 64 -> 64 -> n_out with SELU: mk_head
+ * (models/ingp_color.py:141-151) for n_in = L * n_feat in {32, 64} and n_out in [1, 48] (the
+ * reference's density, colour, gradient and SH heads are 1, 3, 3 and 45 wide). Same layout,
+ * precision and gradient contract as rm_mlp_fwd/rm_mlp_bwd, which are this entry point at
+ * n_in = 32. */
+int rm_field_head_fwd(const float* x_dev, int64_t n, int32_t n_in, const float* w1, const float* b1,
+                      const float* w2, const float* b2, const float* w3, const float* b3, int32_t n_out,
+                      int32_t precision, float* y_dev, rm_stream_t stream);
+int rm_field_head_bwd(const float* x_dev, int64_t n, int32_t n_in, const float* w1, const float* b1,
+                      const float* w2, const float* b2, const float* w3, const float* b3, int32_t n_out,
+                      int32_t precision, const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1,
+                      float* gw2, float* gb2, float* gw3, float* gb3, rm_stream_t stream);
+
+/* ---- the reference model's field glue -------------------------------------------------------
+ * rm_field_query replaces the query-point steps of Model.compute_batch_features
+ * (models/ingp_color.py:314-349) for tets [0, n_tets) of indices_dev: the circumradius of each
+ * tet in FP64 (calculate_circumcenters_torch, utils/topo_utils.py:120-164), the anchor
+ * (anchor 0: the float32 centroid, as with ablate_circumsphere; 1: the FP64 circumcenter),
+ * normalised by (center, scene_scaling) and contracted with its radius by contract_mean_std
+ * (utils/contraction.py:6-20). Writes x_dev [n,3] = (cv/2 + 1)/2 (float32; the circumcenter
+ * anchor's FP64 coordinate is rounded), cr_dev [n] (FP64, as the reference's double radius
+ * makes it) and anchor_dev [n,3] (float32). center is a host pointer to 3 floats. */
+int rm_field_query(const int32_t* indices_dev, const float* vertices_dev, int64_t n_tets, const float* center,
+                   float scene_scaling, int32_t anchor, float* x_dev, double* cr_dev, float* anchor_dev,
+                   rm_stream_t stream);
+/* The erf down-weighting of iNGPDW._encode (models/ingp_color.py:174-183): out = in * s, per
+ * element of the [n, n_levels * n_feat] encoding, s = approx_erf(1 / sqrt(per_level_scale * 4 *
+ * k * cr * scale_multi)) for column i with k = i mod n_levels (the reference views a row as
+ * [n_feat][n_levels]), computed and applied in FP64 and rounded to float32 once. s does not
+ * depend on the encoding and cr is detached, so the same call maps the encoding's gradient. */
+int rm_field_downweight(const float* in_dev, const double* cr_dev, int64_t n, int32_t n_levels, int32_t n_feat,
+                        double scale_multi, double per_level_scale, float* out_dev, rm_stream_t stream);
+/* The output activations of iNGPDW.forward (models/ingp_color.py:186-205) from the four heads'
+ * raw outputs: density = safe_exp(sigma + density_offset) (clamped to log(float32 max)),
+ * rgb + 0.5, grd / sqrt(|grd|^2 + 1) and sh rounded to FP16 (sh_out_dev holds IEEE half bits).
+ * The backward takes the saved density and raw grd; each incoming gradient may be null (zero);
+ * g_sh_dev holds FP16 bits, as the reference's half tensor receives its gradient. */
+int rm_field_outputs_fwd(const float* sigma_dev, const float* rgb_dev, const float* grd_dev, const float* sh_dev,
+                         int64_t n, int32_t n_sh, float density_offset, float* density_dev, float* rgb_out_dev,
+                         float* grd_out_dev, uint16_t* sh_out_dev, rm_stream_t stream);
+int rm_field_outputs_bwd(const float* density_dev, const float* grd_raw_dev, const float* g_density_dev,
+                         const float* g_rgb_dev, const float* g_grd_dev, const uint16_t* g_sh_dev, int64_t n,
+                         int32_t n_sh, float* g_sigma_dev, float* g_rgb_raw_dev, float* g_grd_raw_dev,
+                         float* g_sh_raw_dev, rm_stream_t stream);
+
 /* ---- field glue -----------------------------------------------------------------------------
  * Replaces the PyTorch steps around the field for a list of visible tets (tet_ids_dev, n ids):
  * rm_field_points writes the query point of each listed tet, its centroid

@@ -316,6 +316,87 @@ def mlp_fwd(x, w1, b1, w2, b2, w3, b3):
     return selu(h2) @ w3.T + b3
 
 
+# --------------------------------------------------------------------- the reference model's field
+
+def circumspheres(tets):
+    """FP64 circumcenters [T,3] and radii [T] of tets [T,4,3], by solving
+    2 (v_i - v_0) . c' = |v_i - v_0|^2 (i = 1..3) for c' = c - v_0 (not the reference's
+    cross-product form, utils/topo_utils.py:120-164, which is the same point)."""
+    t = np.asarray(tets, np.float64)
+    E = t[:, 1:] - t[:, :1]
+    rhs = 0.5 * (E * E).sum(-1)
+    rel = np.linalg.solve(E, rhs[..., None])[..., 0]
+    return t[:, 0] + rel, np.linalg.norm(rel, axis=1)
+
+
+def contract_mean_std(x, std, eps=1.1920928955078125e-07):
+    """FP64 contract_mean_std (utils/contraction.py:6-20): z = x inside the unit ball, else
+    (2 - 1/|x|) x/|x|; std scaled by det_13 = ((2|x| - 1)^(1/3) / |x|)^2 (the cube root of the
+    contraction's Jacobian determinant) outside; 2|x| - 1 clamped to [1e-6, 1e6] as safe_pow does."""
+    x = np.asarray(x, np.float64)
+    msq = np.maximum((x * x).sum(-1, keepdims=True), eps)
+    mag = np.sqrt(msq)
+    inside = msq <= 1
+    z = np.where(inside, x, ((2 * mag - 1) / msq) * x)
+    det13 = (np.cbrt(np.clip(2 * mag - 1, 1e-6, 1e6)) / mag) ** 2
+    return z, np.where(inside[..., 0], std, det13[..., 0] * std)
+
+
+def field_query(vertices, indices, center, scene_scaling, anchor="centroid"):
+    """FP64 query of Model.compute_batch_features (models/ingp_color.py:314-349, k_samples 1,
+    alpha 0): anchor (centroid or circumcenter), radius = circumradius, (x, cr) with
+    x = (cv/2 + 1)/2 of the contracted normalised anchor. Returns (x [T,3], cr [T], anchor [T,3])."""
+    tets = np.asarray(vertices, np.float64)[np.asarray(indices, np.int64)]
+    cc, radius = circumspheres(tets)
+    anc = tets.mean(1) if anchor == "centroid" else cc
+    c = np.asarray(center, np.float64).reshape(1, 3)
+    z, cr = contract_mean_std((anc - c) / scene_scaling, radius / scene_scaling)
+    return (z / 2 + 1) / 2, cr, anc
+
+
+def approx_erf(x):
+    """sign(x) sqrt(1 - exp(-(4/pi) x^2)) (models/ingp_color.py:84-86)."""
+    return np.sign(x) * np.sqrt(1 - np.exp(-(4 / np.pi) * x * x))
+
+
+def downweight_scale(cr, n_levels, scale_multi, per_level_scale):
+    """[N, L] scales of iNGPDW._encode (models/ingp_color.py:174-183): approx_erf(safe_div(1,
+    safe_sqrt(per_level_scale * 4 * n * cr * scale_multi))) for n = 0..L-1; safe_sqrt clamps its
+    input to at least float32 tiny, so n = 0 gives 1/sqrt(tiny) and a scale of exactly 1."""
+    tiny = float(np.finfo(np.float32).tiny)
+    n = np.arange(n_levels, dtype=np.float64)[None, :]
+    d = np.maximum(per_level_scale * 4 * n * (np.asarray(cr, np.float64).reshape(-1, 1) * scale_multi), tiny)
+    return approx_erf(1.0 / np.sqrt(d))
+
+
+def downweight(enc, cr, n_levels, n_feat, scale_multi, per_level_scale):
+    """enc [N, L*F] level-major, viewed as [N, F, L] (the reference's reshape(-1, dim, L)) and
+    scaled along the last axis."""
+    e = np.asarray(enc, np.float64).reshape(-1, n_feat, n_levels)
+    s = downweight_scale(cr, n_levels, scale_multi, per_level_scale)
+    return (e * s[:, None, :]).reshape(-1, n_levels * n_feat)
+
+
+def field_outputs(sigma, rgb, grd, sh, density_offset):
+    """iNGPDW.forward's activations (models/ingp_color.py:196-205), FP64: exp(sigma + offset)
+    (input clamped above at log(float32 max)), rgb + 0.5, grd / sqrt(|grd|^2 + 1); sh unrounded."""
+    s = np.minimum(np.asarray(sigma, np.float64) + density_offset, np.log(np.finfo(np.float32).max))
+    g = np.asarray(grd, np.float64)
+    return np.exp(s), np.asarray(rgb, np.float64) + 0.5, g / np.sqrt((g * g).sum(-1, keepdims=True) + 1), \
+        np.asarray(sh, np.float64)
+
+
+def ingp_forward(x, cr, table, res, offs, n_feat, log2_size, heads, scale_multi, per_level_scale,
+                 density_offset, ablate_downweighing=False):
+    """FP64 iNGPDW.forward: hash grid -> down-weighting -> the four heads (density, colour,
+    gradient, SH; each a list [w1, b1, w2, b2, w3, b3]) -> activations."""
+    enc = hashgrid_fwd(x, table, res, offs, n_feat, log2_size)
+    if not ablate_downweighing:
+        enc = downweight(enc, cr, len(res), n_feat, scale_multi, per_level_scale)
+    raw = [mlp_fwd(enc, *[np.asarray(a, np.float64) for a in h]) for h in heads]
+    return field_outputs(*raw, density_offset)
+
+
 # --------------------------------------------------------------------- per-tet feature activation
 
 def _sh_real_scipy(dirs, deg):

@@ -86,6 +86,13 @@ SIGNATURES = {
     "rm_mlp_fwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "rm_mlp_bwd": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _P, _P,
                              _P, _P]),
+    "rm_field_head_fwd": (C.c_int, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+    "rm_field_head_bwd": (C.c_int, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P, _P,
+                                    _P, _P, _P]),
+    "rm_field_query": (C.c_int, [_P, _P, _I64, _P, C.c_float, _I32, _P, _P, _P, _P]),
+    "rm_field_downweight": (C.c_int, [_P, _P, _I64, _I32, _I32, C.c_double, C.c_double, _P, _P]),
+    "rm_field_outputs_fwd": (C.c_int, [_P, _P, _P, _P, _I64, _I32, C.c_float, _P, _P, _P, _P, _P]),
+    "rm_field_outputs_bwd": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P]),
     "rm_field_points": (C.c_int, [_P, _P, _P, _I64, C.c_float, _P, _P]),
     "rm_field_scatter_fwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P]),
     "rm_field_scatter_bwd": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P]),

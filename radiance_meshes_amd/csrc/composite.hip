@@ -195,17 +195,38 @@ __device__ inline float expm1_neg(float x) {
     return x <= 0.25f ? p : e;
 }
 
-// phi'(x): series below 0.05 (truncation < 1e-9 relative), closed form above (quotient through
-// the hardware reciprocal, 1 ulp).
-__device__ inline float dphi_of(float x, float em1, float alpha) {
-    const float series = -0.5f + x * (1.f / 3.f + x * (-1.f / 8.f + x * (1.f / 30.f - x * (1.f / 144.f))));
-    const float closed = (x * alpha + em1) * rcp_(x * x);
-    return x < 1e-6f ? -0.5f : (x < 0.05f ? series : closed);
+
+// Weights of the exit and entry colours in the segment integral, w0 = phi(x) - e^-x and
+// w1 = 1 - phi(x) with phi(x) = (1 - e^-x) / x (interp_version.slang:21-69; their sum is 1 - e^-x). Both are O(x): formed as phi - alpha and 1 - phi they
+// lose about eps / x of relative precision (1% at x = 1e-5, a thin sliver of a crossing), so below
+// x = 0.05 they come from their own series, w1 = sum (-1)^(k+1) x^k / (k+1)! and
+// w0 = sum (-1)^(k+1) k x^k / (k+1)!, to x^5 (truncation under 1e-9 relative).
+__device__ inline void seg_weights(float x, float em1, float alpha, float& w0, float& w1) {
+    if (x < 0.05f) {
+        w1 = x * (0.5f - x * (1.f / 6.f - x * (1.f / 24.f - x * (1.f / 120.f - x * (1.f / 720.f)))));
+        w0 = x * (0.5f - x * (1.f / 3.f - x * (1.f / 8.f - x * (1.f / 30.f - x * (1.f / 144.f)))));
+    } else {
+        const float ph = -em1 * rcp_(x);
+        w0 = ph - alpha;
+        w1 = 1.0f - ph;
+    }
 }
 
-// phi(x) = (1 - e^-x) / x with the reference's small-x branch (interp_version.slang:21-28), the
-// quotient through the hardware reciprocal (1 ulp).
-__device__ inline float phi_of_rcp(float x, float em1) { return x < 1e-6f ? 1.0f - 0.5f * x : -em1 * rcp_(x); }
+// seg_weights and phi'(x) for the backward, on one branch: series below 0.05 (phi' truncated after
+// x^4, under 1e-9 relative), closed forms above through one hardware reciprocal (1 ulp).
+__device__ inline void seg_weights_dphi(float x, float em1, float alpha, float& w0, float& w1, float& dphi) {
+    if (x < 0.05f) {
+        w1 = x * (0.5f - x * (1.f / 6.f - x * (1.f / 24.f - x * (1.f / 120.f - x * (1.f / 720.f)))));
+        w0 = x * (0.5f - x * (1.f / 3.f - x * (1.f / 8.f - x * (1.f / 30.f - x * (1.f / 144.f)))));
+        dphi = -0.5f + x * (1.f / 3.f + x * (-1.f / 8.f + x * (1.f / 30.f - x * (1.f / 144.f))));
+    } else {
+        const float r = rcp_(x);
+        const float ph = -em1 * r;
+        w0 = ph - alpha;
+        w1 = 1.0f - ph;
+        dphi = (x * alpha + em1) * (r * r);
+    }
+}
 
 // DPP lane permute of a 64-bit bit mask within a row of 16 lanes.
 template <int CTRL>
@@ -370,8 +391,8 @@ __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, fl
     read_grads(S, slot, gr);
     const float em1 = expm1_neg(x);
     const float alpha = 1.0f + em1;
-    const float ph = phi_of_rcp(x, em1);
-    const float w0 = ph - alpha, w1 = 1.0f - ph;
+    float w0, w1;
+    seg_weights(x, em1, alpha, w0, w1);
     const float T = __expf(logT);
     const float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
     float gd1 = gd0, gd2 = gd0;
@@ -795,8 +816,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                     read_grads(S, slot, gr);
                     float em1 = expm1_neg(x);
                     float alpha = 1.0f + em1;
-                    float ph = phi_of_rcp(x, em1);
-                    float w0 = ph - alpha, w1 = 1.0f - ph;
+                    float w0, w1, dphi;
+                    seg_weights_dphi(x, em1, alpha, w0, w1, dphi);
                     float gd[3];
                     gd[0] = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
                     gd[1] = gd[0];
@@ -854,7 +875,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         dw0 += drgb[ch] * cx[ch];
                         dw1 += drgb[ch] * ce[ch];
                     }
-                    dxx += alpha * dw0 + (dw0 - dw1) * dphi_of(x, em1, alpha);
+                    dxx += alpha * dw0 + (dw0 - dw1) * dphi;
                     dsig += dxx * dt;
                     ddt += dxx * sig;
                     dsx += ddt;

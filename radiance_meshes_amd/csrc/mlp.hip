@@ -1,4 +1,5 @@
-// Field MLP 32 -> 64 -> 64 -> n_out (n_out <= 16) with SELU hidden activations, on MFMA.
+// Field MLP heads n_in -> 64 -> 64 -> n_out (n_in 32 or 64, n_out <= 48) with SELU hidden
+// activations, on MFMA.
 //
 // Reference head: Linear -> SELU -> Linear -> SELU -> Linear (models/ingp_color.py:141-151),
 // weights row-major [out, in] as torch.nn.Linear. Every product is a chain of 16x16 MFMA tiles
@@ -18,7 +19,7 @@
 // of block s/4 (or s). Activations never leave registers; the input rows come from global memory
 // in that layout (two float4 per lane) and dX leaves the same way.
 // The backward recomputes the forward per tile (as the reference's checkpointed field) keeping
-// the hidden activations (selu' is a function of them), chains dY -> dH2 -> dH1 -> dX through
+// the hidden pre-activations (selu' is taken from them), chains dY -> dH2 -> dH1 -> dX through
 // the transposed weights, and forms the weight gradients
 // gW = dH act^T as MFMA tiles contracted over the tile's points; those need the point index on
 // the lanes' k, so dH and act blocks are transposed through a 16x17 LDS block per 16 units.
@@ -34,13 +35,20 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 
-constexpr int kIn = 32, kHid = 64, kOutMax = 16;
+// Shapes: KI input k-tiles of 16 (n_in = 16 KI: 32 for the training-step field, 64 for the
+// reference model's 8 x 8 encoding), hidden width 64, KO output blocks of 16 (n_out <= 16 KO).
+constexpr int kHid = 64;
 constexpr int kMaxDevices = 64;
 constexpr int kWaves = 4, kThreads = 64 * kWaves;
-constexpr int SW1 = kIn + 1, SW2 = kHid + 1, SW3 = kHid + 1;   // LDS row strides (floats)
+constexpr int SW2 = kHid + 1, SW3 = kHid + 1;                  // LDS row strides (floats)
 constexpr int kTS = 17, kTBlock = 16 * kTS;                    // transpose block (16 x 17 floats)
 constexpr int kTBlocks = 8;                                    // transpose blocks per wave
-constexpr int kNGrad = kOutMax * kHid + kHid * kHid + kHid * kIn + kHid + kHid + kOutMax;
+template <int KI, int KO>
+struct Shape {
+    static constexpr int IN = 16 * KI, OUT = 16 * KO, SW1 = IN + 1;
+    static constexpr int NGRAD = OUT * kHid + kHid * kHid + kHid * IN + kHid + kHid + OUT;
+    static constexpr int TB = kWaves * kTBlocks * kTBlock > NGRAD ? kWaves * kTBlocks * kTBlock : NGRAD;
+};
 
 constexpr float kSeluAlpha = 1.6732632423543772f;
 constexpr float kSeluScale = 1.0507009873554805f;
@@ -57,14 +65,31 @@ __device__ inline void wave_sync() {
 // (the result is at least 0.22 in magnitude there, so the exponential's ~1e-7 error stays ~3e-7
 // relative). Six instructions beside the exponential, where expm1f takes 24; this matters here
 // (32 SELUs per lane per 16-query tile next to the MFMAs).
-__device__ inline float expm1_nonpos(float x) {
+// expm1_from takes the exponential e = e^x already formed.
+__device__ inline float expm1_from(float x, float e) {
     const float q = 0.49999952f + x * (0.16665113f + x * (0.041491002f + x * 0.0075143012f));
     const float p = x + (x * x) * q;
-    return x >= -0.25f ? p : __expf(x) - 1.0f;
+    return x >= -0.25f ? p : e - 1.0f;
 }
+__device__ inline float expm1_nonpos(float x) { return expm1_from(x, __expf(x)); }
 __device__ inline float selu(float x) { return kSeluScale * (x > 0.f ? x : kSeluAlpha * expm1_nonpos(x)); }
-// selu'(h) from a = selu(h): a has the sign of h, and for h <= 0 selu'(h) = scale alpha e^h = a + scale alpha
-__device__ inline float dselu_of_act(float a) { return a > 0.f ? kSeluScale : a + kSeluScale * kSeluAlpha; }
+// selu'(h) from the pre-activation, scale alpha e^h for h <= 0 as torch's SELU backward takes it.
+// (Taking it from a = selu(h) as a + scale alpha would cancel: for h << 0 the float a keeps e^h
+// only to the absolute precision of scale alpha, 1e-3 relative at h = -10.)
+// selu(h) and selu'(h) of a block from one exponential per element: selu' is scale alpha e^h for
+// h <= 0, as torch's SELU backward takes it from the pre-activation. (From a = selu(h) as
+// a + scale alpha it would cancel: for h << 0 the float a holds e^h only to the absolute precision
+// of scale alpha, 1e-3 relative at h = -10.)
+__device__ inline void selu_pair(f4 h, f4& a, f4& d) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float x = h[r];
+        const float e = __expf(x);
+        const float em1 = expm1_from(x, e);
+        a[r] = x > 0.f ? kSeluScale * x : kSeluScale * (kSeluAlpha * em1);
+        d[r] = x > 0.f ? kSeluScale : (kSeluScale * kSeluAlpha) * e;
+    }
+}
 
 __device__ inline short to_bf16(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -133,18 +158,22 @@ __device__ inline void get_block(const float* tb, float (&out)[4]) {
     for (int s = 0; s < 4; ++s) out[s] = tb[c * kTS + 4 * q + s];
 }
 
+template <int KI, int KO>
 struct Smem {
-    float w1[kHid * SW1];
+    using Sh = Shape<KI, KO>;
+    float w1[kHid * Sh::SW1];
     float w2[kHid * SW2];
-    float w3[kOutMax * SW3];
-    float b1[kHid], b2[kHid], b3[kOutMax];
+    float w3[Sh::OUT * SW3];
+    float b1[kHid], b2[kHid], b3[Sh::OUT];
 };
 
-__device__ void load_weights(Smem& S, const float* w1, const float* b1, const float* w2, const float* b2,
+template <int KI, int KO>
+__device__ void load_weights(Smem<KI, KO>& S, const float* w1, const float* b1, const float* w2, const float* b2,
                              const float* w3, const float* b3, int n_out) {
-    for (int i = threadIdx.x; i < kHid * kIn; i += blockDim.x) S.w1[(i / kIn) * SW1 + i % kIn] = w1[i];
+    using Sh = Shape<KI, KO>;
+    for (int i = threadIdx.x; i < kHid * Sh::IN; i += blockDim.x) S.w1[(i / Sh::IN) * Sh::SW1 + i % Sh::IN] = w1[i];
     for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) S.w2[(i / kHid) * SW2 + i % kHid] = w2[i];
-    for (int i = threadIdx.x; i < kOutMax * kHid; i += blockDim.x) {
+    for (int i = threadIdx.x; i < Sh::OUT * kHid; i += blockDim.x) {
         int o = i / kHid;
         S.w3[o * SW3 + i % kHid] = o < n_out ? w3[i] : 0.f;
     }
@@ -152,16 +181,17 @@ __device__ void load_weights(Smem& S, const float* w1, const float* b1, const fl
         S.b1[i] = b1[i];
         S.b2[i] = b2[i];
     }
-    for (int i = threadIdx.x; i < kOutMax; i += blockDim.x) S.b3[i] = i < n_out ? b3[i] : 0.f;
+    for (int i = threadIdx.x; i < Sh::OUT; i += blockDim.x) S.b3[i] = i < n_out ? b3[i] : 0.f;
 }
 
 // Input rows of the tile in the chain layout: xin[kt][r] = x[point c][16 kt + 4q + r].
-__device__ inline void load_x_chain(const float* __restrict__ x, int64_t p, int64_t n, f4 (&xin)[2]) {
+template <int KI>
+__device__ inline void load_x_chain(const float* __restrict__ x, int64_t p, int64_t n, f4 (&xin)[KI]) {
     const int lane = threadIdx.x & 63, q = lane >> 4;
 #pragma unroll
-    for (int kt = 0; kt < 2; ++kt) {
+    for (int kt = 0; kt < KI; ++kt) {
         if (p < n) {
-            const float4 v = *reinterpret_cast<const float4*>(x + p * kIn + 16 * kt + 4 * q);
+            const float4 v = *reinterpret_cast<const float4*>(x + p * (16 * KI) + 16 * kt + 4 * q);
             xin[kt] = f4{v.x, v.y, v.z, v.w};
         } else {
             xin[kt] = f4{0.f, 0.f, 0.f, 0.f};
@@ -169,124 +199,162 @@ __device__ inline void load_x_chain(const float* __restrict__ x, int64_t p, int6
     }
 }
 
-// Forward of one tile: hidden activations a1, a2 (chain layout) and the output block. The
-// backward takes selu' from the activations (dselu_of_act), so no pre-activation is kept.
-template <int P>
-__device__ inline f4 tile_forward(const Smem& S, const f4 (&xin)[2], f4 (&a1)[4], f4 (&a2)[4]) {
+// Forward of one tile in the chain layout and, with OUT, the output blocks. h1, h2 receive the
+// hidden activations selu(H), or with PRE the pre-activations H (the backward takes selu' from H
+// and recomputes selu(H) where it needs the activation).
+template <int P, int KI, int KO, bool OUT, bool PRE = false>
+__device__ inline void tile_forward(const Smem<KI, KO>& S, const f4 (&xin)[KI], f4 (&h1)[4], f4 (&h2)[4],
+                                    f4 (&y)[KO]) {
+    constexpr int SW1 = Shape<KI, KO>::SW1;
     const int lane = threadIdx.x & 63, q = lane >> 4;
+    f4 a1[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
-        f4 d = chain<P, 2>(xin, [&](int i, int k) { return S.w1[(16 * ot + i) * SW1 + k]; });
+        f4 d = chain<P, KI>(xin, [&](int i, int k) { return S.w1[(16 * ot + i) * SW1 + k]; });
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a1[ot][r] = selu(d[r] + S.b1[16 * ot + 4 * q + r]);
+        for (int r = 0; r < 4; ++r) {
+            const float h = d[r] + S.b1[16 * ot + 4 * q + r];
+            a1[ot][r] = selu(h);
+            h1[ot][r] = PRE ? h : a1[ot][r];
+        }
     }
+    f4 a2[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
         f4 d = chain<P, 4>(a1, [&](int i, int k) { return S.w2[(16 * ot + i) * SW2 + k]; });
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a2[ot][r] = selu(d[r] + S.b2[16 * ot + 4 * q + r]);
+        for (int r = 0; r < 4; ++r) {
+            const float h = d[r] + S.b2[16 * ot + 4 * q + r];
+            a2[ot][r] = PRE ? 0.f : selu(h);
+            h2[ot][r] = PRE ? h : a2[ot][r];
+        }
     }
-    f4 y = chain<P, 4>(a2, [&](int i, int k) { return S.w3[i * SW3 + k]; });
+    if constexpr (!OUT) return;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] += S.b3[4 * q + r];
-    return y;
+    for (int ob = 0; ob < KO; ++ob) {
+        y[ob] = chain<P, 4>(a2, [&](int i, int k) { return S.w3[(16 * ob + i) * SW3 + k]; });
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y[ob][r] += S.b3[16 * ob + 4 * q + r];
+    }
 }
 
-template <int P>
+template <int P, int KI, int KO>
 __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(const float* __restrict__ x, int64_t n,
                                                            const float* w1, const float* b1, const float* w2,
                                                            const float* b2, const float* w3, const float* b3,
                                                            int n_out, float* __restrict__ y) {
-    __shared__ Smem S;
-    load_weights(S, w1, b1, w2, b2, w3, b3, n_out);
+    __shared__ Smem<KI, KO> S;
+    load_weights<KI, KO>(S, w1, b1, w2, b2, w3, b3, n_out);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
     const int64_t ntiles = (n + 15) / 16;
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
         asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
         const int64_t p = t * 16 + c;
-        f4 xin[2], a1[4], a2[4];
-        load_x_chain(x, p, n, xin);
-        const f4 out = tile_forward<P>(S, xin, a1, a2);
+        f4 xin[KI], a1[4], a2[4], out[KO];
+        load_x_chain<KI>(x, p, n, xin);
+        tile_forward<P, KI, KO, true>(S, xin, a1, a2, out);
         if (p < n) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int o = 4 * q + r;
-                if (o < n_out) y[p * n_out + o] = out[r];
-            }
+            for (int ob = 0; ob < KO; ++ob)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int o = 16 * ob + 4 * q + r;
+                    if (o < n_out) y[p * n_out + o] = out[ob][r];
+                }
         }
     }
 }
 
-template <int P>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void mlp_bwd_kernel(
+// KI * KO > 2 (the 64-input, up-to-48-output head): the weight-gradient tiles take more than 256
+// registers, so the kernel runs one wave per SIMD with the accumulators beside the VGPRs.
+template <int P, int KI, int KO>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * KO > 2 ? 1 : 2))) void mlp_bwd_kernel(
     const float* __restrict__ x, int64_t n, const float* w1, const float* b1, const float* w2, const float* b2,
     const float* w3, const float* b3, int n_out, const float* __restrict__ gy, float* __restrict__ gx,
     float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3) {
-    __shared__ Smem S;
-    __shared__ float TB[kWaves * kTBlocks * kTBlock];
-    static_assert(kWaves * kTBlocks * kTBlock >= kNGrad, "gradient sums reuse the transpose blocks");
-    load_weights(S, w1, b1, w2, b2, w3, b3, n_out);
+    using Sh = Shape<KI, KO>;
+    constexpr int SW1 = Sh::SW1, IN = Sh::IN, OUT = Sh::OUT;
+    __shared__ Smem<KI, KO> S;
+    __shared__ float TB[Sh::TB];
+    static_assert(Sh::TB >= Sh::NGRAD && Sh::TB >= kWaves * kTBlocks * kTBlock, "reduction buffer too small");
+    load_weights<KI, KO>(S, w1, b1, w2, b2, w3, b3, n_out);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
     float* tb = TB + wave * kTBlocks * kTBlock;
-    // weight-gradient tiles (D layout): gW3 [16 x 64], gW2 [64 x 64], gW1 [64 x 32]; bias partials
-    // per lane for the units it holds in the chain layout
-    f4 g3[4], g2[4][4], g1[4][2];
-    // bias partials from the point fragments: lane (q, c) sums unit 16*ot + c over its points 4q + s
-    float gb1s[4], gb2s[4], gb3s = 0.f;
+    // weight-gradient tiles (D layout): gW3 [OUT x 64], gW2 [64 x 64], gW1 [64 x IN]; bias partials
+    // from the point fragments: lane (q, c) sums unit 16*ot + c over its points 4q + s
+    f4 g3[KO][4], g2[4][4], g1[4][KI];
+    float gb1s[4], gb2s[4], gb3s[KO];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        g3[a] = f4{0.f, 0.f, 0.f, 0.f};
         gb1s[a] = 0.f;
         gb2s[a] = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) g2[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int b = 0; b < 2; ++b) g1[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < KI; ++b) g1[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ob = 0; ob < KO; ++ob) g3[ob][a] = f4{0.f, 0.f, 0.f, 0.f};
     }
+#pragma unroll
+    for (int ob = 0; ob < KO; ++ob) gb3s[ob] = 0.f;
     const int64_t ntiles = (n + 15) / 16;
     for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
         asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
         const int64_t p = t * 16 + c;
-        f4 xin[2], a1[4], a2[4];
-        load_x_chain(x, p, n, xin);
-        tile_forward<P>(S, xin, a1, a2);
-        // dY in the chain layout (unit 4q + r of point c) and as a point fragment (unit c, points 4q + s)
-        f4 dyc[1];
-        float dyp[4];
+        f4 xin[KI], h1[4], h2[4], unused[KO];
+        load_x_chain<KI>(x, p, n, xin);
+        tile_forward<P, KI, KO, false, true>(S, xin, h1, h2, unused);
+        // dY in the chain layout (unit 16 ob + 4q + r of point c) and as point fragments (unit
+        // 16 ob + c, points 4q + s)
+        f4 dyc[KO];
+        float dyp[KO][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int o = 4 * q + r;
-            dyc[0][r] = (p < n && o < n_out) ? gy[p * n_out + o] : 0.f;
-            const int64_t pp = t * 16 + 4 * q + r;
-            dyp[r] = (pp < n && c < n_out) ? gy[pp * n_out + c] : 0.f;
+        for (int ob = 0; ob < KO; ++ob) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 16 * ob + 4 * q + r;
+                dyc[ob][r] = (p < n && o < n_out) ? gy[p * n_out + o] : 0.f;
+                const int64_t pp = t * 16 + 4 * q + r;
+                const int oc = 16 * ob + c;
+                dyp[ob][r] = (pp < n && oc < n_out) ? gy[pp * n_out + oc] : 0.f;
+            }
+            gb3s[ob] += (dyp[ob][0] + dyp[ob][1]) + (dyp[ob][2] + dyp[ob][3]);
         }
-        gb3s += (dyp[0] + dyp[1]) + (dyp[2] + dyp[3]);
         // gW3 += dY A2^T
+        f4 d2[4];
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) put_block(tb + kt * kTBlock, a2[kt]);
+        for (int kt = 0; kt < 4; ++kt) {
+            f4 a;
+            selu_pair(h2[kt], a, d2[kt]);
+            put_block(tb + kt * kTBlock, a);
+        }
         wave_sync();
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
             float a2p[4];
             get_block(tb + kt * kTBlock, a2p);
-            g3[kt] = outer<P>(dyp, a2p, g3[kt]);
+#pragma unroll
+            for (int ob = 0; ob < KO; ++ob) g3[ob][kt] = outer<P>(dyp[ob], a2p, g3[ob][kt]);
         }
         // dH2 = (W3^T dY) * selu'(H2)
         f4 dh2[4];
 #pragma unroll
         for (int ot = 0; ot < 4; ++ot) {
-            f4 d = chain<P, 1>(dyc, [&](int i, int k) { return S.w3[k * SW3 + 16 * ot + i]; });
+            f4 d = chain<P, KO>(dyc, [&](int i, int k) { return S.w3[k * SW3 + 16 * ot + i]; });
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * dselu_of_act(a2[ot][r]);
+            for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * d2[ot][r];
         }
         wave_sync();
         // gW2 += dH2 A1^T
+        f4 d1[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
+            f4 a;
+            selu_pair(h1[b], a, d1[b]);
             put_block(tb + b * kTBlock, dh2[b]);
-            put_block(tb + (4 + b) * kTBlock, a1[b]);
+            put_block(tb + (4 + b) * kTBlock, a);
         }
         wave_sync();
         {
@@ -309,14 +377,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         for (int ot = 0; ot < 4; ++ot) {
             f4 d = chain<P, 4>(dh2, [&](int i, int k) { return S.w2[k * SW2 + 16 * ot + i]; });
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * dselu_of_act(a1[ot][r]);
+            for (int r = 0; r < 4; ++r) dh1[ot][r] = d[r] * d1[ot][r];
         }
         // dX = W1^T dH1
         if (gx) {
 #pragma unroll
-            for (int ot = 0; ot < 2; ++ot) {
+            for (int ot = 0; ot < KI; ++ot) {
                 f4 d = chain<P, 4>(dh1, [&](int i, int k) { return S.w1[k * SW1 + 16 * ot + i]; });
-                if (p < n) *reinterpret_cast<float4*>(gx + p * kIn + 16 * ot + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
+                if (p < n) *reinterpret_cast<float4*>(gx + p * IN + 16 * ot + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
             }
         }
         wave_sync();
@@ -325,13 +393,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         for (int b = 0; b < 4; ++b) put_block(tb + b * kTBlock, dh1[b]);
         wave_sync();
         {
-            float xp[2][4];
+            float xp[KI][4];
 #pragma unroll
-            for (int kt = 0; kt < 2; ++kt)
+            for (int kt = 0; kt < KI; ++kt)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int64_t pp = t * 16 + 4 * q + s;
-                    xp[kt][s] = pp < n ? x[pp * kIn + 16 * kt + c] : 0.f;
+                    xp[kt][s] = pp < n ? x[pp * IN + 16 * kt + c] : 0.f;
                 }
 #pragma unroll
             for (int ot = 0; ot < 4; ++ot) {
@@ -339,32 +407,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                 get_block(tb + ot * kTBlock, dh1p);
                 gb1s[ot] += (dh1p[0] + dh1p[1]) + (dh1p[2] + dh1p[3]);
 #pragma unroll
-                for (int kt = 0; kt < 2; ++kt) g1[ot][kt] = outer<P>(dh1p, xp[kt], g1[ot][kt]);
+                for (int kt = 0; kt < KI; ++kt) g1[ot][kt] = outer<P>(dh1p, xp[kt], g1[ot][kt]);
             }
         }
         wave_sync();
     }
     // sum the waves' gradients in LDS (over the transpose blocks), then one global atomic per element
     __syncthreads();
-    float* G3 = TB;                       // [16][64]
-    float* G2 = G3 + kOutMax * kHid;      // [64][64]
-    float* G1 = G2 + kHid * kHid;         // [64][32]
-    float* B1 = G1 + kHid * kIn;
+    float* G3 = TB;                       // [OUT][64]
+    float* G2 = G3 + OUT * kHid;          // [64][64]
+    float* G1 = G2 + kHid * kHid;         // [64][IN]
+    float* B1 = G1 + kHid * IN;
     float* B2 = B1 + kHid;
     float* B3 = B2 + kHid;
-    for (int i = threadIdx.x; i < kNGrad; i += blockDim.x) TB[i] = 0.f;
+    for (int i = threadIdx.x; i < Sh::NGRAD; i += blockDim.x) TB[i] = 0.f;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int o = 4 * q + r;
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) atomicAdd(&G3[o * kHid + 16 * kt + c], g3[kt][r]);
+        for (int ob = 0; ob < KO; ++ob)
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) atomicAdd(&G3[(16 * ob + o) * kHid + 16 * kt + c], g3[ob][kt][r]);
 #pragma unroll
         for (int ot = 0; ot < 4; ++ot) {
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt) atomicAdd(&G2[(16 * ot + o) * kHid + 16 * kt + c], g2[ot][kt][r]);
 #pragma unroll
-            for (int kt = 0; kt < 2; ++kt) atomicAdd(&G1[(16 * ot + o) * kIn + 16 * kt + c], g1[ot][kt][r]);
+            for (int kt = 0; kt < KI; ++kt) atomicAdd(&G1[(16 * ot + o) * IN + 16 * kt + c], g1[ot][kt][r]);
         }
     }
 #pragma unroll
@@ -372,19 +442,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         atomicAdd(&B1[16 * ot + c], gb1s[ot]);
         atomicAdd(&B2[16 * ot + c], gb2s[ot]);
     }
-    atomicAdd(&B3[c], gb3s);
+#pragma unroll
+    for (int ob = 0; ob < KO; ++ob) atomicAdd(&B3[16 * ob + c], gb3s[ob]);
     __syncthreads();
-    for (int i = threadIdx.x; i < kNGrad; i += blockDim.x) {
+    for (int i = threadIdx.x; i < Sh::NGRAD; i += blockDim.x) {
         const float v = TB[i];
         if (v == 0.f) continue;
-        if (i < kOutMax * kHid) {
+        if (i < OUT * kHid) {
             if (i / kHid < n_out) unsafeAtomicAdd(gw3 + i, v);
-        } else if (i < kOutMax * kHid + kHid * kHid) {
-            unsafeAtomicAdd(gw2 + (i - kOutMax * kHid), v);
-        } else if (i < kOutMax * kHid + kHid * kHid + kHid * kIn) {
-            unsafeAtomicAdd(gw1 + (i - kOutMax * kHid - kHid * kHid), v);
+        } else if (i < OUT * kHid + kHid * kHid) {
+            unsafeAtomicAdd(gw2 + (i - OUT * kHid), v);
+        } else if (i < OUT * kHid + kHid * kHid + kHid * IN) {
+            unsafeAtomicAdd(gw1 + (i - OUT * kHid - kHid * kHid), v);
         } else {
-            const int b = i - (kOutMax * kHid + kHid * kHid + kHid * kIn);
+            const int b = i - (OUT * kHid + kHid * kHid + kHid * IN);
             if (b < kHid) unsafeAtomicAdd(gb1 + b, v);
             else if (b < 2 * kHid) unsafeAtomicAdd(gb2 + (b - kHid), v);
             else if (b - 2 * kHid < n_out) unsafeAtomicAdd(gb3 + (b - 2 * kHid), v);
@@ -419,7 +490,100 @@ inline unsigned grid_for(int64_t n, int cap) {
     return (unsigned)(blocks < cap ? blocks : cap);
 }
 
-std::atomic<int> g_resident[4][kMaxDevices];   // mlp_fwd<0>, mlp_fwd<1>, mlp_bwd<0>, mlp_bwd<1>
+// resident-workgroup caches: [shape][fwd/bwd][precision], shapes (KI, KO) = (2, 1), (2, 3), (4, 1), (4, 3)
+std::atomic<int> g_resident[4][2][2][kMaxDevices];
+
+template <int KI, int KO>
+constexpr int shape_slot() { return (KI == 4 ? 2 : 0) + (KO == 3 ? 1 : 0); }
+
+template <int P, int KI, int KO>
+void launch_fwd(const float* x, int64_t n, const float* w1, const float* b1, const float* w2, const float* b2,
+                const float* w3, const float* b3, int n_out, float* y, hipStream_t st) {
+    auto* cache = g_resident[shape_slot<KI, KO>()][0][P];
+    dim3 grid(grid_for(n, resident_blocks(reinterpret_cast<const void*>(&mlp_fwd_kernel<P, KI, KO>), cache, 1024)));
+    hipLaunchKernelGGL((mlp_fwd_kernel<P, KI, KO>), grid, dim3(kThreads), 0, st, x, n, w1, b1, w2, b2, w3, b3, n_out,
+                       y);
+}
+
+template <int P, int KI, int KO>
+void launch_bwd(const float* x, int64_t n, const float* w1, const float* b1, const float* w2, const float* b2,
+                const float* w3, const float* b3, int n_out, const float* gy, float* gx, float* gw1, float* gb1,
+                float* gw2, float* gb2, float* gw3, float* gb3, hipStream_t st) {
+    auto* cache = g_resident[shape_slot<KI, KO>()][1][P];
+    dim3 grid(grid_for(n, resident_blocks(reinterpret_cast<const void*>(&mlp_bwd_kernel<P, KI, KO>), cache, 512)));
+    hipLaunchKernelGGL((mlp_bwd_kernel<P, KI, KO>), grid, dim3(kThreads), 0, st, x, n, w1, b1, w2, b2, w3, b3, n_out,
+                       gy, gx, gw1, gb1, gw2, gb2, gw3, gb3);
+}
+
+template <int KI, int KO>
+void dispatch_fwd(int precision, const float* x, int64_t n, const float* w1, const float* b1, const float* w2,
+                  const float* b2, const float* w3, const float* b3, int n_out, float* y, hipStream_t st) {
+    if (precision == 0) launch_fwd<0, KI, KO>(x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+    else launch_fwd<1, KI, KO>(x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+}
+
+template <int KI, int KO>
+void dispatch_bwd(int precision, const float* x, int64_t n, const float* w1, const float* b1, const float* w2,
+                  const float* b2, const float* w3, const float* b3, int n_out, const float* gy, float* gx,
+                  float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3, hipStream_t st) {
+    if (precision == 0) launch_bwd<0, KI, KO>(x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+    else launch_bwd<1, KI, KO>(x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+}
+
+// Argument checks of the head entry points; the message names the entry point.
+#define HEAD_CHECK(cond, msg)                                 \
+    do {                                                      \
+        if (!(cond)) {                                        \
+            ::rmesh::set_error(std::string(who) + ": " + msg); \
+            return RM_ERR_INVALID_ARGUMENT;                   \
+        }                                                     \
+    } while (0)
+
+int head_fwd(const char* who, const float* x, int64_t n, int n_in, const float* w1, const float* b1, const float* w2,
+             const float* b2, const float* w3, const float* b3, int n_out, int precision, float* y, hipStream_t st) {
+    HEAD_CHECK(n >= 0, "negative n");
+    HEAD_CHECK(n_in == 32 || n_in == 64, "n_in must be 32 or 64");
+    HEAD_CHECK(n_out >= 1 && n_out <= 48, "n_out must be in [1, 48]");
+    HEAD_CHECK(precision == 0 || precision == 1, "precision must be 0 (fp32) or 1 (bf16)");
+    if (n == 0) return RM_OK;
+    HEAD_CHECK(x && w1 && b1 && w2 && b2 && w3 && b3 && y, "null buffer");
+    HEAD_CHECK(((uintptr_t)x & 15) == 0, "x must be 16-byte aligned");
+    const bool wide = n_out > 16;
+    if (n_in == 32) {
+        if (wide) dispatch_fwd<2, 3>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+        else dispatch_fwd<2, 1>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+    } else {
+        if (wide) dispatch_fwd<4, 3>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+        else dispatch_fwd<4, 1>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, y, st);
+    }
+    RM_CHECK_LAUNCH(who);
+    return RM_OK;
+}
+
+int head_bwd(const char* who, const float* x, int64_t n, int n_in, const float* w1, const float* b1, const float* w2,
+             const float* b2, const float* w3, const float* b3, int n_out, int precision, const float* gy, float* gx,
+             float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3, hipStream_t st) {
+    HEAD_CHECK(n >= 0, "negative n");
+    HEAD_CHECK(n_in == 32 || n_in == 64, "n_in must be 32 or 64");
+    HEAD_CHECK(n_out >= 1 && n_out <= 48, "n_out must be in [1, 48]");
+    HEAD_CHECK(precision == 0 || precision == 1, "precision must be 0 (fp32) or 1 (bf16)");
+    if (n == 0) return RM_OK;
+    HEAD_CHECK(x && w1 && b1 && w2 && b2 && w3 && b3 && gy && gw1 && gb1 && gw2 && gb2 && gw3 && gb3,
+                 "null buffer");
+    HEAD_CHECK(((uintptr_t)x & 15) == 0 && ((uintptr_t)gx & 15) == 0, "x and grad_x must be 16-byte aligned");
+    const bool wide = n_out > 16;
+    if (n_in == 32) {
+        if (wide) dispatch_bwd<2, 3>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+        else dispatch_bwd<2, 1>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+    } else {
+        if (wide) dispatch_bwd<4, 3>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+        else dispatch_bwd<4, 1>(precision, x, n, w1, b1, w2, b2, w3, b3, n_out, gy, gx, gw1, gb1, gw2, gb2, gw3, gb3, st);
+    }
+    RM_CHECK_LAUNCH(who);
+    return RM_OK;
+}
+
+#undef HEAD_CHECK
 
 }  // namespace
 }  // namespace rmesh
@@ -429,46 +593,30 @@ using namespace rmesh;
 extern "C" int rm_mlp_fwd(const float* x_dev, int64_t n, const float* w1, const float* b1, const float* w2,
                           const float* b2, const float* w3, const float* b3, int32_t n_out, int32_t precision,
                           float* y_dev, rm_stream_t stream) {
-    RM_CHECK_ARG(n >= 0, "rm_mlp_fwd: negative n");
-    RM_CHECK_ARG(n_out >= 1 && n_out <= kOutMax, "rm_mlp_fwd: n_out must be in [1, 16]");
-    RM_CHECK_ARG(precision == 0 || precision == 1, "rm_mlp_fwd: precision must be 0 (fp32) or 1 (bf16)");
-    if (n == 0) return RM_OK;
-    RM_CHECK_ARG(x_dev && w1 && b1 && w2 && b2 && w3 && b3 && y_dev, "rm_mlp_fwd: null buffer");
-    RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0, "rm_mlp_fwd: x must be 16-byte aligned");
-    const void* fn = precision == 0 ? reinterpret_cast<const void*>(&mlp_fwd_kernel<0>)
-                                    : reinterpret_cast<const void*>(&mlp_fwd_kernel<1>);
-    dim3 grid(grid_for(n, resident_blocks(fn, g_resident[precision], 1024))), block(kThreads);
-    if (precision == 0)
-        hipLaunchKernelGGL(mlp_fwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
-                           n_out, y_dev);
-    else
-        hipLaunchKernelGGL(mlp_fwd_kernel<1>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
-                           n_out, y_dev);
-    RM_CHECK_LAUNCH("mlp_fwd_kernel");
-    return RM_OK;
+    RM_CHECK_ARG(n_out >= 1 && n_out <= 16, "rm_mlp_fwd: n_out must be in [1, 16]");
+    return head_fwd("rm_mlp_fwd", x_dev, n, 32, w1, b1, w2, b2, w3, b3, n_out, precision, y_dev, as_stream(stream));
 }
 
 extern "C" int rm_mlp_bwd(const float* x_dev, int64_t n, const float* w1, const float* b1, const float* w2,
                           const float* b2, const float* w3, const float* b3, int32_t n_out, int32_t precision,
                           const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1, float* gw2,
                           float* gb2, float* gw3, float* gb3, rm_stream_t stream) {
-    RM_CHECK_ARG(n >= 0, "rm_mlp_bwd: negative n");
-    RM_CHECK_ARG(n_out >= 1 && n_out <= kOutMax, "rm_mlp_bwd: n_out must be in [1, 16]");
-    RM_CHECK_ARG(precision == 0 || precision == 1, "rm_mlp_bwd: precision must be 0 (fp32) or 1 (bf16)");
-    if (n == 0) return RM_OK;
-    RM_CHECK_ARG(x_dev && w1 && b1 && w2 && b2 && w3 && b3 && grad_y_dev && gw1 && gb1 && gw2 && gb2 && gw3 && gb3,
-                 "rm_mlp_bwd: null buffer");
-    RM_CHECK_ARG(((uintptr_t)x_dev & 15) == 0 && ((uintptr_t)grad_x_dev & 15) == 0,
-                 "rm_mlp_bwd: x and grad_x must be 16-byte aligned");
-    const void* fn = precision == 0 ? reinterpret_cast<const void*>(&mlp_bwd_kernel<0>)
-                                    : reinterpret_cast<const void*>(&mlp_bwd_kernel<1>);
-    dim3 grid(grid_for(n, resident_blocks(fn, g_resident[2 + precision], 512))), block(kThreads);
-    if (precision == 0)
-        hipLaunchKernelGGL(mlp_bwd_kernel<0>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
-                           n_out, grad_y_dev, grad_x_dev, gw1, gb1, gw2, gb2, gw3, gb3);
-    else
-        hipLaunchKernelGGL(mlp_bwd_kernel<1>, grid, block, 0, as_stream(stream), x_dev, n, w1, b1, w2, b2, w3, b3,
-                           n_out, grad_y_dev, grad_x_dev, gw1, gb1, gw2, gb2, gw3, gb3);
-    RM_CHECK_LAUNCH("mlp_bwd_kernel");
-    return RM_OK;
+    RM_CHECK_ARG(n_out >= 1 && n_out <= 16, "rm_mlp_bwd: n_out must be in [1, 16]");
+    return head_bwd("rm_mlp_bwd", x_dev, n, 32, w1, b1, w2, b2, w3, b3, n_out, precision, grad_y_dev, grad_x_dev,
+                    gw1, gb1, gw2, gb2, gw3, gb3, as_stream(stream));
+}
+
+extern "C" int rm_field_head_fwd(const float* x_dev, int64_t n, int32_t n_in, const float* w1, const float* b1,
+                                 const float* w2, const float* b2, const float* w3, const float* b3, int32_t n_out,
+                                 int32_t precision, float* y_dev, rm_stream_t stream) {
+    return head_fwd("rm_field_head_fwd", x_dev, n, n_in, w1, b1, w2, b2, w3, b3, n_out, precision, y_dev,
+                    as_stream(stream));
+}
+
+extern "C" int rm_field_head_bwd(const float* x_dev, int64_t n, int32_t n_in, const float* w1, const float* b1,
+                                 const float* w2, const float* b2, const float* w3, const float* b3, int32_t n_out,
+                                 int32_t precision, const float* grad_y_dev, float* grad_x_dev, float* gw1, float* gb1,
+                                 float* gw2, float* gb2, float* gw3, float* gb3, rm_stream_t stream) {
+    return head_bwd("rm_field_head_bwd", x_dev, n, n_in, w1, b1, w2, b2, w3, b3, n_out, precision, grad_y_dev,
+                    grad_x_dev, gw1, gb1, gw2, gb2, gw3, gb3, as_stream(stream));
 }

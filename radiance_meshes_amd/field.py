@@ -2,8 +2,8 @@
 
 * ``HashGrid``     utils/hashgrid.py:5-124 (HashEmbedderOptimized with conditional hashing on the
                    [0,1]^3 box, level-major output)
-* ``FieldMLP``     the field head 32 -> 64 -> 64 -> n_out with SELU (models/ingp_color.py:141-151),
-                   FP32 or BF16 operands on MFMA, FP32 accumulation
+* ``FieldMLP``     a field head {32, 64} -> 64 -> 64 -> n_out (n_out <= 48) with SELU
+                   (models/ingp_color.py:141-151), FP32 or BF16 operands on MFMA, FP32 accumulation
 * ``contract``     utils/contraction.py:6-20 (contract_mean_std, position part), mapped to [0,1]^3
                    as models/ingp_color.py:347-349
 * ``activate``     raw 13 outputs -> [density, base rgb, 3x3 colour gradient] features
@@ -59,13 +59,18 @@ class _HashEncode(torch.autograd.Function):
 
 class HashGrid(torch.nn.Module):
     def __init__(self, n_levels=16, n_features_per_level=2, log2_hashmap_size=19, base_resolution=16,
-                 growth=1.38, device="cuda", seed=0):
+                 growth=1.38, device="cuda", seed=0, level_table=None):
+        """level_table: (res [L] int32, offs [L+1] int64) to use instead of floor(base * growth^l)."""
         super().__init__()
         self.n_levels, self.n_feat, self.log2_size = n_levels, n_features_per_level, log2_hashmap_size
-        self.res, self.offs = level_table(n_levels, base_resolution, growth, log2_hashmap_size)
-        g = torch.Generator().manual_seed(seed)
-        init = (torch.rand((int(self.offs[-1]), n_features_per_level), generator=g) * 2 - 1) * 1e-4
-        self.table = torch.nn.Parameter(init.to(device))
+        if level_table is None:
+            self.res, self.offs = globals()["level_table"](n_levels, base_resolution, growth, log2_hashmap_size)
+        else:
+            self.res = np.ascontiguousarray(level_table[0], np.int32)
+            self.offs = np.ascontiguousarray(level_table[1], np.int64)
+        g = torch.Generator(device=device).manual_seed(seed)
+        init = (torch.rand((int(self.offs[-1]), n_features_per_level), generator=g, device=device) * 2 - 1) * 1e-4
+        self.table = torch.nn.Parameter(init)
         self.out_dim = n_levels * n_features_per_level
 
     def forward(self, x):
@@ -81,8 +86,8 @@ class _MLP(torch.autograd.Function):
         y = torch.empty((N, n_out), device=x.device, dtype=torch.float32)
         ws = [t.contiguous() for t in (w1, b1, w2, b2, w3, b3)]
         with timing.span("mlp_fwd"):
-            check(lib().rm_mlp_fwd(ptr(x), N, *[ptr(t) for t in ws], n_out, precision, ptr(y), stream_of(x.device)),
-              "rm_mlp_fwd")
+            check(lib().rm_field_head_fwd(ptr(x), N, x.shape[1], *[ptr(t) for t in ws], n_out, precision, ptr(y),
+                                          stream_of(x.device)), "rm_field_head_fwd")
         ctx.save_for_backward(x, *ws)
         ctx.precision = precision
         return y
@@ -93,20 +98,21 @@ class _MLP(torch.autograd.Function):
         grads = [torch.zeros_like(t) for t in (w1, b1, w2, b2, w3, b3)]
         gx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
         with timing.span("mlp_bwd"):
-            check(lib().rm_mlp_bwd(ptr(x), x.shape[0], ptr(w1), ptr(b1), ptr(w2), ptr(b2), ptr(w3), ptr(b3),
-                               w3.shape[0], ctx.precision, ptr(gy.contiguous()), ptr(gx), ptr(grads[0]),
-                               ptr(grads[1]), ptr(grads[2]), ptr(grads[3]), ptr(grads[4]), ptr(grads[5]),
-                               stream_of(x.device)), "rm_mlp_bwd")
+            check(lib().rm_field_head_bwd(ptr(x), x.shape[0], x.shape[1], ptr(w1), ptr(b1), ptr(w2), ptr(b2), ptr(w3),
+                                          ptr(b3), w3.shape[0], ctx.precision, ptr(gy.contiguous()), ptr(gx),
+                                          ptr(grads[0]), ptr(grads[1]), ptr(grads[2]), ptr(grads[3]), ptr(grads[4]),
+                                          ptr(grads[5]), stream_of(x.device)), "rm_field_head_bwd")
         return (gx, *grads, None)
 
 
 class FieldMLP(torch.nn.Module):
-    """32 -> 64 -> 64 -> n_out, SELU; weights ~ N(0, 1/fan_in), zero biases."""
+    """n_in -> 64 -> 64 -> n_out, SELU (n_in 32 or 64, n_out <= 48); weights ~ N(0, 1/fan_in), zero
+    biases."""
 
     def __init__(self, n_in=32, hidden=64, n_out=13, precision="fp32", device="cuda", seed=0):
         super().__init__()
-        if n_in != 32 or hidden != 64 or n_out > 16:
-            raise ValueError("the fused MLP kernel is built for 32 -> 64 -> 64 -> n_out (n_out <= 16)")
+        if n_in not in (32, 64) or hidden != 64 or not 1 <= n_out <= 48:
+            raise ValueError("the fused MLP kernels are built for {32, 64} -> 64 -> 64 -> n_out (n_out <= 48)")
         g = torch.Generator().manual_seed(seed)
 
         def w(o, i):
