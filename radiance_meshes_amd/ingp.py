@@ -11,9 +11,9 @@ field half of Model (compute_batch_features, get_cell_values; models/ingp_color.
                              log2 23, base 64, per-level scale 2, 45 SH outputs); k_samples 1.
 * ``compute_batch_features`` query anchor, FP64 circumradius and contraction (rm_field_query),
                              then the backbone; returns (anchor, density, rgb, grd, sh).
-* ``FieldModel``             the field half of Model: backbone, center, scene_scaling, SH degrees,
-                             get_cell_values(camera_center, vertices, indices, mask) ->
-                             (shs [T,K,3], features [T,7]) through activate_output.
+* ``Model``                  the parts of the reference's Model that render() and evaluate.py
+                             use (vertices, indices, get_cell_values through activate_output,
+                             compute_batch_features) and load_ckpt for its ckpt.pth + config.json.
 """
 from __future__ import annotations
 
@@ -218,44 +218,146 @@ def compute_batch_features(backbone, vertices, indices, start, end, center, scen
     return (anc, *backbone(x, cr))
 
 
-class FieldModel(torch.nn.Module):
-    """The field half of the reference's Model (models/ingp_color.py:208-384): a backbone iNGPDW
-    plus the scene normalisation, with get_cell_values as the renderer consumes it."""
+_HEADS = ("density_net", "color_net", "gradient_net", "sh_net")
 
-    def __init__(self, center, scene_scaling, max_sh_deg=3, current_sh_deg=None, ablate_circumsphere=True,
-                 ablate_gradient=False, device="cuda", **backbone_kw):
+
+class Model(torch.nn.Module):
+    """The parts of models/ingp_color.Model (:208-474) that render() and evaluate.py use: the
+    vertices (contracted_vertices + ext_vertices), indices, the backbone iNGPDW and the scene
+    normalisation, get_cell_values, compute_batch_features and load_ckpt. Constructor arguments as
+    the reference's; `density_offset` is consumed here and not passed on, so the backbone keeps
+    its own default (-4), as in the reference."""
+
+    def __init__(self, vertices, ext_vertices, center, scene_scaling, density_offset=-1, current_sh_deg=2,
+                 max_sh_deg=2, ablate_circumsphere=False, ablate_gradient=False, device="cuda", **kwargs):
         super().__init__()
+        self.device = torch.device(device)
+        self.density_offset = density_offset
         self.max_sh_deg = max_sh_deg
-        self.current_sh_deg = max_sh_deg if current_sh_deg is None else current_sh_deg
+        self.current_sh_deg = current_sh_deg
         self.sh_dim = (max_sh_deg + 1) ** 2 - 1
-        self.backbone = iNGPDW(sh_dim=self.sh_dim * 3, device=device, **backbone_kw)
+        self.backbone = iNGPDW(self.sh_dim * 3, device=device, **kwargs)
+        self.register_buffer("ext_vertices", torch.as_tensor(ext_vertices, dtype=torch.float32).to(device))
         self.register_buffer("center", torch.as_tensor(center, dtype=torch.float32).reshape(1, 3).to(device))
-        self.register_buffer("scene_scaling", torch.tensor(float(scene_scaling), device=device))
+        self.register_buffer("scene_scaling", torch.as_tensor(float(scene_scaling), dtype=torch.float32).to(device))
+        self.contracted_vertices = torch.nn.Parameter(torch.as_tensor(vertices, dtype=torch.float32).to(device))
+        self.indices = torch.empty((0, 4), dtype=torch.int32, device=device)
+        self.empty_indices = torch.empty((0, 4), dtype=torch.int32, device=device)
+        self.chunk_size = 308576
+        self.mask_values = True
+        self.feature_dim = 7
+        self.alpha = 0
         self.ablate_circumsphere = ablate_circumsphere
         self.ablate_gradient = ablate_gradient
-        self.chunk_size = 308576
+        self.min_t = 0.1
 
-    def compute_batch_features(self, vertices, indices, start, end):
+    @property
+    def vertices(self):
+        return torch.cat([self.contracted_vertices, self.ext_vertices])
+
+    @property
+    def num_int_verts(self):
+        return self.contracted_vertices.shape[0]
+
+    def compute_batch_features(self, vertices, indices, start, end, circumcenters=None):
+        """(anchor, density, rgb, grd, sh) of tets [start, end) (models/ingp_color.py:314-357).
+        The radius is recomputed in FP64 from the tet when circumcenters are given too (the
+        reference then takes |circumcenter - v0| from them; both agree to float32 rounding)."""
         return compute_batch_features(self.backbone, vertices, indices, start, end, self.center,
                                       float(self.scene_scaling), self.ablate_circumsphere)
 
-    def get_cell_values(self, camera_center, vertices, indices, mask=None):
-        """Model.get_cell_values (models/ingp_color.py:359-384): (shs [T, sh_dim, 3] float32,
-        features [T, 7]) over the (masked) tets, chunked as the reference chunks."""
-        idx = indices[mask] if mask is not None else indices
-        T = idx.shape[0]
-        dev = vertices.device
-        feats, shs = [], []
+    def get_cell_values(self, camera, mask=None, all_circumcenters=None, radii=None):
+        """models/ingp_color.py:361-384: (shs [T, sh_dim, 3] float32, features [T, 7]) over the
+        (masked) tets, in the reference's chunks, through activate_output at camera.camera_center."""
+        indices = self.indices[mask] if mask is not None else self.indices
+        vertices = self.vertices
+        T = indices.shape[0]
+        features = torch.empty((T, self.feature_dim), device=self.device)
+        shs = torch.empty((T, self.sh_dim, 3), device=self.device)
+        cam_center = torch.as_tensor(camera.camera_center).to(self.device)
         for start in range(0, T, self.chunk_size):
             end = min(start + self.chunk_size, T)
-            _, density, rgb, grd, sh = self.compute_batch_features(vertices, idx, start, end)
+            _, density, rgb, grd, sh = self.compute_batch_features(vertices, indices, start, end, all_circumcenters)
             if self.ablate_gradient:
                 grd = torch.zeros_like(grd)
-            centroids = vertices[idx[start:end].long()].mean(dim=1)
+            centroids = vertices[indices[start:end].long()].mean(dim=1)
             sh3 = sh.reshape(-1, self.sh_dim, 3)
-            shs.append(sh3.float())
-            feats.append(activate_output(camera_center.to(dev), density, rgb, grd, sh3, idx[start:end], centroids,
-                                         vertices.detach(), self.current_sh_deg, self.max_sh_deg))
-        if not feats:
-            return (torch.empty((0, self.sh_dim, 3), device=dev), torch.empty((0, 7), device=dev))
-        return torch.cat(shs, 0), torch.cat(feats, 0)
+            shs[start:end] = sh3
+            features[start:end] = activate_output(cam_center, density, rgb, grd, sh3, indices[start:end], centroids,
+                                                  vertices.detach(), self.current_sh_deg, self.max_sh_deg)
+        return shs, features
+
+    # ------------------------------------------------------------------ checkpoints
+    def load_reference_state_dict(self, sd):
+        """Loads a state dict in the reference's naming (Model.state_dict(), with or without the
+        '_orig_mod.' level torch.compile adds under backbone): geometry buffers, the per-level
+        embedding tables (concatenated into the grid's table) and the four heads' Linear layers
+        (indices 0, 2, 4 of each nn.Sequential). Other entries (the unused 'network' head, the
+        encoding's constant buffers, sphere_pattern) are checked where they constrain the shape
+        and otherwise ignored."""
+        sd = {k.replace("backbone._orig_mod.", "backbone."): v for k, v in sd.items()}
+        dev = self.device
+        with torch.no_grad():
+            self.contracted_vertices.data = sd["contracted_vertices"].to(dev, torch.float32).contiguous()
+            self.ext_vertices = sd["ext_vertices"].to(dev, torch.float32).contiguous()
+            self.center.copy_(sd["center"].reshape(1, 3))
+            self.scene_scaling.copy_(sd["scene_scaling"].reshape(()))
+            enc = self.backbone.encoding
+            if "backbone.encoding.level_resolutions" in sd:
+                lr = sd["backbone.encoding.level_resolutions"].cpu().numpy().astype(np.int64)
+                if not np.array_equal(lr, enc.res.astype(np.int64)):
+                    raise ValueError(f"level resolutions {lr.tolist()} differ from this grid's {enc.res.tolist()}")
+            for l in range(enc.n_levels):
+                w = sd[f"backbone.encoding.embeddings.{l}.weight"]
+                rows = int(enc.offs[l + 1] - enc.offs[l])
+                if tuple(w.shape) != (rows, enc.n_feat):
+                    raise ValueError(f"embedding {l}: shape {tuple(w.shape)}, expected {(rows, enc.n_feat)}")
+                enc.table.data[int(enc.offs[l]):int(enc.offs[l + 1])] = w.to(dev, torch.float32)
+            for h in _HEADS:
+                head = getattr(self.backbone, h)
+                if head.n_out == 0:
+                    continue
+                for j, (wn, bn) in enumerate((("w1", "b1"), ("w2", "b2"), ("w3", "b3"))):
+                    getattr(head, wn).data.copy_(sd[f"backbone.{h}.{2 * j}.weight"])
+                    getattr(head, bn).data.copy_(sd[f"backbone.{h}.{2 * j}.bias"])
+
+    def reference_state_dict(self):
+        """The inverse of load_reference_state_dict: CPU tensors under the reference's key names
+        (with torch.compile's '_orig_mod.' level, as its train.py saves them), plus indices and
+        empty_indices as train.py:365-369 adds them."""
+        enc = self.backbone.encoding
+        p = "backbone._orig_mod."
+        sd = {"contracted_vertices": self.contracted_vertices.detach().cpu(), "ext_vertices": self.ext_vertices.cpu(),
+              "center": self.center.cpu(), "scene_scaling": self.scene_scaling.cpu(),
+              p + "encoding.level_resolutions": torch.tensor(enc.res.astype(np.float32))}
+        for l in range(enc.n_levels):
+            sd[p + f"encoding.embeddings.{l}.weight"] = enc.table.detach()[int(enc.offs[l]):int(enc.offs[l + 1])].cpu()
+        for h in _HEADS:
+            head = getattr(self.backbone, h)
+            for j, (wn, bn) in enumerate((("w1", "b1"), ("w2", "b2"), ("w3", "b3"))):
+                sd[p + f"{h}.{2 * j}.weight"] = getattr(head, wn).detach().cpu()
+                sd[p + f"{h}.{2 * j}.bias"] = getattr(head, bn).detach().cpu()
+        sd["indices"] = self.indices.cpu()
+        sd["empty_indices"] = self.empty_indices.cpu()
+        return sd
+
+    @staticmethod
+    def load_ckpt(path, device="cuda"):
+        """models/ingp_color.py:452-474: config.json (the training arguments) and ckpt.pth from
+        the directory `path`. The checkpoint is read with torch.load(weights_only=True): tensors
+        only, nothing in the file is executed."""
+        import json
+        import os
+        with open(os.path.join(path, "config.json")) as f:
+            config = json.load(f)
+        ckpt = torch.load(os.path.join(path, "ckpt.pth"), map_location="cpu", weights_only=True)
+        indices = ckpt.pop("indices")
+        empty = ckpt.pop("empty_indices", torch.empty((0, 4), dtype=indices.dtype))
+        kw = {k: v for k, v in config.items() if k != "device"}
+        model = Model(ckpt["contracted_vertices"], ckpt["ext_vertices"], ckpt["center"],
+                      float(ckpt["scene_scaling"]), device=device, **kw)
+        model.load_reference_state_dict(ckpt)
+        model.min_t = float(model.scene_scaling) * float(config.get("base_min_t", 0.1))
+        model.indices = torch.as_tensor(indices).to(device, torch.int32)
+        model.empty_indices = torch.as_tensor(empty).to(device, torch.int32)
+        return model
