@@ -2,7 +2,10 @@
 
 The reference renders one view per call (evaluate.py:50-63, tile_benchmark.py:59-94). Here:
 
-* scenes (mesh + per-tet features) are registered once and stay in HBM;
+* scenes (mesh + per-tet features) are registered once and stay in HBM; a loaded reference
+  model (ingp.Model, e.g. from Model.load_ckpt) is registered with its field evaluated once, and
+  each view of it gets its view-dependent features (activate_output at the view's camera centre,
+  models/ingp_color.py:361-384) from one kernel on the batch's stream before the batched launch;
 * requests are grouped by resolution bucket; each bucket owns a fixed number of slots with
   preallocated buffers, a pinned host plan, its own stream and (optionally) captured HIP graphs
   of the whole batched forward (`rm_batch_render`: projection, binning and compositing of every
@@ -41,11 +44,23 @@ class Request:
 
 
 @dataclass
+class _Field:
+    """View-independent field outputs of a registered model, per tet."""
+    density: torch.Tensor      # [T]
+    rgb: torch.Tensor          # [T,3]
+    grd: torch.Tensor          # [T,3]
+    sh: torch.Tensor           # [T,K,3] float32 (values of the reference's FP16 SH)
+    centroids: torch.Tensor    # [T,3]
+    sh_degree: int
+
+
+@dataclass
 class _Scene:
     indices: torch.Tensor
     vertices: torch.Tensor
     features: torch.Tensor
     rec: RmScene
+    field: _Field = None       # set for view-dependent (model) scenes
 
 
 def _cam_struct(camera, tile, min_t):
@@ -82,6 +97,9 @@ class Bucket:
         self.nc = torch.empty((npx,), device=dev, dtype=torch.int32)
         self.counts = torch.zeros(2, device=dev, dtype=torch.int64)
         self.counts_host = torch.zeros(2, dtype=torch.int64, pin_memory=True)
+        # per-slot features of view-dependent scenes (sized for the largest of them)
+        t_vd = max([sc.rec.n_tets for sc in server.scenes if sc.field is not None], default=0)
+        self.vfeat = torch.empty((slots, t_vd, server.n_features), device=dev) if t_vd else None
         self.stream = torch.cuda.Stream(device=dev)
         self.graphs = {}
         self.use_graph = use_graph
@@ -96,7 +114,8 @@ class Bucket:
         self.ws = None
         self.ws = torch.empty(lib().rm_batch_workspace_size(self.info, self.pair_cap), dtype=torch.uint8,
                               device=self.server.device)
-        self.bytes = self.ws.numel() + self.img.numel() * 40 // 4 + 2 * self.plan_bytes
+        self.bytes = self.ws.numel() + self.img.numel() * 40 // 4 + 2 * self.plan_bytes + \
+            (self.vfeat.numel() * 4 if self.vfeat is not None else 0)
 
     def _class_of(self, reqs, full=False):
         """(key, launch info, pair capacity) of the capacity class that holds the batch `reqs`."""
@@ -123,8 +142,7 @@ class Bucket:
         keep_aux the distortion images and contributor counts) are copied out on the same stream,
         so the next batch of this bucket cannot overwrite them. full=True launches with the
         bucket's whole capacity."""
-        info = self.server.plan_info([(r.scene, r.camera) for r in reqs], self.plan_host)
-        if info.total_tets > self.info.total_tets or len(reqs) > self.slots:
+        if len(reqs) > self.slots:
             raise RuntimeError("batch exceeds the bucket's capacity")
         key, cinfo, cap = self._class_of(reqs, full)
         hw = self.H * self.W
@@ -135,6 +153,25 @@ class Bucket:
             # on that stream still use them (a single-view render's binning, say). Ordering this
             # stream after the current one keeps the batch from writing memory those kernels use.
             self.stream.wait_stream(cur)
+            # views of model scenes read the features of their own camera, written into their
+            # slot's buffer here (on this stream, after the previous batch that read it)
+            views, extra = [], []
+            n_sc = len(self.server.scenes)
+            for i, r in enumerate(reqs):
+                sc = self.server.scenes[r.scene]
+                if sc.field is None:
+                    views.append((r.scene, r.camera))
+                    continue
+                buf = self.vfeat[i, :sc.rec.n_tets]
+                self.server.view_features(r.scene, r.camera, buf)
+                rec = RmScene(indices_dev=sc.rec.indices_dev, vertices_dev=sc.rec.vertices_dev,
+                              features_dev=buf.data_ptr(), n_tets=sc.rec.n_tets, n_verts=sc.rec.n_verts,
+                              n_features=sc.rec.n_features)
+                views.append((n_sc + len(extra), r.camera))
+                extra.append(rec)
+            info = self.server.plan_info(views, self.plan_host, extra)
+            if info.total_tets > self.info.total_tets:
+                raise RuntimeError("batch exceeds the bucket's capacity")
             self.plan_dev.copy_(self.plan_host, non_blocking=True)
             if self.use_graph:
                 g = self.graphs.get(key)
@@ -215,6 +252,44 @@ class RenderServer:
         self.buckets: dict = {}
 
     # ------------------------------------------------------------------ scenes
+    def add_model(self, model, chunk=1 << 20):
+        """Registers a reference model (ingp.Model: vertices, indices, backbone) as a scene whose
+        features depend on the view: the field (density, rgb, grd, SH) is evaluated once here, in
+        chunks of `chunk` tets; each view gets activate_output at its camera centre (7 features,
+        so the server must have n_features 7). Returns the scene id."""
+        if self.n_features != 7:
+            raise ValueError("model scenes have 7 features: create the server with n_features=7")
+        with torch.no_grad():
+            V = model.vertices.detach().to(self.device, torch.float32).contiguous()
+            idx = model.indices.to(self.device, torch.int32).contiguous()
+            T = idx.shape[0]
+            outs = [model.compute_batch_features(V, idx, s, min(s + chunk, T)) for s in range(0, T, chunk)]
+            density = torch.cat([o[1].reshape(-1) for o in outs])
+            rgb = torch.cat([o[2].reshape(-1, 3) for o in outs])
+            grd = torch.cat([o[3].reshape(-1, 3) for o in outs])
+            K = model.sh_dim
+            sh = torch.cat([o[4].reshape(-1, K, 3).float() for o in outs]) if K else None
+            centroids = V[idx.long()].mean(dim=1)
+        sid = self.add_scene(V, idx, torch.zeros((T, 7), device=self.device))
+        self.scenes[sid].field = _Field(density.contiguous(), rgb.contiguous(), grd.contiguous(),
+                                        sh.contiguous() if sh is not None else None, centroids.contiguous(),
+                                        int(model.current_sh_deg))
+        return sid
+
+    def view_features(self, sid, camera, out=None):
+        """[T,7] features of model scene `sid` seen from `camera` (rm_activate_fwd at its centre) on
+        the current stream, into `out` when given."""
+        sc = self.scenes[sid]
+        f = sc.field
+        T = sc.rec.n_tets
+        out = torch.empty((T, 7), device=self.device) if out is None else out
+        center = torch.as_tensor(np.asarray(camera.cam_pos, np.float32).reshape(3)).to(self.device, non_blocking=True)
+        K = 0 if f.sh is None else f.sh.shape[1]
+        check(lib().rm_activate_fwd(ptr(center), ptr(f.density), ptr(f.rgb), ptr(f.grd), ptr(f.sh) if K else None,
+                                    K, f.sh_degree, ptr(sc.indices), ptr(f.centroids), ptr(sc.vertices), T, ptr(out),
+                                    stream_of(self.device)), "rm_activate_fwd")
+        return out
+
     def add_scene(self, vertices, indices, features):
         """Registers a device-resident scene and returns its id. Buckets sized for the scenes
         registered before are dropped (their workspaces and per-scene pair estimates no longer
@@ -256,13 +331,14 @@ class RenderServer:
         rg = RenderGrid(H, W, self.tile, self.tile)
         tcam = self._tcam(camera)
         with torch.no_grad():
+            feats = sc.features if sc.field is None else self.view_features(sid, camera)
             sidx, ranges, vs, cc, mask, rect = vertex_and_tile_shader(sc.indices, sc.vertices, tcam, rg)
             jit = torch.full((H, W, 2), 0.5, device=self.device)
             img = torch.empty((H, W, 4), device=self.device)
             dist = torch.empty((H, W, 5), device=self.device)
             nc = torch.empty((H, W), device=self.device, dtype=torch.int32)
             check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(sc.indices),
-                                         ptr(sc.vertices), ptr(sc.features), sc.features.shape[1], ptr(jit),
+                                         ptr(sc.vertices), ptr(feats), feats.shape[1], ptr(jit),
                                          ptr(img), ptr(dist), ptr(nc), stream_of(self.device)), "rm_composite_fwd")
         return (img, dist, nc) if aux else img
 
@@ -271,16 +347,19 @@ class RenderServer:
         from . import scenes
         return scenes.orbit_camera(self.probe_radius, 0.0, 0.0, W, H, 50.0)
 
-    def plan_info(self, views, plan_host=None):
-        """rm_batch_plan over [(scene, camera), ...]; packs into `plan_host` (pinned) when given."""
+    def plan_info(self, views, plan_host=None, extra=()):
+        """rm_batch_plan over [(scene, camera), ...]; packs into `plan_host` (pinned) when given.
+        `extra` scene records follow the registered ones (ids len(scenes) + i): views of model
+        scenes, each with its own features."""
         n = len(views)
         cams = (RmCamera * n)(*[_cam_struct(c, self.tile, self.min_t) for _, c in views])
         sid = (C.c_int32 * n)(*[s for s, _ in views])
-        scs = (RmScene * len(self.scenes))(*[sc.rec for sc in self.scenes])
+        recs = [sc.rec for sc in self.scenes] + list(extra)
+        scs = (RmScene * len(recs))(*recs)
         info = RmBatchInfo()
         buf = None if plan_host is None else C.c_void_p(plan_host.data_ptr())
         nbytes = 0 if plan_host is None else plan_host.numel()
-        check(lib().rm_batch_plan(scs, len(self.scenes), cams, sid, n, buf, nbytes, C.byref(info)),
+        check(lib().rm_batch_plan(scs, len(recs), cams, sid, n, buf, nbytes, C.byref(info)),
               "rm_batch_plan")
         return info
 
