@@ -296,8 +296,8 @@ int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_out_dev, co
                     const int64_t* offs, int32_t n_levels, int32_t n_feat, int32_t log2_size,
                     float* grad_table_dev, rm_stream_t stream);
 /* The same backward (the adjoint of HashEmbedderOptimized.forward, utils/hashgrid.py:117-124)
- * with a workspace: for n_feat 2, the hashed levels with res >= 256 are summed by slot ownership
- * (updates sorted into 2048-slot buckets, one workgroup per bucket, no float atomics) instead of
+ * with a workspace: for n_feat 2, the hashed levels with res >= 512 are summed by slot ownership
+ * (updates sorted into 4096-slot buckets, one workgroup per bucket, no float atomics) instead of
  * by atomics. rm_hashgrid_bwd_workspace_size returns the bytes needed (0: no level qualifies,
  * and rm_hashgrid_bwd_ws then runs rm_hashgrid_bwd's path without a workspace). */
 size_t rm_hashgrid_bwd_workspace_size(int64_t n, const int32_t* res, const int64_t* offs, int32_t n_levels,

@@ -336,26 +336,29 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
 // aggregation above issues one global atomic per update. Instead the updates of levels
 // [lp, L) are sorted by bucket (4096 consecutive slots of one level) with a counting sort, and one
 // workgroup per bucket sums its updates in LDS and adds them to the table with plain stores:
-//  1. part_count: per 4096-point chunk, the number of updates per bucket (LDS histogram),
-//  2. part_offsets: exclusive offsets per (chunk, bucket), bucket-major,
-//  3. part_scatter: the same walk as (1), each update stored at its bucket's next position as
-//     (slot within the bucket, value pair),
+//  1. part_count: per 512-point chunk, the number of updates per bucket (LDS histogram),
+//  2. part_column / part_bucket_scan: exclusive offsets per (chunk, bucket), bucket-major,
+//  3. part_scatter: per chunk and level, the updates sorted by bucket in LDS, then each bucket's
+//     run written to its place as (slot within the bucket, value pair) with coalesced stores,
 //  4. part_accumulate: per bucket, the sum of its updates added to the table.
-constexpr int kPartPoints = 4096;              // query points per chunk (16 per lane)
+// Staging a chunk-level in LDS keeps every run's cache lines whole: writing the records straight
+// from the lanes left each of the ~900 runs a workgroup feeds partly written over the workgroup's
+// lifetime, more open lines than the L2 holds, and the partial evictions wrote ~1.45x the bytes.
+constexpr int kPartPoints = 512;               // query points per chunk (one per lane)
 constexpr int kPartLog2Slots = 12;
 constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket (64 KB of FP64 LDS sums)
 constexpr int kPartMaxBuckets = 8192;
+constexpr int kPartMaxLevelBuckets = 512;      // buckets of one level: one per lane
 constexpr int kPartAccThreads = 512;
 
-__global__ __launch_bounds__(256) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
-                                                         int lp, int nb_level, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kPartPoints) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
+                                                                 int lp, int nb_level, uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_hist[kPartMaxBuckets];
     const int nb = (g.L - lp) * nb_level;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) s_hist[b] = 0;
     __syncthreads();
-    for (int i = 0; i < kPartPoints / 256; ++i) {
-        const int64_t n = (int64_t)blockIdx.x * kPartPoints + i * 256 + threadIdx.x;
-        if (n >= N) break;
+    const int64_t n = (int64_t)blockIdx.x * kPartPoints + threadIdx.x;
+    if (n < N) {
         float xv[3];
         load_point(x, n, xv);
         for (int l = lp; l < g.L; ++l) {
@@ -431,41 +434,81 @@ __global__ __launch_bounds__(256) void part_bucket_scan_kernel(const uint32_t* _
     }
 }
 
-__global__ __launch_bounds__(256) void part_scatter_kernel(const float* __restrict__ x, int64_t N,
-                                                           const float* __restrict__ gout, GridArgs g, int lp,
-                                                           int nb_level, const uint32_t* __restrict__ hist,
-                                                           const uint32_t* __restrict__ bucket_start,
-                                                           uint16_t* __restrict__ rslot, float2* __restrict__ rval) {
-    __shared__ uint32_t s_cur[kPartMaxBuckets];
-    const int nb = (g.L - lp) * nb_level;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        s_cur[b] = bucket_start[b] + hist[(int64_t)b * gridDim.x + blockIdx.x];
-    __syncthreads();
-    for (int i = 0; i < kPartPoints / 256; ++i) {
-        const int64_t n = (int64_t)blockIdx.x * kPartPoints + i * 256 + threadIdx.x;
-        if (n >= N) break;
-        float xv[3];
-        load_point(x, n, xv);
-        for (int l = lp; l < g.L; ++l) {
+__global__ __launch_bounds__(kPartPoints) void part_scatter_kernel(const float* __restrict__ x, int64_t N,
+                                                                   const float* __restrict__ gout, GridArgs g, int lp,
+                                                                   int nb_level, int64_t n_chunks,
+                                                                   const uint32_t* __restrict__ hist,
+                                                                   const uint32_t* __restrict__ bucket_start,
+                                                                   uint16_t* __restrict__ rslot,
+                                                                   float2* __restrict__ rval) {
+    constexpr int kRec = kPartPoints * 8;
+    __shared__ uint16_t s_slot[kRec];
+    __shared__ uint16_t s_bk[kRec];       // bucket of each staged record
+    __shared__ float2 s_val[kRec];
+    __shared__ uint32_t s_off[kPartMaxLevelBuckets + 1];   // counts, then exclusive offsets
+    __shared__ uint32_t s_dst[kPartMaxLevelBuckets];       // each bucket's first record in rslot/rval
+    __shared__ uint32_t s_wave[kPartPoints / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t n = (int64_t)blockIdx.x * kPartPoints + t;
+    const bool valid = n < N;
+    float xv[3] = {0.f, 0.f, 0.f};
+    if (valid) load_point(x, n, xv);
+    for (int l = lp; l < g.L; ++l) {
+        for (int b = t; b <= nb_level; b += blockDim.x) s_off[b] = 0;
+        __syncthreads();
+        int bx = 0, by = 0, bz = 0;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (valid) cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
+        uint32_t h[8], pos[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            h[c] = hashed_index(g, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
+            pos[c] = valid ? atomicAdd(&s_off[h[c] >> kPartLog2Slots], 1u) : 0u;
+        }
+        __syncthreads();
+        // exclusive scan of the counts (nb_level <= blockDim.x: one bucket per lane)
+        const int lb = l - lp;
+        const uint32_t cnt = t < nb_level ? s_off[t] : 0u;
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int i = 0; i < wave; ++i) pre += s_wave[i];
+        if (t < nb_level) {
+            s_off[t] = pre + incl - cnt;
+            const int64_t bk = (int64_t)lb * nb_level + t;
+            s_dst[t] = bucket_start[bk] + hist[bk * n_chunks + blockIdx.x];
+        }
+        if (t == nb_level - 1) s_off[nb_level] = pre + incl;
+        __syncthreads();
+        if (valid) {
             const float2 go = *reinterpret_cast<const float2*>(gout + (n * g.L + l) * 2);
-            int bx, by, bz;
-            float fx, fy, fz;
-            cell_of(xv, g, l, bx, by, bz, fx, fy, fz);
-            // the 8 position claims go out together, then the 16 stores
-            uint32_t h[8], pos[8];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) h[c] = hashed_index(g, bx + ((c >> 2) & 1), by + ((c >> 1) & 1), bz + (c & 1));
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                pos[c] = atomicAdd(&s_cur[(l - lp) * nb_level + (int)(h[c] >> kPartLog2Slots)], 1u);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
                 const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-                rslot[pos[c]] = (uint16_t)(h[c] & (kPartSlots - 1));
-                rval[pos[c]] = make_float2(w * go.x, w * go.y);
+                const uint32_t bk = h[c] >> kPartLog2Slots;
+                const uint32_t j = s_off[bk] + pos[c];
+                s_slot[j] = (uint16_t)(h[c] & (kPartSlots - 1));
+                s_bk[j] = (uint16_t)bk;
+                s_val[j] = make_float2(w * go.x, w * go.y);
             }
         }
+        __syncthreads();
+        // write-out: consecutive staged records of a bucket go to consecutive records of its run
+        const uint32_t total = s_off[nb_level];
+        for (uint32_t j = t; j < total; j += blockDim.x) {
+            const int bk = s_bk[j];
+            const uint32_t dst = s_dst[bk] + (j - s_off[bk]);
+            rslot[dst] = s_slot[j];
+            rval[dst] = s_val[j];
+        }
+        __syncthreads();
     }
 }
 
@@ -532,14 +575,14 @@ __global__ __launch_bounds__(kPartAccThreads) void part_accumulate_kernel(const 
 }
 
 // First level of the partitioned suffix, or L when the partitioned backward does not apply:
-// n_feat 2, hashed levels with res >= 256 (a suffix, as resolutions grow with the level), whole
+// n_feat 2, hashed levels with res >= 512 (a suffix, as resolutions grow with the level), whole
 // buckets per level, and few enough buckets for the LDS histograms.
 int partition_first_level(const GridArgs& g, int log2_size) {
     if (g.F != 2 || log2_size < kPartLog2Slots) return g.L;
     int lp = g.L;
-    while (lp > 0 && !g.dense[lp - 1] && g.res[lp - 1] >= 256) --lp;
+    while (lp > 0 && !g.dense[lp - 1] && g.res[lp - 1] >= 512) --lp;
     const int64_t nb = (int64_t)(g.L - lp) << (log2_size - kPartLog2Slots);
-    if (lp == g.L || nb > kPartMaxBuckets) return g.L;
+    if (lp == g.L || nb > kPartMaxBuckets || (1 << (log2_size - kPartLog2Slots)) > kPartMaxLevelBuckets) return g.L;
     return lp;
 }
 
@@ -697,11 +740,13 @@ extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* gr
         int r = hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, lp, grad_table_dev, st);
         if (r) return r;
     }
-    hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, g, lp, P.nb_level, hist);
+    hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)P.n_chunks), dim3(kPartPoints), 0, st, x_dev, n, g, lp,
+                       P.nb_level, hist);
     hipLaunchKernelGGL(part_column_kernel, dim3((unsigned)P.nb), dim3(256), 0, st, hist, P.n_chunks, total);
     hipLaunchKernelGGL(part_bucket_scan_kernel, dim3(1), dim3(256), 0, st, (const uint32_t*)total, P.nb, start);
-    hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)P.n_chunks), dim3(256), 0, st, x_dev, n, grad_out_dev, g, lp,
-                       P.nb_level, (const uint32_t*)hist, (const uint32_t*)start, rslot, rval);
+    hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)P.n_chunks), dim3(kPartPoints), 0, st, x_dev, n,
+                       grad_out_dev, g, lp, P.nb_level, P.n_chunks, (const uint32_t*)hist, (const uint32_t*)start,
+                       rslot, rval);
     hipLaunchKernelGGL(part_accumulate_kernel, dim3((unsigned)P.nb), dim3(kPartAccThreads), 0, st, (const uint32_t*)start,
                        (const uint16_t*)rslot, (const float2*)rval, g, lp, P.nb_level, grad_table_dev);
     RM_CHECK_LAUNCH("rm_hashgrid_bwd_ws");
