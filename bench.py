@@ -437,6 +437,57 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes": src.numel() * 4, "launches": 5}), flush=True)
 
 
+class BusySampler:
+    """Mean of the GPU activity the driver reports (sysfs gpu_busy_percent of the card torch uses,
+    matched by PCI bus id when several cards are listed), sampled every `period` s on a thread
+    between start() and stop(); None where the file is not readable."""
+
+    def __init__(self, period=0.01):
+        import glob
+        import threading
+        self.period, self.samples, self._stop = period, [], threading.Event()
+        self.path = None
+        paths = sorted(glob.glob("/sys/class/drm/card*/device/gpu_busy_percent"))
+        if len(paths) > 1:
+            try:
+                import torch
+                pr = torch.cuda.get_device_properties(0)
+                bus = f"{pr.pci_domain_id:04x}:{pr.pci_bus_id:02x}:{pr.pci_device_id:02x}"
+                paths = [q for q in paths if os.path.realpath(os.path.dirname(q)).split("/")[-1].startswith(bus)]
+            except Exception:
+                paths = []
+        if len(paths) == 1:
+            try:
+                open(paths[0]).read()
+                self.path = paths[0]
+            except OSError:
+                self.path = None
+        self._thread = None
+
+    def _run(self):
+        while not self._stop.is_set():
+            try:
+                self.samples.append(float(open(self.path).read()))
+            except (OSError, ValueError):
+                pass
+            self._stop.wait(self.period)
+
+    def start(self):
+        import threading
+        self.samples = []
+        self._stop.clear()
+        if self.path:
+            self._thread = threading.Thread(target=self._run, daemon=True)
+            self._thread.start()
+        return self
+
+    def stop(self):
+        self._stop.set()
+        if self._thread is not None:
+            self._thread.join()
+        return round(float(np.mean(self.samples)), 1) if self.samples else None
+
+
 def serve_leg(args, meshes):
     """Batched serving against per-frame rendering on one Poisson trace (forward only, fp32).
     meshes: [(n_points, pts, tets)], registered as device-resident scenes with seeded features."""
@@ -476,12 +527,17 @@ def serve_leg(args, meshes):
     log(f"[bench] serving: {len(trace)} requests over {args.serve_seconds} s at {args.serve_rate}/s")
     keep = lambda r: r.rid % 25 == 0   # noqa: E731  (results kept for the batched/per-frame comparison)
     runs = {}
+    busy = {}
+    sampler = BusySampler()
+    log(f"[bench] GPU activity from {sampler.path}" if sampler.path else "[bench] GPU activity: no readable sysfs file")
     for mode in ("batched", "per_frame"):
         reqs = [Request(r.rid, r.scene, r.camera, r.arrival) for r in trace]
+        sampler.start()
         runs[mode] = (srv.serve(reqs, mode=mode, keep=keep,
                                 progress=lambda d, n, t, m=mode: log(f"[bench]   {m}: {d}/{n} done at {t:.1f} s")),
                       reqs)
-        log(f"[bench] {mode}: {json.dumps(runs[mode][0])}")
+        busy[mode] = sampler.stop()
+        log(f"[bench] {mode}: {json.dumps(runs[mode][0])}, GPU busy {busy[mode]}%")
     # every kept frame: batched result against the per-frame result of the same request
     max_err, n_cmp = 0.0, 0
     for rb, rp in zip(runs["batched"][1], runs["per_frame"][1]):
@@ -499,15 +555,19 @@ def serve_leg(args, meshes):
                      "seconds": args.serve_saturate_seconds}
         for mode in ("batched", "per_frame"):
             reqs = [Request(r.rid, r.scene, r.camera, r.arrival) for r in strace]
+            sampler.start()
             st = srv.serve(reqs, mode=mode, keep=lambda r: False,
                            progress=lambda d, n, t, m=mode: log(f"[bench]   saturated {m}: {d}/{n} done at {t:.1f} s"))
-            log(f"[bench] saturated {mode}: {json.dumps(st)}")
+            b = sampler.stop()
+            log(f"[bench] saturated {mode}: {json.dumps(st)}, GPU busy {b}%")
             saturated[mode] = {k: round(st[k], 3) for k in ("fps", "p50_ms", "p99_ms", "mean_batch")}
+            saturated[mode]["gpu_busy_pct"] = b
         saturated["fps_ratio"] = round(saturated["batched"]["fps"] / saturated["per_frame"]["fps"], 3)
     return {"saturated": saturated, "fps_batched": round(sb["fps"], 2), "fps_per_frame": round(sp["fps"], 2),
             "fps_ratio": round(sb["fps"] / sp["fps"], 3),
             "p50_ms_batched": round(sb["p50_ms"], 2), "p99_ms_batched": round(sb["p99_ms"], 2),
             "p50_ms_per_frame": round(sp["p50_ms"], 2), "p99_ms_per_frame": round(sp["p99_ms"], 2),
+            "gpu_busy_pct_batched": busy["batched"], "gpu_busy_pct_per_frame": busy["per_frame"],
             "mean_batch": round(sb["mean_batch"], 2), "overflows": sb["overflows"],
             "max_abs_err_batched_vs_per_frame": max_err, "frames_compared": n_cmp,
             "trace": {"requests": len(trace), "seconds": args.serve_seconds, "rate_hz": args.serve_rate,
