@@ -151,10 +151,40 @@ __global__ __launch_bounds__(kLmPoints * kLmWaves) void hashgrid_fwd_lm_kernel(c
 // Per corner, a wave whose lanes hit at most 32 distinct slot runs sums each run of equal slot
 // indices with a segmented scan and issues one atomic per run (its last lane); otherwise every
 // lane adds its own contribution directly.
+// A float atomic costs one memory transaction per cache line it touches, whatever the number of
+// lanes that hit the line. With n_feat >= 4 each update is a row of F contiguous floats, so the
+// wave's updates of one corner go through LDS and out transposed: F consecutive lanes add the F
+// features of one update, one transaction per update instead of F.
+template <int F>
+__device__ inline void flush_rows(float* __restrict__ tb, uint32_t h, const float (&v)[F], bool writes, float* s_t,
+                                  uint32_t* s_h, int lane) {
+    constexpr int S = F + 1;
+#pragma unroll
+    for (int f = 0; f < F; ++f) s_t[lane * S + f] = v[f];
+    s_h[lane] = writes ? h : 0xFFFFFFFFu;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    constexpr int PER = 64 / F;   // updates per instruction
+#pragma unroll
+    for (int j = 0; j < F; ++j) {
+        const int e = j * PER + lane / F, f = lane % F;
+        const uint32_t he = s_h[e];
+        const float val = s_t[e * S + f];
+        if (he != 0xFFFFFFFFu && val != 0.f) unsafeAtomicAdd(tb + (int64_t)he * F + f, val);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int F>
 __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restrict__ x, int64_t N,
                                                            const float* __restrict__ gout, GridArgs g,
                                                            float* __restrict__ gtable) {
+    constexpr bool kRows = F >= 4;
+    __shared__ float s_t[kRows ? 4 * 64 * (F + 1) : 1];
+    __shared__ uint32_t s_h[kRows ? 4 * 64 : 1];
+    float* wt = s_t + (kRows ? (threadIdx.x >> 6) * 64 * (F + 1) : 0);
+    uint32_t* wh = s_h + (kRows ? (threadIdx.x >> 6) * 64 : 0);
     const int l = blockIdx.y;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -178,10 +208,13 @@ __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restri
         const bool head = lane == 0 || h != hprev;
         const uint64_t heads = __ballot(head);
         if (__popcll(heads) > 32) {
-            if (valid)
+            if constexpr (kRows) {
+                flush_rows<F>(tb, h, v, valid, wt, wh, lane);
+            } else if (valid) {
 #pragma unroll
                 for (int f = 0; f < F; ++f)
                     if (v[f] != 0.f) unsafeAtomicAdd(tb + (int64_t)h * F + f, v[f]);
+            }
             continue;
         }
         // segmented inclusive scan over runs of equal h
@@ -199,10 +232,13 @@ __global__ __launch_bounds__(256) void hashgrid_bwd_kernel(const float* __restri
             if (lane >= o) seg = seg || su;
         }
         const bool next_head = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
-        if (next_head && h != 0xFFFFFFFFu)
+        if constexpr (kRows) {
+            flush_rows<F>(tb, h, v, next_head && h != 0xFFFFFFFFu, wt, wh, lane);
+        } else if (next_head && h != 0xFFFFFFFFu) {
 #pragma unroll
             for (int f = 0; f < F; ++f)
                 if (v[f] != 0.f) unsafeAtomicAdd(tb + (int64_t)h * F + f, v[f]);
+        }
     }
 }
 
@@ -315,18 +351,18 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
                 for (int f = 0; f < F; ++f) unsafeAtomicAdd(gtable + (g.offs[l] + (int64_t)h) * F + f, v[f]);
         }
         __syncthreads();
+        // consecutive lanes take the F features of one slot, so they share one atomic transaction
         float* tb = gtable + g.offs[l] * F;
-        for (int p = tid; p < kAggSlots; p += kAggPoints) {
-            const uint32_t k = s_key[p];
+        for (int p = tid; p < kAggSlots * F; p += kAggPoints) {
+            const int slot = p / F;
+            const uint32_t k = s_key[slot];
             if (k == kEmptySlot) continue;
-#pragma unroll
-            for (int f = 0; f < F; ++f) {
-                const float v = (float)s_val[p * F + f];
-                if (v != 0.f) unsafeAtomicAdd(tb + (int64_t)k * F + f, v);
-                s_val[p * F + f] = 0.0;
-            }
-            s_key[p] = kEmptySlot;
+            const float v = (float)s_val[p];
+            if (v != 0.f) unsafeAtomicAdd(tb + (int64_t)k * F + (p - slot * F), v);
+            s_val[p] = 0.0;
         }
+        __syncthreads();
+        for (int p = tid; p < kAggSlots; p += kAggPoints) s_key[p] = kEmptySlot;
         __syncthreads();
     }
 }
