@@ -155,25 +155,43 @@ __device__ inline double erf_scale(int k, double cr, double scale_multi, double 
     return r > 0.0 ? e : (r < 0.0 ? -e : 0.0);
 }
 
-// One thread per (point, k): the L scales of a point are computed once each and applied to the
-// F columns j*L + k of its row (the reference views the [L*F] encoding row as [F][L] and scales
-// the last axis; with the level-major layout column i belongs to group i % L).
+// One thread per four consecutive elements of the [n, L*F] encoding (float4 in and out, so a wave
+// reads and writes one contiguous kilobyte); column i belongs to group i % L (the reference views
+// the level-major row as [F][L] and scales the last axis). Each thread evaluates the scales of its
+// four columns itself (four FP64 exponentials per 16 bytes in and out).
+__global__ __launch_bounds__(256) void downweight4_kernel(const float4* __restrict__ in, const double* __restrict__ cr,
+                                                          int64_t n4, int C, int L, double scale_multi, double pls4,
+                                                          float4* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const int64_t e = 4 * i;
+    const int64_t p = e / C;
+    const int col = (int)(e - p * C);
+    const double c = cr[p];
+    const float4 v = in[i];
+    float4 o;
+    o.x = (float)((double)v.x * erf_scale(col % L, c, scale_multi, pls4));
+    o.y = (float)((double)v.y * erf_scale((col + 1) % L, c, scale_multi, pls4));
+    o.z = (float)((double)v.z * erf_scale((col + 2) % L, c, scale_multi, pls4));
+    o.w = (float)((double)v.w * erf_scale((col + 3) % L, c, scale_multi, pls4));
+    out[i] = o;
+}
+
+// Rows whose width is not a multiple of 4: one thread per element.
 __global__ __launch_bounds__(256) void downweight_kernel(const float* __restrict__ in, const double* __restrict__ cr,
-                                                         int64_t n, int L, int F, double scale_multi, double pls4,
+                                                         int64_t n_el, int C, int L, double scale_multi, double pls4,
                                                          float* __restrict__ out) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * L) return;
-    const int64_t p = i / L;
-    const int k = (int)(i - p * L);
-    const double s = erf_scale(k, cr[p], scale_multi, pls4);
-    const int64_t row = p * (int64_t)L * F;
-    for (int j = 0; j < F; ++j) {
-        const int64_t col = row + (int64_t)j * L + k;
-        out[col] = (float)((double)in[col] * s);
-    }
+    if (i >= n_el) return;
+    const int64_t p = i / C;
+    const int col = (int)(i - p * C);
+    out[i] = (float)((double)in[i] * erf_scale(col % L, cr[p], scale_multi, pls4));
 }
 
+// One thread per element of the [n, n_sh] SH block (coalesced); threads below n also do that
+// query's density, colour and gradient.
 __global__ __launch_bounds__(256) void outputs_fwd_kernel(const float* __restrict__ sigma, const float* __restrict__ rgb,
                                                           const float* __restrict__ grd, const float* __restrict__ sh,
                                                           int64_t n, int n_sh, float density_offset,
@@ -181,6 +199,7 @@ __global__ __launch_bounds__(256) void outputs_fwd_kernel(const float* __restric
                                                           float* __restrict__ grd_out, __half* __restrict__ sh_out) {
 #pragma clang fp contract(off)
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n * n_sh) sh_out[p] = __float2half_rn(sh[p]);
     if (p >= n) return;
     float s = sigma[p] + density_offset;
     s = fminf(fmaxf(s, -kMaxF), kLogMaxF);
@@ -194,7 +213,6 @@ __global__ __launch_bounds__(256) void outputs_fwd_kernel(const float* __restric
     const float den = sqrtf(((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]) + 1.0f);
 #pragma unroll
     for (int d = 0; d < 3; ++d) grd_out[3 * p + d] = g[d] / den;
-    for (int j = 0; j < n_sh; ++j) sh_out[p * n_sh + j] = __float2half_rn(sh[p * n_sh + j]);
 }
 
 __global__ __launch_bounds__(256) void outputs_bwd_kernel(const float* __restrict__ density,
@@ -205,6 +223,8 @@ __global__ __launch_bounds__(256) void outputs_bwd_kernel(const float* __restric
                                                           float* __restrict__ g_grd_raw, float* __restrict__ g_sh_raw) {
 #pragma clang fp contract(off)
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the FP16 cast passes its (FP16) gradient back as float32
+    if (p < n * n_sh) g_sh_raw[p] = g_sh ? __half2float(g_sh[p]) : 0.0f;
     if (p >= n) return;
     // safe_exp's backward: y * grad (utils/safe_math.py:231-233), also where the input was clamped
     g_sigma[p] = g_density ? density[p] * g_density[p] : 0.0f;
@@ -221,8 +241,6 @@ __global__ __launch_bounds__(256) void outputs_bwd_kernel(const float* __restric
     const float k = dot / ((s * s) * s);
 #pragma unroll
     for (int d = 0; d < 3; ++d) g_grd_raw[3 * p + d] = gy[d] / s - g[d] * k;
-    // the FP16 cast passes its (FP16) gradient back as float32
-    for (int j = 0; j < n_sh; ++j) g_sh_raw[p * n_sh + j] = g_sh ? __half2float(g_sh[p * n_sh + j]) : 0.0f;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)ceil_div(n, 256); }
@@ -259,8 +277,16 @@ extern "C" int rm_field_downweight(const float* in_dev, const double* cr_dev, in
                  "rm_field_downweight: n_levels and n_feat must be in [1, 64]");
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(in_dev && cr_dev && out_dev, "rm_field_downweight: null buffer");
-    hipLaunchKernelGGL(downweight_kernel, dim3(blocks_for(n * n_levels)), dim3(256), 0, as_stream(stream), in_dev,
-                       cr_dev, n, n_levels, n_feat, scale_multi, per_level_scale * 4.0, out_dev);
+    const int C = n_levels * n_feat;
+    if (C % 4 == 0 && ((uintptr_t)in_dev & 15) == 0 && ((uintptr_t)out_dev & 15) == 0) {
+        const int64_t n4 = n * C / 4;
+        hipLaunchKernelGGL(downweight4_kernel, dim3(blocks_for(n4)), dim3(256), 0, as_stream(stream),
+                           reinterpret_cast<const float4*>(in_dev), cr_dev, n4, C, n_levels, scale_multi,
+                           per_level_scale * 4.0, reinterpret_cast<float4*>(out_dev));
+    } else {
+        hipLaunchKernelGGL(downweight_kernel, dim3(blocks_for(n * C)), dim3(256), 0, as_stream(stream), in_dev,
+                           cr_dev, n * C, C, n_levels, scale_multi, per_level_scale * 4.0, out_dev);
+    }
     RM_CHECK_LAUNCH("downweight_kernel");
     return RM_OK;
 }
@@ -274,7 +300,7 @@ extern "C" int rm_field_outputs_fwd(const float* sigma_dev, const float* rgb_dev
     RM_CHECK_ARG(sigma_dev && rgb_dev && grd_dev && density_dev && rgb_out_dev && grd_out_dev,
                  "rm_field_outputs_fwd: null buffer");
     RM_CHECK_ARG(n_sh == 0 || (sh_dev && sh_out_dev), "rm_field_outputs_fwd: null sh buffer");
-    hipLaunchKernelGGL(outputs_fwd_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), sigma_dev, rgb_dev,
+    hipLaunchKernelGGL(outputs_fwd_kernel, dim3(blocks_for(n_sh > 1 ? n * n_sh : n)), dim3(256), 0, as_stream(stream), sigma_dev, rgb_dev,
                        grd_dev, sh_dev, n, n_sh, density_offset, density_dev, rgb_out_dev, grd_out_dev,
                        reinterpret_cast<__half*>(sh_out_dev));
     RM_CHECK_LAUNCH("outputs_fwd_kernel");
@@ -290,7 +316,7 @@ extern "C" int rm_field_outputs_bwd(const float* density_dev, const float* grd_r
     RM_CHECK_ARG(density_dev && grd_raw_dev && g_sigma_dev && g_rgb_raw_dev && g_grd_raw_dev,
                  "rm_field_outputs_bwd: null buffer");
     RM_CHECK_ARG(n_sh == 0 || g_sh_raw_dev, "rm_field_outputs_bwd: null sh gradient buffer");
-    hipLaunchKernelGGL(outputs_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), density_dev,
+    hipLaunchKernelGGL(outputs_bwd_kernel, dim3(blocks_for(n_sh > 1 ? n * n_sh : n)), dim3(256), 0, as_stream(stream), density_dev,
                        grd_raw_dev, g_density_dev, g_rgb_dev, g_grd_dev, reinterpret_cast<const __half*>(g_sh_dev), n,
                        n_sh, g_sigma_dev, g_rgb_raw_dev, g_grd_raw_dev, g_sh_raw_dev);
     RM_CHECK_LAUNCH("outputs_bwd_kernel");
