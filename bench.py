@@ -218,6 +218,18 @@ def delaunay_leg(args, pts, tets, reps=3):
                              "points_per_s": round(n_cpu / cpu_s)}}
 
 
+def tet_err_line(votes, P, G, N, workload, tile, precision):
+    """The error-vote kernel's figures: algorithmic bytes, achieved GB/s and, from the profile's
+    SQ_INSTS_VALU pass, its VALU issue fraction (the kernel is bound by vector-ALU work)."""
+    algo = 96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"]
+    out = {**votes, "algorithmic_bytes": algo, "achieved_gbs": round(algo / (votes["mean_ms"] * 1e-3) / 1e9, 2)}
+    insts = measured_traffic(workload, tile, precision, "tet_err", key="valu_insts")
+    if insts:
+        out["valu"] = {"insts_per_launch": insts,
+                       "frac": round(insts / (N_CU * CLOCK_HZ * votes["mean_ms"] * 1e-3), 4)}
+    return out
+
+
 def measured_traffic(workload, tile, precision, span, key="traffic"):
     """HBM bytes per launch of `span`'s kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
     recorded for this workload (profiles/traffic_*.json), or None when none was recorded; with
@@ -340,9 +352,7 @@ def run_config(args, pts, tets, cam, tile):
     return {"last": last, "value": round(mrays, 3), "ms_per_step": round(ms, 3), "steps": steps, "roofline": roofline,
             "hbm_frac_all_kernels": round(total_algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "breakdown_ms": breakdown,
-            "tet_err": {**votes, "algorithmic_bytes": 96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"],
-                        "achieved_gbs": round((96 * P + 8 * G + 8 * N + 72 * votes["voted_tets"])
-                                              / (votes["mean_ms"] * 1e-3) / 1e9, 2)},
+            "tet_err": tet_err_line(votes, P, G, N, workload, tile, args.precision),
             "config": {"workload": workload, "points": int(V),
                        "tets": int(T), "visible_tets": n_vis, "tile_pairs": P, "evaluations": n_eval,
                        "width": W, "height": H, "tile": tile,

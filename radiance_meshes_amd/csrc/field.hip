@@ -22,10 +22,18 @@ struct GridArgs {
     uint32_t mask;
 };
 
+// c mod r for the corner coordinates a clamped query produces: 0 <= c <= r + 1 (the cell index is
+// at most r, plus the corner offset), so at most two subtractions (two only when r = 1) replace
+// the integer division a % by a runtime divisor costs.
+__device__ inline int wrap_corner(int c, int r) {
+    c = c >= r ? c - r : c;
+    return c >= r ? c - r : c;
+}
+
 __device__ inline uint32_t corner_index(const GridArgs& g, int l, int cx, int cy, int cz) {
     if (g.dense[l]) {
         int r = g.res[l];
-        return (uint32_t)((cx % r) + (cy % r) * r + (cz % r) * r * r);
+        return (uint32_t)(wrap_corner(cx, r) + wrap_corner(cy, r) * r + wrap_corner(cz, r) * r * r);
     }
     return ((uint32_t)cx * 1u ^ (uint32_t)cy * 2654435761u ^ (uint32_t)cz * 805459861u) & g.mask;
 }
@@ -386,6 +394,7 @@ constexpr int kPartSlots = 1 << kPartLog2Slots;  // table slots per bucket (64 K
 constexpr int kPartMaxBuckets = 8192;
 constexpr int kPartMaxLevelBuckets = 512;      // buckets of one level: one per lane
 constexpr int kPartAccThreads = 512;
+constexpr int kPartMaxLevels = 8;              // levels in the partitioned suffix (their grad rows preloaded)
 
 __global__ __launch_bounds__(kPartPoints) void part_count_kernel(const float* __restrict__ x, int64_t N, GridArgs g,
                                                                  int lp, int nb_level, uint32_t* __restrict__ hist) {
@@ -489,7 +498,17 @@ __global__ __launch_bounds__(kPartPoints) void part_scatter_kernel(const float* 
     const bool valid = n < N;
     float xv[3] = {0.f, 0.f, 0.f};
     if (valid) load_point(x, n, xv);
-    for (int l = lp; l < g.L; ++l) {
+    // the query's gradient rows of all partitioned levels at once: they are contiguous, and read
+    // level by level they were fetched from HBM once per level (the lines do not survive a level)
+    float2 gor[kPartMaxLevels];
+#pragma unroll
+    for (int i = 0; i < kPartMaxLevels; ++i)
+        gor[i] = (valid && lp + i < g.L) ? *reinterpret_cast<const float2*>(gout + (n * g.L + lp + i) * 2)
+                                         : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kPartMaxLevels; ++i) {
+        const int l = lp + i;
+        if (l >= g.L) break;
         for (int b = t; b <= nb_level; b += blockDim.x) s_off[b] = 0;
         __syncthreads();
         int bx = 0, by = 0, bz = 0;
@@ -523,7 +542,7 @@ __global__ __launch_bounds__(kPartPoints) void part_scatter_kernel(const float* 
         if (t == nb_level - 1) s_off[nb_level] = pre + incl;
         __syncthreads();
         if (valid) {
-            const float2 go = *reinterpret_cast<const float2*>(gout + (n * g.L + l) * 2);
+            const float2 go = gor[i];
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
@@ -616,7 +635,7 @@ __global__ __launch_bounds__(kPartAccThreads) void part_accumulate_kernel(const 
 int partition_first_level(const GridArgs& g, int log2_size) {
     if (g.F != 2 || log2_size < kPartLog2Slots) return g.L;
     int lp = g.L;
-    while (lp > 0 && !g.dense[lp - 1] && g.res[lp - 1] >= 512) --lp;
+    while (lp > 0 && !g.dense[lp - 1] && g.res[lp - 1] >= 512 && g.L - lp < kPartMaxLevels) --lp;
     const int64_t nb = (int64_t)(g.L - lp) << (log2_size - kPartLog2Slots);
     if (lp == g.L || nb > kPartMaxBuckets || (1 << (log2_size - kPartLog2Slots)) > kPartMaxLevelBuckets) return g.L;
     return lp;
