@@ -227,7 +227,8 @@ int rm_activate_bwd(const float* camera_center_dev, const float* density_dev, co
  * the same binned lists as rm_composite_fwd, starting from log-transmittance 1 (that kernel's
  * initial state) and stopping after a blend below -5.5; every blended tet receives
  * w = exp(logT) * sigma * dt. Adds into tet_err_dev [n_tets,16] f32 (caller-zeroed):
- *   0: w, 1: w*pixel_err, 2: w*pixel_err^2, 3, 4: untouched, 5: w*ssim_err,
+ *   0: w, 1: w*pixel_err, 2: w*pixel_err^2, 3, 4: untouched (3 carries the call's vote counts
+ *   while it runs and is 0 again on return), 5: w*ssim_err,
  *   6-8: w*entry point, 9-11: w*exit point,
  * then writes 12: channel 0 and 13-15: (channels 6-8 + channels 9-11) / 2 -- the sums of the
  * reference's per-vote w and w*segment midpoint, so they stay consistent when calls accumulate --

@@ -1291,6 +1291,7 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
                     case 0: case 12: val = v[0]; break;
                     case 1: val = v[1]; break;
                     case 2: val = v[2]; break;
+                    case 3: val = (float)n_hit; break;   // the vote count, moved out by the finish kernel
                     case 5: val = v[3]; break;
                     case 6: val = v[4]; break;
                     case 7: val = v[5]; break;
@@ -1298,12 +1299,13 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
                     case 9: val = v[7]; break;
                     case 10: val = v[8]; break;
                     case 11: val = v[9]; break;
-                    default: val = 0.f; break;   // channels 3 and 4 carry no vote
+                    default: val = 0.f; break;   // channel 4 carries no vote
                 }
                 // channels 12-15 (w again, w * segment midpoint) follow from 0 and 6-11: written
-                // once per tet by tet_err_finish_kernel instead of four more atomics per vote row
-                if (gl != 3 && gl != 4 && gl < 12) unsafeAtomicAdd(tet_err + 16 * (int64_t)tet + gl, val);
-                if (gl == 0) atomicAdd(tet_count + 2 * (int64_t)tet, n_hit);
+                // once per tet by tet_err_finish_kernel instead of four more atomics per vote row.
+                // The count rides in channel 3 of the same row (one atomic transaction for the
+                // row's lanes instead of a second one for the count); the finish kernel moves it.
+                if (gl != 4 && gl < 12) unsafeAtomicAdd(tet_err + 16 * (int64_t)tet + gl, val);
                 if (gl == 1) atomicMax(tet_count + 2 * (int64_t)tet + 1, wmax);
             }
         }
@@ -1311,11 +1313,17 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void tet_err_kernel(
 }
 
 // Channels 12-15 of the vote rows from the accumulated 0 and 6-11: 12 = sum w, 13-15 = sum of
-// w * (entry + exit) / 2, the reference's per-vote midpoint terms summed.
-__global__ __launch_bounds__(256) void tet_err_finish_kernel(float4* __restrict__ err, int64_t n_tets) {
+// w * (entry + exit) / 2, the reference's per-vote midpoint terms summed. Channel 3 held this
+// call's vote count (integer sums, exact below 2^24): it is added to count[t][0] and cleared.
+__global__ __launch_bounds__(256) void tet_err_finish_kernel(float4* __restrict__ err, int* __restrict__ count,
+                                                             int64_t n_tets) {
     const int64_t t = blockIdx.x * 256ll + threadIdx.x;
     if (t >= n_tets) return;
     const float4 r0 = err[4 * t], r1 = err[4 * t + 1], r2 = err[4 * t + 2];
+    if (r0.w != 0.f) {
+        count[2 * t] += (int)r0.w;
+        err[4 * t] = make_float4(r0.x, r0.y, r0.z, 0.f);
+    }
     // r1 = channels 4-7, r2 = 8-11: entry (6, 7, 8) and exit (9, 10, 11)
     err[4 * t + 3] = make_float4(r0.x, 0.5f * (r1.z + r2.y), 0.5f * (r1.w + r2.z), 0.5f * (r2.x + r2.w));
 }
@@ -1657,7 +1665,7 @@ extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, i
 #undef RM_ERRV
     if (n_tets > 0)
         hipLaunchKernelGGL(tet_err_finish_kernel, dim3((unsigned)ceil_div(n_tets, 256)), dim3(256), 0, st,
-                           (float4*)tet_err_dev, n_tets);
+                           (float4*)tet_err_dev, tet_count_dev, n_tets);
     RM_CHECK_LAUNCH("tet_err_kernel");
     return RM_OK;
 }
