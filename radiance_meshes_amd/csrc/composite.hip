@@ -197,10 +197,11 @@ __device__ inline float expm1_neg(float x) {
 
 
 // Weights of the exit and entry colours in the segment integral, w0 = phi(x) - e^-x and
-// w1 = 1 - phi(x) with phi(x) = (1 - e^-x) / x (interp_version.slang:21-69; their sum is 1 - e^-x). Both are O(x): formed as phi - alpha and 1 - phi they
-// lose about eps / x of relative precision (1% at x = 1e-5, a thin sliver of a crossing), so below
-// x = 0.05 they come from their own series, w1 = sum (-1)^(k+1) x^k / (k+1)! and
-// w0 = sum (-1)^(k+1) k x^k / (k+1)!, to x^5 (truncation under 1e-9 relative).
+// w1 = 1 - phi(x) with phi(x) = (1 - e^-x) / x (interp_version.slang:21-69); their sum is
+// 1 - e^-x. Both are O(x): formed as phi - alpha and 1 - phi they lose about eps / x of relative
+// precision (1% at x = 1e-5, a thin sliver of a crossing), so below x = 0.05 they come from their
+// own series, w1 = sum (-1)^(k+1) x^k / (k+1)! and w0 = sum (-1)^(k+1) k x^k / (k+1)!, to x^5
+// (truncation under 1e-9 relative).
 __device__ inline void seg_weights(float x, float em1, float alpha, float& w0, float& w1) {
     if (x < 0.05f) {
         w1 = x * (0.5f - x * (1.f / 6.f - x * (1.f / 24.f - x * (1.f / 120.f - x * (1.f / 720.f)))));
