@@ -313,33 +313,56 @@ __device__ inline int plane_rec(float3 af, float3 bf, float3 cf, float3 of, floa
     return flip;
 }
 
+// The global data of one list entry: its four vertices and its feature row.
+template <int F>
+struct TetData {
+    float3 v[4];
+    float f[F];
+};
+
+template <int F>
+__device__ inline void load_tet_data(TetData<F>& d, int4 q, int tet, const float* __restrict__ verts,
+                                     const float* __restrict__ feats) {
+    d.v[0] = load_v(verts, q.x);
+    d.v[1] = load_v(verts, q.y);
+    d.v[2] = load_v(verts, q.z);
+    d.v[3] = load_v(verts, q.w);
+    const float* f = feats + (int64_t)F * tet;
+#pragma unroll
+    for (int k = 0; k < F; ++k) d.f[k] = f[k];
+}
+
+// Writes the slot's LDS records (face planes, base colour, gradients, ...) of tet from its data.
 template <int WG, int F, bool BWD, bool P64, bool VTX>
-__device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
-                                 const int4* __restrict__ indices, const float* __restrict__ verts,
-                                 const float* __restrict__ feats) {
+__device__ inline void stage_data(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
+                                  const TetData<F>& d) {
+    // the roundings of the base colour are fixed (no contraction chosen by the compiler), so that
+    // every kernel staging a tet, whatever its surrounding code, stages the same bits
+#pragma clang fp contract(off)
     using T = typename Prec<P64>::R;
     constexpr int NGRD = NG_<F>::NGRD;
-    int4 q = indices[tet];
-    float3 v0 = load_v(verts, q.x), v1 = load_v(verts, q.y), v2 = load_v(verts, q.z), v3 = load_v(verts, q.w);
+    const float3 v0 = d.v[0], v1 = d.v[1], v2 = d.v[2], v3 = d.v[3];
     float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
     int signs = 0;
     signs |= plane_rec<T>(v0, v1, v2, v3, cp, S.pl[0][slot]) << 0;
     signs |= plane_rec<T>(v0, v1, v3, v2, cp, S.pl[1][slot]) << 1;
     signs |= plane_rec<T>(v0, v2, v3, v1, cp, S.pl[2][slot]) << 2;
     signs |= plane_rec<T>(v1, v2, v3, v0, cp, S.pl[3][slot]) << 3;
-    const float* f = feats + (int64_t)F * tet;
+    const float* f = d.f;
     float g[NGRD];
 #pragma unroll
     for (int k = 0; k < NGRD; ++k) g[k] = f[4 + k];
     float3 cv = cp - v0;
+    // g . (cam - v0) as two fused multiply-adds on the first product
+    auto gdot = [&](float a, float b, float c) { return fmaf(c, cv.z, fmaf(b, cv.y, a * cv.x)); };
     float b0, b1, b2;
     if (F == 7) {
-        float g0 = g[0] * cv.x + g[1] * cv.y + g[2] * cv.z;
+        float g0 = gdot(g[0], g[1], g[2]);
         b0 = f[1] + g0; b1 = f[2] + g0; b2 = f[3] + g0;
     } else {
-        b0 = f[1] + (g[0] * cv.x + g[1] * cv.y + g[2] * cv.z);
-        b1 = f[2] + (g[3 % NGRD] * cv.x + g[4 % NGRD] * cv.y + g[5 % NGRD] * cv.z);
-        b2 = f[3] + (g[6 % NGRD] * cv.x + g[7 % NGRD] * cv.y + g[8 % NGRD] * cv.z);
+        b0 = f[1] + gdot(g[0], g[1], g[2]);
+        b1 = f[2] + gdot(g[3 % NGRD], g[4 % NGRD], g[5 % NGRD]);
+        b2 = f[3] + gdot(g[6 % NGRD], g[7 % NGRD], g[8 % NGRD]);
     }
     S.fa[slot] = make_float4(fmaxf(f[0], 0.f), b0, b1, b2);
 #pragma unroll
@@ -360,6 +383,56 @@ __device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int t
         S.meta[1][slot] = signs;
     }
 }
+
+template <int WG, int F, bool BWD, bool P64, bool VTX>
+__device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
+                                 const int4* __restrict__ indices, const float* __restrict__ verts,
+                                 const float* __restrict__ feats) {
+    TetData<F> d;
+    load_tet_data<F>(d, indices[tet], tet, verts, feats);
+    stage_data(S, slot, tet, cam, d);
+}
+
+// Load pipeline over a tile's chunks of list entries, walked front to back. Each lane owns at most
+// one entry per chunk; entry(c) gives the lane's tet in chunk c, or -1 when it has none. While
+// chunk c is walked, the entries of chunk c + 3, the index rows of chunk c + 2 and the vertices and
+// features of chunk c + 1 are in flight, so the three dependent fetches (list entry, index row,
+// vertex and feature rows) of a chunk overlap the walks of the chunks before it (18 + F registers).
+template <int F>
+struct ChunkPipe {
+    int t0, t1, t2;   // this lane's entries in the next chunk to stage and the two after it
+    int4 q1;          // index row of t1
+    TetData<F> d0;    // vertices and features of t0
+
+    template <class E>
+    __device__ inline void start(E entry, const int4* __restrict__ indices, const float* __restrict__ verts,
+                                 const float* __restrict__ feats) {
+        t0 = entry(0);
+        t1 = entry(1);
+        t2 = entry(2);
+        q1 = t1 >= 0 ? indices[t1] : make_int4(0, 0, 0, 0);
+        if (t0 >= 0) load_tet_data<F>(d0, indices[t0], t0, verts, feats);
+    }
+
+    // stages this lane's entry of the current chunk (a lane with no entry stages nothing)
+    template <class St>
+    __device__ inline void stage(St& S, int slot, const CamArgs& cam) {
+        if (t0 >= 0) stage_data(S, slot, t0, cam, d0);
+    }
+
+    // issues the loads of the chunks after chunk c, the one just staged
+    template <class E>
+    __device__ inline void advance(E entry, int c, const int4* __restrict__ indices,
+                                   const float* __restrict__ verts, const float* __restrict__ feats) {
+        const int tn = entry(c + 3);
+        const int4 qn = t2 >= 0 ? indices[t2] : make_int4(0, 0, 0, 0);
+        if (t1 >= 0) load_tet_data<F>(d0, q1, t1, verts, feats);
+        t0 = t1;
+        t1 = t2;
+        t2 = tn;
+        q1 = qn;
+    }
+};
 
 template <int WG, int F, bool BWD, bool P64, bool VTX>
 __device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot, float* g) {
@@ -456,13 +529,18 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
     bool alive = inside;
     __syncthreads();
     const int nchunks = s_chunks;
+    auto entry = [&](int cc) {
+        const int kk = cc * C::GROUP + r;
+        return kk < n ? (int)(sorted[rg.x + kk] - tet_base) : -1;
+    };
+    ChunkPipe<F> pipe;
+    pipe.start(entry, indices, verts, feats);
     for (int c = 0; c < nchunks; ++c) {
         if (!__syncthreads_or(alive)) break;
         uint64_t blend_bits = 0;
-        int k = c * C::GROUP + r;
-        if (k < n)
-            stage_tet(S, threadIdx.x, (int)(sorted[rg.x + k] - tet_base), cam, indices, verts, feats);
+        pipe.stage(S, threadIdx.x, cam);
         __syncthreads();
+        pipe.advance(entry, c, indices, verts, feats);
         if (alive) {
             int cnt = n - c * C::GROUP;
             cnt = cnt < C::GROUP ? cnt : C::GROUP;

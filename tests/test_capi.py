@@ -53,14 +53,20 @@ def test_argument_errors_without_device(lib):
 
 
 def test_hashgrid_workspace_rule(lib):
-    """Slot-ownership backward applies to the hashed levels with res >= 256 (n_feat 2 only)."""
+    """Slot-ownership backward applies to the finest run of hashed levels with res >= 512, at most
+    8 of them (n_feat 2 only)."""
     from radiance_meshes_amd.field import level_table
-    res, offs = level_table(16, 16, 1.38, 19)          # levels 9..15 (res 290..2005) qualify
+    res, offs = level_table(16, 16, 1.38, 19)          # levels 11..15 (res 553..2005) qualify
     n = 1_000_000
     ws = lib.rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, 16, 2, 19)
-    n_part = sum(1 for r in res if int(r) ** 3 > 2 ** 19 and r >= 256)
-    assert n_part == 7
+    n_part = sum(1 for r in res if int(r) ** 3 > 2 ** 19 and r >= 512)
+    assert n_part == 5
     assert ws >= n * 8 * n_part * 10                  # (slot, value pair) per update
+    assert ws < n * 8 * (n_part + 1) * 10             # and no level below res 512
+    res24, offs24 = level_table(24, 16, 1.38, 19)      # 13 levels of res >= 512: the finest 8 qualify
+    ws24 = lib.rm_hashgrid_bwd_workspace_size(n, res24.ctypes.data, offs24.ctypes.data, 24, 2, 19)
+    assert sum(1 for r in res24 if r >= 512) == 13
+    assert n * 8 * 8 * 10 <= ws24 < n * 8 * 9 * 10
     assert lib.rm_hashgrid_bwd_workspace_size(n, res.ctypes.data, offs.ctypes.data, 16, 4, 19) == 0
     res8, offs8 = level_table(8, 16, 1.38, 19)         # finest resolution 106: no level qualifies
     assert lib.rm_hashgrid_bwd_workspace_size(n, res8.ctypes.data, offs8.ctypes.data, 8, 2, 19) == 0
