@@ -20,7 +20,9 @@
 // are accumulated as adjoints of the plane records (per plane: sum of u = ds/den and of
 // u (x - v0), x the crossing point), summed across the wavefront (or 16-lane tile group) with cross-lane shuffles,
 // and converted once per (wave, tet) to vertex and feature gradients, which lanes of the
-// wave then add with one float atomic each.
+// wave then add with one float atomic each. At 4x4 tiles with per-tet rows the lanes hold tets
+// instead of pixels (composite_bwd_tet_lanes_kernel): each lane sums its tet's adjoints over the
+// tile's 16 pixels and the walk state comes from prefix sums across the lanes.
 #include <mutex>
 #include <string>
 
@@ -332,10 +334,11 @@ __device__ inline void load_tet_data(TetData<F>& d, int4 q, int tet, const float
     for (int k = 0; k < F; ++k) d.f[k] = f[k];
 }
 
-// Writes the slot's LDS records (face planes, base colour, gradients, ...) of tet from its data.
-template <int WG, int F, bool BWD, bool P64, bool VTX>
-__device__ inline void stage_data(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
-                                  const TetData<F>& d) {
+// The per-view records of one tet: its four outward face planes (n_i, e_i) in the plane
+// precision, (sigma, folded base rgb) and the colour gradients; returns the face sign bits.
+template <int F, bool P64>
+__device__ inline int tet_record(const TetData<F>& d, const CamArgs& cam, typename Prec<P64>::PL (&pl)[4],
+                                 float4& fa, float (&g)[NG_<F>::NGRD]) {
     // the roundings of the base colour are fixed (no contraction chosen by the compiler), so that
     // every kernel staging a tet, whatever its surrounding code, stages the same bits
 #pragma clang fp contract(off)
@@ -344,12 +347,11 @@ __device__ inline void stage_data(Stage<WG, F, BWD, P64, VTX>& S, int slot, int 
     const float3 v0 = d.v[0], v1 = d.v[1], v2 = d.v[2], v3 = d.v[3];
     float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
     int signs = 0;
-    signs |= plane_rec<T>(v0, v1, v2, v3, cp, S.pl[0][slot]) << 0;
-    signs |= plane_rec<T>(v0, v1, v3, v2, cp, S.pl[1][slot]) << 1;
-    signs |= plane_rec<T>(v0, v2, v3, v1, cp, S.pl[2][slot]) << 2;
-    signs |= plane_rec<T>(v1, v2, v3, v0, cp, S.pl[3][slot]) << 3;
+    signs |= plane_rec<T>(v0, v1, v2, v3, cp, pl[0]) << 0;
+    signs |= plane_rec<T>(v0, v1, v3, v2, cp, pl[1]) << 1;
+    signs |= plane_rec<T>(v0, v2, v3, v1, cp, pl[2]) << 2;
+    signs |= plane_rec<T>(v1, v2, v3, v0, cp, pl[3]) << 3;
     const float* f = d.f;
-    float g[NGRD];
 #pragma unroll
     for (int k = 0; k < NGRD; ++k) g[k] = f[4 + k];
     float3 cv = cp - v0;
@@ -364,7 +366,25 @@ __device__ inline void stage_data(Stage<WG, F, BWD, P64, VTX>& S, int slot, int 
         b1 = f[2] + gdot(g[3 % NGRD], g[4 % NGRD], g[5 % NGRD]);
         b2 = f[3] + gdot(g[6 % NGRD], g[7 % NGRD], g[8 % NGRD]);
     }
-    S.fa[slot] = make_float4(fmaxf(f[0], 0.f), b0, b1, b2);
+    fa = make_float4(fmaxf(f[0], 0.f), b0, b1, b2);
+    return signs;
+}
+
+// Writes the slot's LDS records (face planes, base colour, gradients, ...) of tet from its data.
+template <int WG, int F, bool BWD, bool P64, bool VTX>
+__device__ inline void stage_data(Stage<WG, F, BWD, P64, VTX>& S, int slot, int tet, const CamArgs& cam,
+                                  const TetData<F>& d) {
+    using T = typename Prec<P64>::R;
+    constexpr int NGRD = NG_<F>::NGRD;
+    const float3 v0 = d.v[0], v1 = d.v[1], v2 = d.v[2], v3 = d.v[3];
+    float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
+    typename Prec<P64>::PL pl[4];
+    float4 fa;
+    float g[NGRD];
+    const int signs = tet_record<F, P64>(d, cam, pl, fa, g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) S.pl[i][slot] = pl[i];
+    S.fa[slot] = fa;
 #pragma unroll
     for (int k = 0; k < (NGRD + 3) / 4; ++k)
         S.fg[k][slot] = make_float4(g[4 * k], 4 * k + 1 < NGRD ? g[(4 * k + 1) % NGRD] : 0.f,
@@ -451,9 +471,11 @@ __device__ inline void read_grads(const Stage<WG, F, BWD, P64, VTX>& S, int slot
 
 // Blend of one hit into the pixel state (integrate_tetra_segment and the distortion moments);
 // returns true when the pixel terminates (log-transmittance below -5.5) after this tet.
+// log T is summed in FP64 and stored rounded once: the backward rebuilds each tet's log T from the
+// stored final value and the x of the tets behind it, so that value carries no accumulated error.
 template <int WG, int F, typename St, typename H>
 __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, float rdx, float rdy, float rdz,
-                                           float mt, float& cr, float& cg, float& cb, float& logT, float& X,
+                                           float mt, float& cr, float& cg, float& cb, double& logT, float& X,
                                            float& Y, float& Z, float& W, float& V, bool& blended) {
     constexpr int NGRD = NG_<F>::NGRD;
     const float4 fa = S.fa[slot];
@@ -467,7 +489,7 @@ __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, fl
     const float alpha = 1.0f + em1;
     float w0, w1;
     seg_weights(x, em1, alpha, w0, w1);
-    const float T = __expf(logT);
+    const float T = __expf((float)logT);
     const float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
     float gd1 = gd0, gd2 = gd0;
     if (F == 13) {
@@ -479,7 +501,7 @@ __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, fl
     cr += T * (wsum * fa.y + wsl * gd0);
     cg += T * (wsum * fa.z + wsl * gd1);
     cb += T * (wsum * fa.w + wsl * gd2);
-    logT -= x;
+    logT -= (double)x;
     const float te = h.se - mt, tx_ = h.sx - mt;
     const float m1 = fa.x * (te + tx_) * 0.5f;
     X += 2.0f * m1 * Z;
@@ -487,7 +509,7 @@ __device__ __forceinline__ bool blend_step(const St& S, int slot, const H& h, fl
     Z += fa.x;
     W += m1;
     V += fa.x * fa.x * dt * (1.0f / 3.0f);
-    return logT < -5.5f;
+    return logT < -5.5;
 }
 
 // Forward of the tiles of workgroup b of one view. Tile ids are offset by tile_base, the sorted
@@ -523,7 +545,8 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
     }
     const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
     const float mt = cam.min_t;
-    float cr = 0.f, cg = 0.f, cb = 0.f, logT = 0.f;
+    float cr = 0.f, cg = 0.f, cb = 0.f;
+    double logT = 0.0;
     float X = 0.f, Y = 0.f, Z = 0.f, W = 0.f, V = 0.f;
     int nc = 0;
     bool alive = inside;
@@ -594,7 +617,7 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
         }
     }
     if (inside) {
-        out_img[pix] = make_float4(cr, cg, cb, logT);
+        out_img[pix] = make_float4(cr, cg, cb, (float)logT);
         float* d = out_dist + 5 * pix;
         d[0] = X; d[1] = Y; d[2] = Z; d[3] = W; d[4] = V;
         out_nc[pix] = nc;
@@ -816,7 +839,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     __syncthreads();
     int nc = 0;
     RayT<T> ray = {0, 0, 1};
-    float A = 0.f, Z = 0.f, W = 0.f;
+    double A = 0.0;   // log T, stepped back in FP64 from the stored final value
+    float Z = 0.f, W = 0.f;
     float dCr = 0.f, dCg = 0.f, dCb = 0.f, dA = 0.f, dX = 0.f, dY = 0.f, dZ = 0.f, dW = 0.f, dV = 0.f;
     if (inside) {
         nc = ncontrib[pix];
@@ -914,8 +938,8 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
                         rgb[ch] = w0 * cx[ch] + w1 * ce[ch];
                     }
                     // undo this tet's blend
-                    const float Ap = A + x;
-                    const float T = __expf(Ap);
+                    const double Ap = A + (double)x;
+                    const float T = __expf((float)Ap);
                     const float te = h.se - mt, tx_ = h.sx - mt;
                     const float m1 = sig * (te + tx_) * 0.5f;
                     const float zp = Z - sig, wp = W - m1;
@@ -1117,6 +1141,309 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void composite_bwd_kernel(
     }
 }
 
+// ------------------------------------------------------------- backward, one lane per tet --
+//
+// The 4x4-tile backward with the roles of lanes and pixels exchanged: one wave per tile, a chunk
+// holds up to 64 of the tile's list entries (with FLAGS only the entries some pixel blended in the
+// forward, gathered densely across the list), lane j the j-th entry counted from the back of what
+// is left, its plane records, colour and cam - v0 in registers. The wave then walks the tile's 16
+// pixels; every lane evaluates its tet for the pixel and adds the (pixel, tet) adjoints to its own
+// 32 sums, so a (tile, tet) pair's sums are complete in one lane with no cross-lane reduction.
+//
+// The back-to-front walk of a pixel (bwd_alpha_blend, alphablend_shader_interp.slang:273-369)
+// carries log T, the distortion moments Z and W and the adjoints dA, dZ, dW from tet to tet; each
+// changes by a term of the tet it steps over. At lane j the walk has stepped over the entries of
+// the later chunks and lanes 0..j-1 of this one, so its state is the state at the back of the
+// chunk plus exclusive prefix sums over the lanes: one wave scan of (x, sigma, m1) and, as dA's
+// terms dT e^A need the transmittance, a second one of dT e^A. The state at the back of the chunk
+// lives in lane p of six registers for pixel p.
+
+// Inclusive prefix sum over the 64 lanes of a wave: DPP row shifts sum within each row of 16 lanes,
+// row broadcasts then carry the totals of rows 0 (into 1 and 3) and 0-1 (into 2 and 3).
+template <int CTRL, int ROWMASK>
+__device__ inline float dpp_rm(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+__device__ inline float wave_scan_incl(float x) {
+    x += dpp_rm<0x111, 0xF>(x);   // row_shr:1
+    x += dpp_rm<0x112, 0xF>(x);   // row_shr:2
+    x += dpp_rm<0x114, 0xF>(x);   // row_shr:4
+    x += dpp_rm<0x118, 0xF>(x);   // row_shr:8
+    x += dpp_rm<0x142, 0xA>(x);   // row_bcast:15 into rows 1 and 3
+    x += dpp_rm<0x143, 0xC>(x);   // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// Exclusive prefix sum over the wave: the values shifted one lane up (wave_shr:1; lane 0 reads 0),
+// then the inclusive scan, so that lane j's sum never contains its own term (subtracting it from
+// the inclusive sum would leave a rounding error of the size of that term).
+__device__ inline float wave_scan_excl(float x) { return wave_scan_incl(dpp_rm<0x138, 0xF>(x)); }
+
+// m ? a : b for a lane mask m of all ones or all zeros, as bit operations (kept branch-free)
+__device__ inline float bitsel(int m, float a, float b) {
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ inline float lane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double lane_f(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <int F, bool P64, bool FLAGS>
+__global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
+    const int* __restrict__ ncontrib, const float4* __restrict__ gimg, const float* __restrict__ gdist,
+    float* __restrict__ rows, const uint8_t* __restrict__ flags) {
+    constexpr int NGRD = NG_<F>::NGRD;
+    constexpr int NCH = F == 7 ? 1 : 3;
+    constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;   // dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], q[12]
+    constexpr int PU = 4 + NCH + 3 * NCH;
+    static_assert(NV <= kRowFloats, "row too small");
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    __shared__ int s_pos[128];                      // gathered list positions, a ring
+    __shared__ float s_sum[64][kRowFloats + 1];     // the chunk's per-lane sums, read back row-wise
+    __shared__ int s_tet[64];
+
+    const int lane = threadIdx.x;
+    const int tile = remap_block(blockIdx.x, gridDim.x);
+    const int tx = tile % cam.gw, ty = tile / cam.gw;
+    const int2 rg = load_range(ranges, true, tile, n_sorted);
+    const int n = rg.y - rg.x;
+
+    // pixel lane p < 16: the pixel's constants and its walk state at the back of the next chunk
+    const int px = tx * 4 + (lane & 3), py = ty * 4 + ((lane >> 2) & 3);
+    const bool inside = lane < 16 && px < cam.W && py < cam.H;
+    const int64_t pix = (int64_t)py * cam.W + px;
+    int pnc = 0;
+    RayT<T> pray = {0, 0, 1};
+    float sA = 0.f, sZ = 0.f, sW = 0.f, sdA = 0.f, sdZ = 0.f, sdW = 0.f;
+    float pdCr = 0.f, pdCg = 0.f, pdCb = 0.f, pdX = 0.f, pdY = 0.f, pdV = 0.f;
+    if (inside) {
+        pnc = ncontrib[pix];
+        const float2 j = jitter[pix];
+        pray = make_ray<T>(cam, px, py, j.x, j.y);
+        sA = img[pix].w;
+        sZ = dist[5 * pix + 2];
+        sW = dist[5 * pix + 3];
+        const float4 gi = gimg[pix];
+        pdCr = gi.x; pdCg = gi.y; pdCb = gi.z; sdA = gi.w;
+        const float* gd = gdist + 5 * pix;
+        pdX = gd[0]; pdY = gd[1]; sdZ = gd[2]; sdW = gd[3]; pdV = gd[4];
+    }
+    // entries at or beyond every pixel's n_contrib are never revisited
+    int nmax = pnc;
+    nmax = max(nmax, __shfl_xor(nmax, 1, 64));
+    nmax = max(nmax, __shfl_xor(nmax, 2, 64));
+    nmax = max(nmax, __shfl_xor(nmax, 4, 64));
+    nmax = max(nmax, __shfl_xor(nmax, 8, 64));
+    nmax = min(__builtin_amdgcn_readfirstlane(nmax), n);
+    const float mt = cam.min_t;
+
+    int hi = nmax;         // list positions [0, hi) not yet gathered
+    int head = 0, npend = 0;
+    for (;;) {
+        // gather up to 64 entries, back to front, into the ring
+        while (npend < 64 && hi > 0) {
+            const int k = hi - 1 - lane;
+            const bool f = k >= 0 && (!FLAGS || flags[rg.x + k] != 0);
+            const uint64_t m = __ballot(f);
+            if (f) s_pos[(head + npend + __popcll(m & ((1ull << lane) - 1ull))) & 127] = k;
+            npend += __popcll(m);
+            hi -= 64;
+        }
+        if (npend == 0) break;
+        const int cnt = npend < 64 ? npend : 64;
+        __syncthreads();
+        const bool valid = lane < cnt;
+        const int kpos = valid ? s_pos[(head + lane) & 127] : 0x7FFFFFFF;
+        head += cnt;
+        npend -= cnt;
+        const int kmin = __builtin_amdgcn_readlane(kpos, cnt - 1);   // the chunk's first list position
+
+        // this lane's tet: plane records, colour and cam - v0 in registers
+        PL pl[4];
+        float4 fa = make_float4(0.f, 0.f, 0.f, 0.f);
+        float gr[NGRD];
+        T cv0 = 0, cv1 = 0, cv2 = 0;
+        int tet = -1;
+        if (valid) {
+            tet = sorted[rg.x + kpos];
+            TetData<F> d;
+            load_tet_data<F>(d, indices[tet], tet, verts, feats);
+            tet_record<F, P64>(d, cam, pl, fa, gr);
+            cv0 = (T)cam.cam_pos[0] - (T)d.v[0].x;
+            cv1 = (T)cam.cam_pos[1] - (T)d.v[0].y;
+            cv2 = (T)cam.cam_pos[2] - (T)d.v[0].z;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {   // planes no ray crosses ahead of the camera: a miss
+                pl[i].x = 0; pl[i].y = 0; pl[i].z = 0; pl[i].w = 1;
+            }
+#pragma unroll
+            for (int k = 0; k < NGRD; ++k) gr[k] = 0.f;
+        }
+        const float sig = fa.x;
+        float acc[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] = 0.f;
+        bool touched = false;
+
+        for (int p = 0; p < 16; ++p) {
+            const int ncp = __builtin_amdgcn_readlane(pnc, p);
+            if (ncp <= kmin) continue;                         // no entry of the chunk is revisited
+            const RayT<T> ray = {lane_f(pray.dx, p), lane_f(pray.dy, p), lane_f(pray.dz, p)};
+            const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
+            const auto h = intersect(pl, ray, mt);
+            const float dt = h.dt;
+            const float x = sig * dt;
+            const bool act = valid && kpos < ncp && h.hit && x > 0.f;
+            const float te = h.se - mt, tx_ = h.sx - mt;
+            const float m1 = sig * (te + tx_) * 0.5f;
+            const float xa = act ? x : 0.f, sa = act ? sig : 0.f, ma = act ? m1 : 0.f;
+            const float ex = wave_scan_excl(xa), es = wave_scan_excl(sa), em = wave_scan_excl(ma);
+            const float A = lane_f(sA, p) + ex;
+            const float Z = lane_f(sZ, p) - es;
+            const float W = lane_f(sW, p) - em;
+            const float dXp = lane_f(pdX, p), dYp = lane_f(pdY, p), dVp = lane_f(pdV, p);
+            const float dZ = lane_f(sdZ, p) + dXp * 2.0f * em;
+            const float dW = lane_f(sdW, p) + dYp * 2.0f * es;
+            const float dC[3] = {lane_f(pdCr, p), lane_f(pdCg, p), lane_f(pdCb, p)};
+            float em1 = 0.f, alpha = 0.f, w0 = 0.f, w1 = 0.f, dphi = 0.f, Tr = 0.f, dT = 0.f;
+            float gd[3] = {0.f, 0.f, 0.f}, ce[3] = {0.f, 0.f, 0.f}, cx[3] = {0.f, 0.f, 0.f};
+            if (act) {
+                em1 = expm1_neg(x);
+                alpha = 1.0f + em1;
+                seg_weights_dphi(x, em1, alpha, w0, w1, dphi);
+                gd[0] = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
+                gd[1] = gd[0];
+                gd[2] = gd[0];
+                if (F == 13) {
+                    gd[1] = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
+                    gd[2] = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
+                }
+                const float bse[3] = {fa.y, fa.z, fa.w};
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    ce[ch] = bse[ch] + h.se * gd[ch];
+                    cx[ch] = bse[ch] + h.sx * gd[ch];
+                    dT += dC[ch] * (w0 * cx[ch] + w1 * ce[ch]);
+                }
+                Tr = __expf(A + x);    // transmittance in front of this tet
+            }
+            const float dtt = act ? dT * Tr : 0.f;
+            const float edt = wave_scan_excl(dtt);
+            const float dA = lane_f(sdA, p) + edt;
+            if (act) {
+                touched = true;
+                const float zp = Z - sig, wp = W - m1;
+                float drgb[3];
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) drgb[ch] = dC[ch] * Tr;
+                float dxx = -dA;
+                const float dm1 = dXp * 2.0f * zp + dW;
+                const float dm0 = dYp * 2.0f * wp + dZ;
+                float dsig = dm0 + dm1 * (te + tx_) * 0.5f + dVp * 2.0f * sig * dt * (1.0f / 3.0f);
+                float ddt = dVp * sig * sig * (1.0f / 3.0f);
+                float dse = dm1 * sig * 0.5f, dsx = dm1 * sig * 0.5f;
+                // entry / exit points relative to v0, formed in the plane precision from cam - v0
+                // and s d, so the sums below carry no cancellation against the camera distance
+                const T sec = h.se_p > (T)mt ? h.se_p : (T)mt;
+                const float qex = (float)(cv0 + sec * ray.dx), qey = (float)(cv1 + sec * ray.dy),
+                            qez = (float)(cv2 + sec * ray.dz);
+                const float qxx = (float)(cv0 + h.sx_p * ray.dx), qxy = (float)(cv1 + h.sx_p * ray.dy),
+                            qxz = (float)(cv2 + h.sx_p * ray.dz);
+                float dw0 = 0.f, dw1 = 0.f;
+                float dqe[3], dqx[3];
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    acc[1 + ch] += drgb[ch] * (w0 + w1);
+                    dqx[ch] = drgb[ch] * w0;
+                    dqe[ch] = drgb[ch] * w1;
+                    dw0 += drgb[ch] * cx[ch];
+                    dw1 += drgb[ch] * ce[ch];
+                }
+                dxx += alpha * dw0 + (dw0 - dw1) * dphi;
+                dsig += dxx * dt;
+                ddt += dxx * sig;
+                dsx += ddt;
+                dse -= ddt;
+                acc[0] += dsig;
+                if (F == 7) {
+                    const float qe = dqe[0] + dqe[1] + dqe[2], qx = dqx[0] + dqx[1] + dqx[2];
+                    acc[4] += qe + qx;
+                    acc[5] += qe * qex + qx * qxx;
+                    acc[6] += qe * qey + qx * qxy;
+                    acc[7] += qe * qez + qx * qxz;
+                    dse += qe * gd[0];
+                    dsx += qx * gd[0];
+                } else {
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        acc[4 + ch] += dqe[ch] + dqx[ch];
+                        acc[7 + 3 * ch] += dqe[ch] * qex + dqx[ch] * qxx;
+                        acc[8 + 3 * ch] += dqe[ch] * qey + dqx[ch] * qxy;
+                        acc[9 + 3 * ch] += dqe[ch] * qez + dqx[ch] * qxz;
+                        dse += dqe[ch] * gd[ch];
+                        dsx += dqx[ch] * gd[ch];
+                    }
+                }
+                // plane adjoints: s_i = -e_i / den_i; per plane the sums of u = ds / den and of
+                // u (x - v0) over its crossing points x (the entering plane only counts when its
+                // crossing lies beyond min_t, where the clamped entry is the crossing)
+                const float ue = (h.se_raw > mt) ? dse * rcp_(h.den_e) : 0.f;
+                const float ux = dsx * rcp_(h.den_x);
+                // the entering plane (den < 0) and the exiting one (den >= 0) are never the same
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int me = -(int)(h.ie == i), mx = -(int)(h.ix == i);
+                    const float u = __int_as_float((__float_as_int(ue) & me) | (__float_as_int(ux) & mx));
+                    acc[PU + i] += u;
+                    acc[PU + 4 + 3 * i] += u * bitsel(me, qex, qxx);
+                    acc[PU + 5 + 3 * i] += u * bitsel(me, qey, qxy);
+                    acc[PU + 6 + 3 * i] += u * bitsel(me, qez, qxz);
+                }
+            }
+            // pixel p's state at the back of the next chunk: add the whole chunk's terms
+            const float tX = lane_f(ex + xa, 63), tS = lane_f(es + sa, 63), tM = lane_f(em + ma, 63),
+                        tD = lane_f(edt + dtt, 63);
+            if (lane == p) {
+                sA += tX;
+                sZ -= tS;
+                sW -= tM;
+                sdZ += dXp * 2.0f * tM;
+                sdW += dYp * 2.0f * tS;
+                sdA += tD;
+            }
+        }
+
+        // the chunk's sums leave through LDS, so that each atomic instruction adds two whole rows
+        // (consecutive lanes on consecutive floats of a 128-byte row)
+        const uint64_t tm = __ballot(touched);
+        if (tm != 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) s_sum[lane][i] = acc[i];
+            s_tet[lane] = tet;
+            __syncthreads();
+            const int half = lane >> 5, pos = lane & 31;
+            const int j = row_sum_index(pos);
+            for (int s0 = 0; s0 < cnt; s0 += 2) {
+                if (((tm >> s0) & 3ull) == 0) continue;
+                const int s = s0 + half;
+                if (s < cnt && ((tm >> s) & 1ull) && j < NV) {
+                    const float v = s_sum[s][j];
+                    if (v != 0.f) unsafeAtomicAdd(rows + (int64_t)s_tet[s] * kRowFloats + pos, v);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int TS, int F, bool P64>
 int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st,
@@ -1236,7 +1563,18 @@ int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
         const int64_t n4 = n_tets * (kRowFloats / 4);
         hipLaunchKernelGGL(zero_f4_kernel, dim3((unsigned)ceil_div(n4, 256)), dim3(256), 0, st, (float4*)rows, n4);
     }
-    if constexpr (ROWS && C::GROUP <= 64) {
+    if constexpr (ROWS && TS == 4) {
+        if (flags)
+            hipLaunchKernelGGL((composite_bwd_tet_lanes_kernel<F, P64, true>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
+                               sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
+                               (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist, rows,
+                               flags);
+        else
+            hipLaunchKernelGGL((composite_bwd_tet_lanes_kernel<F, P64, false>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
+                               sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
+                               (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist, rows,
+                               (const uint8_t*)nullptr);
+    } else if constexpr (ROWS && C::GROUP <= 64) {
         if (flags) {
             hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS, true>), dim3(sgw * sgh), dim3(C::WG), 0, st,
                                a, sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,

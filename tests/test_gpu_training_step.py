@@ -19,7 +19,8 @@ Checked here:
   the hash grid; feature, vertex, MLP-weight and hash-table gradients against the FP64 chain
   (orc_composite_bwd on the sampled tiles, then the activation, MLP and hash-grid adjoints in
   numpy FP64 on the GPU's query points): per-element relative error <= 1e-3 with a floor of
-  1e-4 of the group's largest gradient.
+  1e-4 of the group's largest gradient (MLP weights: or within the FP32 summation bound of the
+  element's terms, for elements that cancel).
 """
 import numpy as np
 import pytest
@@ -211,12 +212,23 @@ def test_field_gradients(step):
     gh1 = (gh2 @ w2) * _selu_prime(f["h1"])
     ref.update(w1=gh1.T @ f["h0"], b1=gh1.sum(0))
     gh0 = gh1 @ w1
+    # magnitude of each element's terms: the same chain on absolute values. An element that is the
+    # sum of 13k terms cancelling to 1/1000 of their magnitude cannot come out of FP32 sums to 1e-3
+    # of itself; for those the bound is FP32 summation rounding, 64 * 2^-24 of the terms' magnitude
+    mg = np.abs(graw)
+    mh2 = (mg @ np.abs(w3)) * _selu_prime(f["h2"])
+    mh1 = (mh2 @ np.abs(w2)) * _selu_prime(f["h1"])
+    mag = {"w3": mg.T @ np.abs(f["a2"]), "b3": mg.sum(0), "w2": mh2.T @ np.abs(f["a1"]), "b2": mh2.sum(0),
+           "w1": mh1.T @ np.abs(f["h0"]), "b1": mh1.sum(0)}
     m = s["fld"].mlp
     for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
         g = getattr(m, name).grad.cpu().numpy().astype(np.float64)
         rel = _rel(g, ref[name])
-        print(f"MLP {name}: max per-element rel err vs FP64 {rel.max():.2e}")
-        assert rel.max() <= 1e-3, name
+        cond = np.abs(g - ref[name]) <= 64 * 2.0 ** -24 * mag[name]
+        print(f"MLP {name}: max per-element rel err vs FP64 {rel.max():.2e}; elements past 1e-3 within the FP32 "
+              f"summation bound: {int(((rel > 1e-3) & cond).sum())} of {rel.size}")
+        assert ((rel <= 1e-3) | cond).all(), name
+        assert (rel <= 1e-3).mean() >= 0.999, name
     g = s["fld"].grid
     gt = O.hashgrid_bwd(f["x"], gh0, g.res, g.offs, g.n_feat, g.log2_size, int(g.offs[-1]))
     gt_gpu = g.table.grad.cpu().numpy().astype(np.float64)
