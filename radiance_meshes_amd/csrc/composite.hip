@@ -1745,11 +1745,140 @@ __global__ __launch_bounds__(256) void tet_err_finish_kernel(float4* __restrict_
     err[4 * t + 3] = make_float4(r0.x, 0.5f * (r1.z + r2.y), 0.5f * (r1.w + r2.z), 0.5f * (r2.x + r2.w));
 }
 
+// The 4x4-tile votes with one lane per tet: one wave per tile walks the list front to back in
+// chunks of 64 entries, lane j holding entry j of the chunk (its plane records and sigma in
+// registers) and summing its tet's votes over the tile's 16 pixels. A pixel's log T in front of
+// lane j is its log T at the chunk start minus the exclusive prefix sum of the blended x over lanes
+// 0..j-1; the walk's stop (after the blend that takes log T below -5.5) makes every lane from the
+// first one whose log T in front is below -5.5 vote nothing, the pixel ending there.
+template <int F, bool P64>
+__global__ __launch_bounds__(64) void tet_err_tet_lanes_kernel(
+    CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
+    const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
+    const float* __restrict__ pixel_err, const float* __restrict__ ssim_err, float* __restrict__ tet_err,
+    int* __restrict__ tet_count) {
+    using T = typename Prec<P64>::R;
+    using PL = typename Prec<P64>::PL;
+    constexpr int NVOTE = 12;                   // row channels 0-11 (3: the vote count, 4: none)
+    __shared__ float s_row[64][NVOTE + 1];
+    __shared__ int s_tet[64];
+    const int lane = threadIdx.x;
+    const int tile = remap_block(blockIdx.x, gridDim.x);
+    const int tx = tile % cam.gw, ty = tile / cam.gw;
+    const int2 rg = load_range(ranges, true, tile, n_sorted);
+    const int n = rg.y - rg.x;
+    // pixel lane p < 16: its ray, errors and walk state
+    const int px = tx * 4 + (lane & 3), py = ty * 4 + ((lane >> 2) & 3);
+    const bool inside = lane < 16 && px < cam.W && py < cam.H;
+    RayT<T> pray = {0, 0, 1};
+    float perr = 0.f, serr = 0.f, plogT = 1.0f;
+    if (inside) {
+        const int64_t pix = (int64_t)py * cam.W + px;
+        pray = make_ray<T>(cam, px, py, 0.5f, 0.5f);
+        perr = pixel_err[pix];
+        serr = ssim_err[pix];
+    }
+    uint32_t alive = (uint32_t)__ballot(inside) & 0xFFFFu;   // pixels still walking
+    const float mt = cam.min_t;
+    const float3 cp = f3(cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2]);
+    for (int c0 = 0; c0 < n && alive != 0; c0 += 64) {
+        const int k = c0 + lane;
+        const bool valid = k < n;
+        PL pl[4];
+        float sig = 0.f;
+        int tet = -1;
+        if (valid) {
+            tet = sorted[rg.x + k];
+            const int4 q = indices[tet];
+            const float3 v0 = load_v(verts, q.x), v1 = load_v(verts, q.y), v2 = load_v(verts, q.z),
+                         v3 = load_v(verts, q.w);
+            plane_rec<T>(v0, v1, v2, v3, cp, pl[0]);
+            plane_rec<T>(v0, v1, v3, v2, cp, pl[1]);
+            plane_rec<T>(v0, v2, v3, v1, cp, pl[2]);
+            plane_rec<T>(v1, v2, v3, v0, cp, pl[3]);
+            sig = fmaxf(feats[(int64_t)F * tet], 0.f);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {   // planes no ray crosses ahead of the camera: a miss
+                pl[i].x = 0; pl[i].y = 0; pl[i].z = 0; pl[i].w = 1;
+            }
+        }
+        float v[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) v[i] = 0.f;
+        int votes = 0, wmax = 0;
+        for (int p = 0; p < 16; ++p) {
+            if (!((alive >> p) & 1u)) continue;
+            const RayT<T> ray = {lane_f(pray.dx, p), lane_f(pray.dy, p), lane_f(pray.dz, p)};
+            const auto h = intersect(pl, ray, mt);
+            const float x = sig * h.dt;
+            const bool hit = valid && h.hit && x > 0.f;
+            const float xa = hit ? x : 0.f;
+            const float ex = wave_scan_excl(xa);
+            const float lt = lane_f(plogT, p) - ex;           // log T in front of this entry
+            const uint64_t stop = __ballot(valid && lt < -5.5f);
+            const int first = stop ? __builtin_ctzll(stop) : 64;
+            if (hit && lane < first) {
+                const float w = __expf(lt) * x;   // hardware exponential: ~4e-7 relative for log T in [-5.5, 1]
+                const float e = lane_f(perr, p), s2 = lane_f(serr, p);
+                const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
+                v[0] += w;
+                v[1] += w * e;
+                v[2] += w * e * e;
+                v[3] += w * s2;
+                v[4] += w * (cp.x + h.se * rdx);
+                v[5] += w * (cp.y + h.se * rdy);
+                v[6] += w * (cp.z + h.se * rdz);
+                v[7] += w * (cp.x + h.sx * rdx);
+                v[8] += w * (cp.y + h.sx * rdy);
+                v[9] += w * (cp.z + h.sx * rdz);
+                ++votes;
+                wmax = max(wmax, (int)(65535.f * w));
+            }
+            if (first < 64) {
+                alive &= ~(1u << p);                          // the pixel stopped inside this chunk
+            } else {
+                const float nl = lane_f(plogT, p) - lane_f(ex + xa, 63);
+                if (lane == p) plogT = nl;
+                if (nl < -5.5f) alive &= ~(1u << p);
+            }
+        }
+        // rows leave through LDS: each atomic instruction adds four 64-byte rows, 16 lanes each
+        const uint64_t vm = __ballot(votes > 0);
+        if (vm != 0) {
+            s_row[lane][0] = v[0]; s_row[lane][1] = v[1]; s_row[lane][2] = v[2];
+            s_row[lane][3] = (float)votes;    // the vote count, moved out by the finish kernel
+            s_row[lane][4] = 0.f;             // channel 4 carries no vote
+            s_row[lane][5] = v[3];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s_row[lane][6 + i] = v[4 + i];
+            s_tet[lane] = tet;
+            if (votes > 0) atomicMax(tet_count + 2 * (int64_t)tet + 1, wmax);
+            __syncthreads();
+            const int ch = lane & 15;
+            for (int s0 = 0; s0 < 64; s0 += 4) {
+                if (((vm >> s0) & 0xFull) == 0) continue;
+                const int s = s0 + (lane >> 4);
+                if (((vm >> s) & 1ull) && ch < NVOTE && ch != 4) {
+                    const float val = s_row[s][ch];
+                    if (val != 0.f) unsafeAtomicAdd(tet_err + 16 * (int64_t)s_tet[s] + ch, val);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <int TS, int F, bool P64>
 int launch_tet_err(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices,
                    const float* verts, const float* feats, const float* perr, const float* serr, float* err,
                    int* count, hipStream_t st) {
     using C = Cfg<TS>;
+    if constexpr (TS == 4) {
+        hipLaunchKernelGGL((tet_err_tet_lanes_kernel<F, P64>), dim3(a.gw * a.gh), dim3(64), 0, st, a, sorted,
+                           n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, perr, serr, err, count);
+        return 0;
+    }
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
     hipLaunchKernelGGL((tet_err_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted, n_sorted,
                        (const int2*)ranges, (const int4*)indices, verts, feats, perr, serr, err, count);
