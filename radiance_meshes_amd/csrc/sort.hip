@@ -332,22 +332,23 @@ __global__ void zero_ranges_kernel(int2* ranges, int64_t G) {
     if (i < G) ranges[i] = make_int2(0, 0);
 }
 
+// Grid-stride over the sorted entries (a launch sized for a capacity may use fewer workgroups).
 template <typename TileOf>
 __global__ __launch_bounds__(kThreads) void ranges_kernel(TileOf tile_of, int64_t cap, const int64_t* n_dev,
                                                           int32_t* ranges) {
     const int64_t n = eff_n(cap, n_dev);
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t cur = tile_of(i);
-    if (i == 0) ranges[2 * (int64_t)cur] = 0;
-    else {
-        uint32_t prev = tile_of(i - 1);
-        if (cur != prev) {
-            ranges[2 * (int64_t)prev + 1] = (int32_t)i;
-            ranges[2 * (int64_t)cur] = (int32_t)i;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t cur = tile_of(i);
+        if (i == 0) ranges[2 * (int64_t)cur] = 0;
+        else {
+            uint32_t prev = tile_of(i - 1);
+            if (cur != prev) {
+                ranges[2 * (int64_t)prev + 1] = (int32_t)i;
+                ranges[2 * (int64_t)cur] = (int32_t)i;
+            }
         }
+        if (i == n - 1) ranges[2 * (int64_t)cur + 1] = (int32_t)n;
     }
-    if (i == n - 1) ranges[2 * (int64_t)cur + 1] = (int32_t)n;
 }
 
 struct TileOfKey {
@@ -383,11 +384,16 @@ __global__ __launch_bounds__(kThreads) void bin_count_kernel(const int32_t* __re
     }
 }
 
-// Order-preserving compaction of the visible tets into (sortable depth key, tet id).
+__device__ inline uint32_t blocks_of_rect(int4 r);
+
+// Order-preserving compaction of the visible tets into (sortable depth key, tet id); with bcnt,
+// also each visible tet's number of overlapped 8x8-tile blocks (indexed by tet).
 __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __restrict__ tiles,
                                                            const float* __restrict__ vs, int dstride, int64_t T,
                                                            const uint32_t* __restrict__ blk_vis_off,
-                                                           uint32_t* __restrict__ vkey, uint32_t* __restrict__ vidx) {
+                                                           uint32_t* __restrict__ vkey, uint32_t* __restrict__ vidx,
+                                                           const int4* __restrict__ rect = nullptr,
+                                                           uint32_t* __restrict__ bcnt = nullptr) {
     __shared__ uint32_t lds[4];
     int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     bool vis[kScanItems];
@@ -404,6 +410,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __rest
         if (vis[k]) {
             vkey[pos] = sortable_key(vs[dstride * (base + k) + (dstride - 1)]);
             vidx[pos] = (uint32_t)(base + k);
+            if (bcnt) bcnt[base + k] = blocks_of_rect(rect[base + k]);
             ++pos;
         }
 }
@@ -439,11 +446,15 @@ struct TileMapBatch {   // global tile id of the tet's view in a batch
     }
 };
 
-template <typename TileMap>
+struct RectTiles {  // the tet's rect of tiles
+    __device__ int4 operator()(int4 r) const { return r; }
+};
+
+template <typename TileMap, typename RectMap = RectTiles>
 __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ poff, const int4* __restrict__ rect,
     int64_t nvis_cap, const int64_t* nvis_dev, int64_t P_cap, const int64_t* P_dev, TileMap tile_of,
-    uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet) {
+    uint32_t* __restrict__ ptile, uint32_t* __restrict__ ptet, RectMap rect_of = RectMap{}) {
     __shared__ uint32_t s_off[kPairsPerBlock];
     __shared__ int4 s_rect[kPairsPerBlock];
     __shared__ uint32_t s_tet[kPairsPerBlock];
@@ -467,7 +478,7 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
         uint32_t t = order[lo + j];
         s_off[j] = poff[lo + j];
         s_tet[j] = t;
-        s_rect[j] = rect[t];
+        s_rect[j] = rect_of(rect[t]);
     }
     __syncthreads();
     for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
@@ -487,19 +498,243 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     }
 }
 
+// ---- tile lists without a sort of the pairs: blocks of 8x8 tiles --------------------------
+//
+// The depth-ordered visible tets are first expanded into (block, tet) pairs over the blocks of
+// 8x8 tiles their rects overlap (about 1.8 per tet at 4x4-pixel tiles, a quarter of the tile
+// pairs) and stably sorted by block (two radix passes over the block id). One workgroup per block
+// then walks the block's tets, still in depth order: a first walk counts the pairs of each of its
+// 64 tiles, a scan over all tiles gives each tile's first position, and a second walk writes every
+// tet to its tiles' lists directly. Within a chunk of 256 entries an entry's place in a tile's
+// list is the number of earlier entries that cover the tile: each lane packs its coverage of the
+// 64 tiles as 16 registers of four 8-bit counters and a wave-wide inclusive scan of those
+// registers sums all 64 counters at once (a field never exceeds 255). The lists equal those of a
+// stable sort of the (tile, tet) pairs by tile.
+constexpr int kBinBlock = 8;                         // tiles per block side
+constexpr int kBinLocal = kBinBlock * kBinBlock;     // tiles per block
+constexpr int kBinRegs = kBinLocal / 4;              // packed 8-bit counters per lane
+
+__device__ inline int4 block_rect_of(int4 r) {       // rect of tiles (half-open) -> rect of blocks
+    return make_int4(r.x / kBinBlock, r.y / kBinBlock, (r.z - 1) / kBinBlock + 1, (r.w - 1) / kBinBlock + 1);
+}
+struct RectBlocks {
+    __device__ int4 operator()(int4 r) const { return block_rect_of(r); }
+};
+__device__ inline uint32_t blocks_of_rect(int4 r) {
+    const int4 b = block_rect_of(r);
+    return (uint32_t)((b.z - b.x) * (b.w - b.y));
+}
+struct LoadBlockCount {  // blocks overlapped by tet order[i] (bcnt is indexed by tet)
+    const uint32_t* bcnt;
+    const uint32_t* order;
+    __device__ uint32_t operator()(int64_t i) const { return bcnt[order[i]]; }
+};
+
+// The tet's tiles within block (bx, by), as local tile coordinates [lx0, lx1) x [ly0, ly1).
+__device__ inline int4 local_rect(int4 r, int bx, int by) {
+    const int ox = bx * kBinBlock, oy = by * kBinBlock;
+    return make_int4(max(r.x - ox, 0), max(r.y - oy, 0), min(r.z - ox, kBinBlock), min(r.w - oy, kBinBlock));
+}
+
+// A block's list is cut into segments of kSegEntries; seg_first[b] (an exclusive scan over the
+// blocks) numbers them, so that every segment is one workgroup of the count and emit kernels.
+constexpr int kSegEntries = kThreads;
+
+struct LoadSegCount {  // segments of block i
+    const int2* branges;
+    __device__ uint32_t operator()(int64_t i) const {
+        const int2 r = branges[i];
+        return (uint32_t)((r.y - r.x + kSegEntries - 1) / kSegEntries);
+    }
+};
+
+// seg_desc[g] = (first entry, end entry, block, block's first segment) of segment g; one thread
+// per block writes its segments' descriptors.
+__global__ __launch_bounds__(kThreads) void seg_desc_kernel(const int2* __restrict__ branges,
+                                                            const uint32_t* __restrict__ seg_first, int64_t nb,
+                                                            int4* __restrict__ seg_desc) {
+    const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b >= nb) return;
+    const int2 r = branges[b];
+    const uint32_t f = seg_first[b], n = (uint32_t)((r.y - r.x + kSegEntries - 1) / kSegEntries);
+    for (uint32_t j = 0; j < n; ++j) {
+        const int e0 = r.x + (int)j * kSegEntries;
+        seg_desc[f + j] = make_int4(e0, min(e0 + kSegEntries, r.y), (int)b, (int)f);
+    }
+}
+
+// Workgroups of the segment kernels: one per segment up to an estimate of the segment count from
+// the pair count (the blocks hold about a quarter of the tile pairs at 8x8 tiles per block), each
+// looping over further segments in grid strides when there are more.
+inline int64_t seg_grid(int64_t n_pairs, int64_t nb) { return ceil_div(n_pairs, 4 * kSegEntries) + nb; }
+
+// Column-major per-segment counts, 0 past the last segment (those entries are never written).
+struct LoadSegColumn {
+    const uint32_t* cnt;
+    int64_t stride;
+    const int64_t* nseg_dev;
+    __device__ uint32_t operator()(int64_t i) const { return (i % stride) < *nseg_dev ? cnt[i] : 0u; }
+};
+
+// Pairs of each of the block's 64 tiles within segment g, column-major: cnt[t * stride + g].
+__global__ __launch_bounds__(kThreads) void seg_count_kernel(const uint32_t* __restrict__ blist,
+                                                             const int4* __restrict__ seg_desc,
+                                                             const int64_t* __restrict__ nseg_dev,
+                                                             const int4* __restrict__ rect, int32_t bgw,
+                                                             int64_t stride, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t s_cnt[kBinLocal];
+    const int64_t nseg = *nseg_dev;
+    const int tid = threadIdx.x;
+    for (int64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const int4 d = seg_desc[g];
+        if (tid < kBinLocal) s_cnt[tid] = 0;
+        __syncthreads();
+        const int64_t i = (int64_t)d.x + tid;
+        if (i < d.y) {
+            const int4 l = local_rect(rect[blist[i]], d.z % bgw, d.z / bgw);
+            for (int y = l.y; y < l.w; ++y)
+                for (int x = l.x; x < l.z; ++x) atomicAdd(&s_cnt[y * kBinBlock + x], 1u);
+        }
+        __syncthreads();
+        if (tid < kBinLocal) cnt[(int64_t)tid * stride + g] = s_cnt[tid];
+    }
+}
+
+// Pairs per tile (from the column prefix sums at the block's first and one-past-last segment).
+__global__ __launch_bounds__(kThreads) void tile_total_kernel(const uint32_t* __restrict__ cpre,
+                                                              const int2* __restrict__ branges,
+                                                              const uint32_t* __restrict__ seg_first,
+                                                              int32_t gw, int32_t gh, int32_t bgw, int64_t stride,
+                                                              uint32_t* __restrict__ tcount) {
+    const int64_t tile = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (tile >= (int64_t)gw * gh) return;
+    const int tx = (int)(tile % gw), ty = (int)(tile / gw);
+    const int b = (ty / kBinBlock) * bgw + tx / kBinBlock;
+    const int t = (ty % kBinBlock) * kBinBlock + tx % kBinBlock;
+    const int2 r = branges[b];
+    const uint32_t f = seg_first[b], e = f + (uint32_t)((r.y - r.x + kSegEntries - 1) / kSegEntries);
+    tcount[tile] = cpre[t * stride + e] - cpre[t * stride + f];
+}
+
+// ranges[t] = [tstart, tstart + tcount) for tiles with pairs, (0, 0) otherwise.
+__global__ __launch_bounds__(kThreads) void tile_ranges_kernel(const uint32_t* __restrict__ tcount,
+                                                               const uint32_t* __restrict__ tstart, int64_t G,
+                                                               int32_t* __restrict__ ranges) {
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= G) return;
+    const uint32_t c = tcount[t], st = tstart[t];
+    ranges[2 * t] = c ? (int32_t)st : 0;
+    ranges[2 * t + 1] = c ? (int32_t)(st + c) : 0;
+}
+
+// Inclusive sum over the wave of packed counters (DPP row shifts, then row broadcasts).
+template <int CTRL, int ROWMASK>
+__device__ inline uint32_t dpp_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, false);
+}
+__device__ inline uint32_t wave_scan_u32(uint32_t x) {
+    x += dpp_u<0x111, 0xF>(x);   // row_shr:1
+    x += dpp_u<0x112, 0xF>(x);   // row_shr:2
+    x += dpp_u<0x114, 0xF>(x);   // row_shr:4
+    x += dpp_u<0x118, 0xF>(x);   // row_shr:8
+    x += dpp_u<0x142, 0xA>(x);   // row_bcast:15 into rows 1 and 3
+    x += dpp_u<0x143, 0xC>(x);   // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// Bits 0..3 of b spread to bytes 0..3.
+__device__ inline uint32_t spread4(uint32_t b) {
+    return (b & 1u) | ((b & 2u) << 7) | ((b & 4u) << 14) | ((b & 8u) << 21);
+}
+
+// Writes each tet of segment g into the lists of the block's tiles it covers. A tile's first free
+// position: its start, plus the pairs of the block's earlier segments (column prefix sums), plus
+// the earlier entries of this segment that cover it (packed wave scans).
+__global__ __launch_bounds__(kThreads) void seg_emit_kernel(const uint32_t* __restrict__ blist,
+                                                            const int4* __restrict__ seg_desc,
+                                                            const int64_t* __restrict__ nseg_dev,
+                                                            const int4* __restrict__ rect, int32_t gw, int32_t gh,
+                                                            int32_t bgw, int64_t stride,
+                                                            const uint32_t* __restrict__ cpre,
+                                                            const uint32_t* __restrict__ tstart, int64_t n_pairs,
+                                                            int32_t* __restrict__ out) {
+    __shared__ uint32_t s_run[kBinLocal];                 // first free position of each tile's list
+    __shared__ uint32_t s_wtot[kThreads / 64][kBinRegs];  // per wave: the segment's packed counts
+    __shared__ uint32_t s_excl[kThreads][kBinRegs + 1];   // per lane: earlier entries covering each tile
+    const int64_t nseg = *nseg_dev;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const int4 d = seg_desc[g];
+        const int bx = d.z % bgw, by = d.z / bgw;
+        const int64_t i = (int64_t)d.x + tid;
+        int4 l = make_int4(0, 0, 0, 0);
+        uint32_t tet = 0;
+        if (i < d.y) {
+            tet = blist[i];
+            l = local_rect(rect[tet], bx, by);
+        }
+        if (tid < kBinLocal) {
+            const int tx = bx * kBinBlock + (tid % kBinBlock), ty = by * kBinBlock + (tid / kBinBlock);
+            uint32_t st = 0;
+            if (tx < gw && ty < gh)
+                st = tstart[(int64_t)ty * gw + tx] + cpre[tid * stride + g] - cpre[tid * stride + d.w];
+            s_run[tid] = st;
+        }
+        const uint32_t xb = ((1u << l.z) - 1u) & ~((1u << l.x) - 1u);   // covered columns
+        const uint32_t lo = spread4(xb & 15u), hi = spread4(xb >> 4);
+        uint32_t m[kBinRegs], sc[kBinRegs];
+#pragma unroll
+        for (int k = 0; k < kBinRegs; ++k) {
+            const int y = k >> 1;
+            m[k] = (y >= l.y && y < l.w) ? ((k & 1) ? hi : lo) : 0u;
+            sc[k] = wave_scan_u32(m[k]);
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int k = 0; k < kBinRegs; ++k) s_wtot[w][k] = sc[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBinRegs; ++k) {
+            uint32_t pre = 0;
+            for (int q = 0; q < w; ++q) pre += s_wtot[q][k];   // at most 192 per field
+            s_excl[tid][k] = sc[k] - m[k] + pre;                 // at most 255 per field
+        }
+        if (i < d.y) {
+            for (int y = l.y; y < l.w; ++y)
+                for (int x = l.x; x < l.z; ++x) {
+                    const int t = y * kBinBlock + x;
+                    const uint32_t rank = (s_excl[tid][t >> 2] >> (8 * (t & 3))) & 255u;
+                    const int64_t pos = (int64_t)s_run[t] + rank;
+                    if (pos < n_pairs) out[pos] = (int32_t)tet;   // always, when the counts add up to n_pairs
+                }
+        }
+        __syncthreads();   // s_run, s_wtot and s_excl are rewritten for the next segment
+    }
+}
+
 // Fused-path workspace layout (offsets in bytes). The count phase only uses the first region.
 struct BinLayout {
     size_t blk_vis, blk_pairs, partial, counts2, vkey, vidx, vkey2, vidx2, vkey3, vidx3, poff, ptile, ptet,
-        ptile2, ptile3, ptet3, rcounts, total;
+        ptile2, ptile3, ptet3, rcounts, ptet2, branges, tcount, tstart, seg_first, seg_block, seg_cnt, seg_pre,
+        bcnt, total;
 };
 
-inline BinLayout bin_layout(int64_t T, int64_t P) {
+// Segments of the block lists: at most ceil(P / kSegEntries) plus one partial segment per block
+// (at most one block per tile); each of the 64 per-tile columns has one extra entry past that.
+inline int64_t seg_capacity(int64_t P, int64_t G) { return ceil_div(P, kSegEntries) + G; }
+
+// G (tiles) sizes the block binning's per-tile and per-block buffers; 0 leaves them out.
+inline BinLayout bin_layout(int64_t T, int64_t P, int64_t G = 0) {
     BinLayout L;
     size_t o = 0;
     int64_t nbT = scan_blocks(T > 0 ? T : 1);
     int64_t big = T > P ? T : P;
     int64_t nb_rs = rs_blocks(big > 0 ? big : 1);
     int64_t scan_n = (256 * nb_rs) > (T + 1) ? 256 * nb_rs : T + 1;
+    scan_n = scan_n > G + 1 ? scan_n : G + 1;
+    const int64_t seg_stride = G > 0 ? seg_capacity(P, G) + 1 : 0;
+    scan_n = scan_n > kBinLocal * seg_stride ? scan_n : kBinLocal * seg_stride;
     L.blk_vis = o; o += align256(4 * (size_t)nbT);
     L.blk_pairs = o; o += align256(4 * (size_t)nbT);
     L.partial = o; o += scan_ws_bytes(scan_n);
@@ -517,6 +752,15 @@ inline BinLayout bin_layout(int64_t T, int64_t P) {
     L.ptile2 = o; o += align256(4 * (size_t)P);
     L.ptile3 = o; o += align256(4 * (size_t)P);
     L.ptet3 = o; o += align256(4 * (size_t)P);
+    L.ptet2 = o; o += G > 0 ? align256(4 * (size_t)P) : 0;
+    L.branges = o; o += align256(8 * (size_t)G);       // blocks: at most one per tile
+    L.tcount = o; o += align256(4 * (size_t)G);
+    L.tstart = o; o += align256(4 * (size_t)G);
+    L.seg_first = o; o += align256(4 * (size_t)G);
+    L.seg_block = o; o += G > 0 ? align256(16 * (size_t)seg_stride) : 0;   // segment descriptors (int4)
+    L.bcnt = o; o += G > 0 ? align256(4 * (size_t)T) : 0;
+    L.seg_cnt = o; o += align256(4 * (size_t)(kBinLocal * seg_stride));
+    L.seg_pre = o; o += align256(4 * (size_t)(kBinLocal * seg_stride));
     L.total = o;
     return L;
 }
@@ -651,8 +895,7 @@ extern "C" int rm_tile_ranges(const int64_t* sorted_keys_dev, int64_t n, int32_t
 }
 
 extern "C" size_t rm_bin_workspace_size(int64_t n_tets, int64_t n_pairs, int64_t n_tiles) {
-    (void)n_tiles;
-    return bin_layout(n_tets, n_pairs).total;
+    return bin_layout(n_tets, n_pairs, n_tiles > 0 ? n_tiles : 1).total;
 }
 
 extern "C" int rm_bin_count(const int32_t* tiles_touched_dev, int64_t n_tets, int64_t* counts_dev,
@@ -690,16 +933,16 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     RM_CHECK_ARG(n_pairs >= 0 && n_pairs < (1ll << 31), "rm_bin_sort: n_pairs out of range");
     RM_CHECK_ARG(grid_height > 0 && grid_width > 0, "rm_bin_sort: bad grid");
     RM_CHECK_ARG(ranges_dev && workspace_dev, "rm_bin_sort: null buffer");
-    BinLayout L = bin_layout(n_tets, n_pairs);
+    int64_t G = (int64_t)grid_height * grid_width;
+    BinLayout L = bin_layout(n_tets, n_pairs, G);
     if (workspace_bytes < L.total) {
         set_error("rm_bin_sort: workspace too small");
         return RM_ERR_WORKSPACE_TOO_SMALL;
     }
     hipStream_t st = as_stream(stream);
-    int64_t G = (int64_t)grid_height * grid_width;
-    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
-                       (int2*)ranges_dev, G);
     if (n_pairs == 0 || n_visible == 0) {
+        hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                           (int2*)ranges_dev, G);
         RM_CHECK_LAUNCH("rm_bin_sort");
         return RM_OK;
     }
@@ -711,27 +954,61 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     // 1. compact visible tets (blk_vis holds exclusive offsets after rm_bin_count)
     hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles_touched_dev, vs_dev, 3,
                        n_tets, (const uint32_t*)WS(uint32_t, L.blk_vis), WS(uint32_t, L.vkey),
-                       WS(uint32_t, L.vidx));
+                       WS(uint32_t, L.vidx), (const int4*)rect_dev, WS(uint32_t, L.bcnt));
     // 2. stable sort of the visible tets by depth key (32 bits)
     radix_sort<uint32_t>(WS(uint32_t, L.vkey), WS(uint32_t, L.vidx), WS(uint32_t, L.vkey2), WS(uint32_t, L.vidx2),
                          WS(uint32_t, L.vkey3), WS(uint32_t, L.vidx3), n_visible, 0, 32, rcounts, partial, st);
     const uint32_t* order = WS(uint32_t, L.vidx2);
-    // 3. pair offsets in depth order
-    device_scan(LoadGather{tiles_touched_dev, order}, n_visible, WS(uint32_t, L.poff), false, partial,
-                (int64_t*)nullptr, st);
-    // 4. expand pairs
-    hipLaunchKernelGGL(emit_pairs_kernel<TileMapSingle>, dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)),
+    // 3. (block, tet) pair offsets in depth order; the pair count stays on the device (at most
+    //    n_pairs: a tet overlaps no more blocks than tiles)
+    const int32_t bgw = (grid_width + kBinBlock - 1) / kBinBlock, bgh = (grid_height + kBinBlock - 1) / kBinBlock;
+    const int64_t NB = (int64_t)bgw * bgh;
+    int64_t* nbp = WS(int64_t, L.counts2);
+    device_scan(LoadBlockCount{(const uint32_t*)WS(uint32_t, L.bcnt), order}, n_visible, WS(uint32_t, L.poff), false,
+                partial, nbp, st);
+    // 4. expand (block, tet) pairs in depth order and sort them stably by block
+    hipLaunchKernelGGL((emit_pairs_kernel<TileMapSingle, RectBlocks>), dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)),
                        dim3(kThreads), 0, st, order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev,
-                       n_visible, (const int64_t*)nullptr, n_pairs, (const int64_t*)nullptr,
-                       TileMapSingle{grid_width}, WS(uint32_t, L.ptile), WS(uint32_t, L.ptet));
-    // 5. stable sort of pairs by tile id over bits [0, bit_length(G))
-    int msb = bit_length(G);
-    radix_sort<uint32_t>(WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), WS(uint32_t, L.ptile2),
-                         (uint32_t*)sorted_tet_dev, WS(uint32_t, L.ptile3), WS(uint32_t, L.ptet3), n_pairs, 0,
-                         msb, rcounts, partial, st);
-    // 6. ranges from the sorted tile ids
-    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)ceil_div(n_pairs, kThreads)), dim3(kThreads), 0,
-                       st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, (const int64_t*)nullptr, ranges_dev);
+                       n_visible, (const int64_t*)nullptr, n_pairs, (const int64_t*)nbp, TileMapSingle{bgw},
+                       WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), RectBlocks{});
+    radix_sort<uint32_t>(WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), WS(uint32_t, L.ptile2), WS(uint32_t, L.ptet2),
+                         WS(uint32_t, L.ptile3), WS(uint32_t, L.ptet3), n_pairs, 0, bit_length(NB - 1) > 0 ? bit_length(NB - 1) : 1, rcounts,
+                         partial, st, nbp);
+    int2* branges = WS(int2, L.branges);
+    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st, branges, NB);
+    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)std::min<int64_t>(ceil_div(n_pairs, kThreads), 8192)),
+                       dim3(kThreads), 0, st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, (const int64_t*)nbp,
+                       (int32_t*)branges);
+    // 5. the block lists cut into segments; per segment and tile the pairs, prefix sums of those
+    //    per tile over the segments, pairs and first position of each tile, then every tet
+    //    written into its tiles' lists
+    const uint32_t* blist = WS(uint32_t, L.ptet2);
+    uint32_t* seg_first = WS(uint32_t, L.seg_first);
+    int64_t* nsegp = nbp + 1;
+    device_scan(LoadSegCount{(const int2*)branges}, NB, seg_first, false, partial, nsegp, st);
+    const int64_t segcap = seg_capacity(n_pairs, G), stride = segcap + 1;
+    uint32_t* cnt = WS(uint32_t, L.seg_cnt);
+    uint32_t* cpre = WS(uint32_t, L.seg_pre);
+    int4* seg_desc = WS(int4, L.seg_block);
+    const unsigned sgrid = (unsigned)std::min(seg_grid(n_pairs, NB), segcap);
+    hipLaunchKernelGGL(seg_desc_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st,
+                       (const int2*)branges, (const uint32_t*)seg_first, NB, seg_desc);
+    hipLaunchKernelGGL(seg_count_kernel, dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
+                       (const int64_t*)nsegp, (const int4*)rect_dev, bgw, stride, cnt);
+    device_scan(LoadSegColumn{cnt, stride, (const int64_t*)nsegp}, kBinLocal * stride, cpre, false, partial,
+                (int64_t*)nullptr, st);
+    uint32_t* tcount = WS(uint32_t, L.tcount);
+    uint32_t* tstart = WS(uint32_t, L.tstart);
+    hipLaunchKernelGGL(tile_total_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (const uint32_t*)cpre, (const int2*)branges, (const uint32_t*)seg_first, grid_width,
+                       grid_height, bgw, stride, tcount);
+    device_scan(LoadU32{tcount}, G, tstart, false, partial, (int64_t*)nullptr, st);
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (const uint32_t*)tcount, (const uint32_t*)tstart, G, ranges_dev);
+    hipLaunchKernelGGL(seg_emit_kernel, dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
+                       (const int64_t*)nsegp, (const int4*)rect_dev,
+                       grid_width, grid_height, bgw, stride, (const uint32_t*)cpre, (const uint32_t*)tstart, n_pairs,
+                       sorted_tet_dev);
     RM_CHECK_LAUNCH("rm_bin_sort");
     return RM_OK;
 }

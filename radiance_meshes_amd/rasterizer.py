@@ -119,6 +119,13 @@ class VertexShader(torch.autograd.Function):
         return None, grad_vertices, None, None
 
 
+# Binning workspace kept between calls, one per (device, stream): the count phase leaves its results
+# at the start of the workspace and the sort phase needs room for the pairs, so a workspace that is
+# already large enough saves allocating a larger one and copying those results into it. Calls on one
+# stream run in order, so reusing the buffer there is safe.
+_bin_ws = {}
+
+
 def bin_and_sort(tiles_touched, rect_tile_space, vs_tetra, render_grid):
     """Fused binning: (tile, depth)-ordered tet list and per-tile [start, end) ranges, bit-identical
     to the reference's keys + stable CUB sort + compute_tile_ranges (tile_shader_slang.py:36-68)."""
@@ -128,16 +135,21 @@ def bin_and_sort(tiles_touched, rect_tile_space, vs_tetra, render_grid):
     L = lib()
     st = stream_of(dev)
     ws_bytes = L.rm_bin_workspace_size(T, 0, G)
-    ws = torch.empty((max(ws_bytes, 1),), device=dev, dtype=torch.uint8)
+    key = (str(dev), torch.cuda.current_stream(dev).cuda_stream)
+    ws = _bin_ws.get(key)
+    if ws is None or ws.numel() < ws_bytes:
+        ws = torch.empty((max(ws_bytes, 1),), device=dev, dtype=torch.uint8)
     counts = torch.empty((2,), device=dev, dtype=torch.int64)
     with timing.span("bin_count"):
         check(L.rm_bin_count(ptr(tiles_touched), T, ptr(counts), ptr(ws), ws_bytes, st), "rm_bin_count")
     n_vis, P = (int(v) for v in counts.tolist())  # host sync, as the reference's index_buffer_offset[-1]
     full = L.rm_bin_workspace_size(T, P, G)
     if full > ws.numel():
-        ws2 = torch.empty((full,), device=dev, dtype=torch.uint8)
-        ws2[:ws.numel()].copy_(ws)
+        # room for a quarter more pairs, so that small changes of the view reuse it
+        ws2 = torch.empty((max(full, L.rm_bin_workspace_size(T, P + P // 4, G)),), device=dev, dtype=torch.uint8)
+        ws2[:ws_bytes].copy_(ws[:ws_bytes])
         ws = ws2
+    _bin_ws[key] = ws
     sorted_idx = torch.empty((P,), device=dev, dtype=torch.int32)
     ranges = torch.empty((G, 2), device=dev, dtype=torch.int32)
     with timing.span("bin_sort"):
