@@ -100,11 +100,14 @@ int rm_tile_ranges(const int64_t* sorted_keys_dev, int64_t n, int32_t* ranges_de
 
 /* ---- fused binning: the MI355X path behind vertex_and_tile_shader ------------------------
  * Replaces tile_shader_slang.py:36-68 (cumsum + generate_keys + CUB sort + ranges) by a
- * stable radix sort of the visible tets by depth key followed by a stable radix sort of the
- * (tile, tet) pairs by tile id. The result is bit-identical to the reference's stable sort of
+ * stable radix sort of the visible tets by depth key, a stable partition of the depth-ordered
+ * tets into blocks of 8x8 tiles, and per block a direct write of every tet into the lists of
+ * the tiles it covers. The result is bit-identical to the reference's stable sort of
  * (tile << 32 | sortable(depth)) keys (equal keys keep tet-index order).
  * Phase 1 (rm_bin_count) compacts visible tets, scans pair offsets and writes the totals
- * {n_visible, n_pairs} to counts_dev (int64[2]); the caller reads n_pairs to size phase 2. */
+ * {n_visible, n_pairs} to counts_dev (int64[2]); the caller reads n_pairs to size phase 2.
+ * rm_bin_workspace_size needs the grid's tile count n_tiles (grid_height * grid_width) for
+ * phase 2; its first rm_bin_workspace_size(n_tets, 0, n_tiles) bytes hold phase 1's state. */
 size_t rm_bin_workspace_size(int64_t n_tets, int64_t n_pairs, int64_t n_tiles);
 int rm_bin_count(const int32_t* tiles_touched_dev, int64_t n_tets, int64_t* counts_dev,
                  void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
