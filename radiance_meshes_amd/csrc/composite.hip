@@ -1192,7 +1192,7 @@ __device__ inline double lane_f(double v, int l) {
 }
 
 template <int F, bool P64, bool FLAGS>
-__global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
+__global__ __launch_bounds__(64, 3) void composite_bwd_tet_lanes_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
@@ -1206,8 +1206,14 @@ __global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
     using T = typename Prec<P64>::R;
     using PL = typename Prec<P64>::PL;
     __shared__ int s_pos[128];                      // gathered list positions, a ring
-    __shared__ float s_sum[64][kRowFloats + 1];     // the chunk's per-lane sums, read back row-wise
+    // each lane's plane records while the pixels are walked, then the chunk's per-lane sums read
+    // back row-wise (the records live in LDS rather than registers: two more waves per CU)
+    __shared__ union {
+        PL pl[4][64];
+        float sum[64][kRowFloats + 1];
+    } s_u;
     __shared__ int s_tet[64];
+    auto& s_sum = s_u.sum;
 
     const int lane = threadIdx.x;
     const int tile = remap_block(blockIdx.x, gridDim.x);
@@ -1265,7 +1271,7 @@ __global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
         npend -= cnt;
         const int kmin = __builtin_amdgcn_readlane(kpos, cnt - 1);   // the chunk's first list position
 
-        // this lane's tet: plane records, colour and cam - v0 in registers
+        // this lane's tet: plane records (to LDS), colour and cam - v0 in registers
         PL pl[4];
         float4 fa = make_float4(0.f, 0.f, 0.f, 0.f);
         float gr[NGRD];
@@ -1287,6 +1293,8 @@ __global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
 #pragma unroll
             for (int k = 0; k < NGRD; ++k) gr[k] = 0.f;
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s_u.pl[i][lane] = pl[i];
         const float sig = fa.x;
         float acc[NV];
 #pragma unroll
@@ -1298,7 +1306,8 @@ __global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
             if (ncp <= kmin) continue;                         // no entry of the chunk is revisited
             const RayT<T> ray = {lane_f(pray.dx, p), lane_f(pray.dy, p), lane_f(pray.dz, p)};
             const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
-            const auto h = intersect(pl, ray, mt);
+            const PL plp[4] = {s_u.pl[0][lane], s_u.pl[1][lane], s_u.pl[2][lane], s_u.pl[3][lane]};
+            const auto h = intersect(plp, ray, mt);
             const float dt = h.dt;
             const float x = sig * dt;
             const bool act = valid && kpos < ncp && h.hit && x > 0.f;
@@ -1424,6 +1433,7 @@ __global__ __launch_bounds__(64) void composite_bwd_tet_lanes_kernel(
         // the chunk's sums leave through LDS, so that each atomic instruction adds two whole rows
         // (consecutive lanes on consecutive floats of a 128-byte row)
         const uint64_t tm = __ballot(touched);
+        __syncthreads();   // every lane's plane records are read before the sums overwrite them
         if (tm != 0) {
 #pragma unroll
             for (int i = 0; i < NV; ++i) s_sum[lane][i] = acc[i];
