@@ -377,7 +377,8 @@ int rm_field_outputs_bwd(const float* density_dev, const float* grd_raw_dev, con
  * rm_field_scatter_fwd writes the activated head outputs raw [n, n_features] into the listed tets'
  * rows of features [T, n_features] (models/ingp_color.py:200-203: density exp(raw) with raw clamped
  * at log(1e20), base colour raw + 0.5, colour-gradient rows as they are); other rows are untouched.
- * rm_field_scatter_bwd writes grad_raw [n, n_features] from grad_features [T, n_features]. */
+ * rm_field_scatter_bwd writes grad_raw [n, n_features] from grad_features [T, n_features].
+ * raw (forward) and grad_raw (backward) must be 16-byte aligned. */
 int rm_field_points(const int32_t* indices_dev, const float* vertices_dev, const int32_t* tet_ids_dev, int64_t n,
                     float scale, float* points_dev, rm_stream_t stream);
 int rm_field_scatter_fwd(const float* raw_dev, const int32_t* tet_ids_dev, int64_t n, int32_t n_features,

@@ -41,32 +41,65 @@ __global__ __launch_bounds__(256) void field_points_kernel(const int4* __restric
     }
 }
 
+// Four consecutive elements of the [n, F] head output per thread (one 16-byte load or store of
+// the dense side); F is a template parameter so that splitting an element index is a multiply.
+template <int F>
 __global__ __launch_bounds__(256) void field_scatter_fwd_kernel(const float* __restrict__ raw,
                                                                 const int32_t* __restrict__ tet_ids, int64_t n,
-                                                                int F, float* __restrict__ feats) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * F) return;
-    const int64_t i = e / F;
-    const int f = (int)(e - i * F);
-    const float v = raw[e];
-    const float o = f == 0 ? expf(fminf(v, kLogMaxDensity)) : (f < 4 ? v + 0.5f : v);
-    feats[(int64_t)tet_ids[i] * F + f] = o;
+                                                                float* __restrict__ feats) {
+    const int64_t nf = n * F;
+    const int64_t e0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (e0 >= nf) return;
+    float v[4];
+    if (e0 + 3 < nf) {
+        const float4 q = *reinterpret_cast<const float4*>(raw + e0);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = e0 + k < nf ? raw[e0 + k] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        if (e >= nf) break;
+        const int64_t i = e / F;
+        const int f = (int)(e - i * F);
+        const float o = f == 0 ? expf(fminf(v[k], kLogMaxDensity)) : (f < 4 ? v[k] + 0.5f : v[k]);
+        feats[(int64_t)tet_ids[i] * F + f] = o;
+    }
 }
 
+template <int F>
 __global__ __launch_bounds__(256) void field_scatter_bwd_kernel(const float* __restrict__ raw,
                                                                 const int32_t* __restrict__ tet_ids, int64_t n,
-                                                                int F, const float* __restrict__ gfeats,
+                                                                const float* __restrict__ gfeats,
                                                                 float* __restrict__ graw) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * F) return;
-    const int64_t i = e / F;
-    const int f = (int)(e - i * F);
-    const float g = gfeats[(int64_t)tet_ids[i] * F + f];
-    if (f == 0) {
-        const float v = raw[e];
-        graw[e] = v <= kLogMaxDensity ? g * expf(v) : 0.f;
+    const int64_t nf = n * F;
+    const int64_t e0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (e0 >= nf) return;
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t e = e0 + k;
+        o[k] = 0.f;
+        if (e < nf) {
+            const int64_t i = e / F;
+            const int f = (int)(e - i * F);
+            const float g = gfeats[(int64_t)tet_ids[i] * F + f];
+            if (f == 0) {
+                const float vr = raw[e];
+                o[k] = vr <= kLogMaxDensity ? g * expf(vr) : 0.f;
+            } else {
+                o[k] = g;
+            }
+        }
+    }
+    if (e0 + 3 < nf) {
+        *reinterpret_cast<float4*>(graw + e0) = make_float4(o[0], o[1], o[2], o[3]);
     } else {
-        graw[e] = g;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (e0 + k < nf) graw[e0 + k] = o[k];
     }
 }
 
@@ -94,8 +127,13 @@ extern "C" int rm_field_scatter_fwd(const float* raw_dev, const int32_t* tet_ids
     RM_CHECK_ARG(n_features == 7 || n_features == 13, "rm_field_scatter_fwd: n_features must be 7 or 13");
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(raw_dev && tet_ids_dev && features_dev, "rm_field_scatter_fwd: null buffer");
-    hipLaunchKernelGGL(field_scatter_fwd_kernel, dim3((unsigned)ceil_div(n * n_features, 256)), dim3(256), 0,
-                       as_stream(stream), raw_dev, tet_ids_dev, n, (int)n_features, features_dev);
+    RM_CHECK_ARG(((uintptr_t)raw_dev & 15) == 0, "rm_field_scatter_fwd: raw must be 16-byte aligned");
+    if (n_features == 7)
+        hipLaunchKernelGGL(field_scatter_fwd_kernel<7>, dim3((unsigned)ceil_div(n * 7, 1024)), dim3(256), 0,
+                           as_stream(stream), raw_dev, tet_ids_dev, n, features_dev);
+    else
+        hipLaunchKernelGGL(field_scatter_fwd_kernel<13>, dim3((unsigned)ceil_div(n * 13, 1024)), dim3(256), 0,
+                           as_stream(stream), raw_dev, tet_ids_dev, n, features_dev);
     RM_CHECK_LAUNCH("field_scatter_fwd_kernel");
     return RM_OK;
 }
@@ -106,8 +144,13 @@ extern "C" int rm_field_scatter_bwd(const float* raw_dev, const int32_t* tet_ids
     RM_CHECK_ARG(n_features == 7 || n_features == 13, "rm_field_scatter_bwd: n_features must be 7 or 13");
     if (n == 0) return RM_OK;
     RM_CHECK_ARG(raw_dev && tet_ids_dev && grad_features_dev && grad_raw_dev, "rm_field_scatter_bwd: null buffer");
-    hipLaunchKernelGGL(field_scatter_bwd_kernel, dim3((unsigned)ceil_div(n * n_features, 256)), dim3(256), 0,
-                       as_stream(stream), raw_dev, tet_ids_dev, n, (int)n_features, grad_features_dev, grad_raw_dev);
+    RM_CHECK_ARG(((uintptr_t)grad_raw_dev & 15) == 0, "rm_field_scatter_bwd: grad_raw must be 16-byte aligned");
+    if (n_features == 7)
+        hipLaunchKernelGGL(field_scatter_bwd_kernel<7>, dim3((unsigned)ceil_div(n * 7, 1024)), dim3(256), 0,
+                           as_stream(stream), raw_dev, tet_ids_dev, n, grad_features_dev, grad_raw_dev);
+    else
+        hipLaunchKernelGGL(field_scatter_bwd_kernel<13>, dim3((unsigned)ceil_div(n * 13, 1024)), dim3(256), 0,
+                           as_stream(stream), raw_dev, tet_ids_dev, n, grad_features_dev, grad_raw_dev);
     RM_CHECK_LAUNCH("field_scatter_bwd_kernel");
     return RM_OK;
 }
