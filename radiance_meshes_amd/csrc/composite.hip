@@ -418,6 +418,11 @@ __device__ inline void stage_tet(Stage<WG, F, BWD, P64, VTX>& S, int slot, int t
 // chunk c is walked, the entries of chunk c + 3, the index rows of chunk c + 2 and the vertices and
 // features of chunk c + 1 are in flight, so the three dependent fetches (list entry, index row,
 // vertex and feature rows) of a chunk overlap the walks of the chunks before it (18 + F registers).
+// The index, vertex and feature loads are issued by every lane, a lane with no entry reading the
+// rows of tet 0 (so the pipeline is only started when the workgroup has entries, hence tets): with
+// loads under a divergent branch the number of loads issued between two points depends on the
+// path, and the compiler's wait counting then falls back to waiting for every load in flight,
+// which put a wait for the prefetched chunk at the top of the walk.
 template <int F>
 struct ChunkPipe {
     int t0, t1, t2;   // this lane's entries in the next chunk to stage and the two after it
@@ -430,8 +435,9 @@ struct ChunkPipe {
         t0 = entry(0);
         t1 = entry(1);
         t2 = entry(2);
-        q1 = t1 >= 0 ? indices[t1] : make_int4(0, 0, 0, 0);
-        if (t0 >= 0) load_tet_data<F>(d0, indices[t0], t0, verts, feats);
+        q1 = indices[t1 >= 0 ? t1 : 0];
+        const int u0 = t0 >= 0 ? t0 : 0;
+        load_tet_data<F>(d0, indices[u0], u0, verts, feats);
     }
 
     // stages this lane's entry of the current chunk (a lane with no entry stages nothing)
@@ -445,8 +451,8 @@ struct ChunkPipe {
     __device__ inline void advance(E entry, int c, const int4* __restrict__ indices,
                                    const float* __restrict__ verts, const float* __restrict__ feats) {
         const int tn = entry(c + 3);
-        const int4 qn = t2 >= 0 ? indices[t2] : make_int4(0, 0, 0, 0);
-        if (t1 >= 0) load_tet_data<F>(d0, q1, t1, verts, feats);
+        const int4 qn = indices[t2 >= 0 ? t2 : 0];
+        load_tet_data<F>(d0, q1, t1 >= 0 ? t1 : 0, verts, feats);
         t0 = t1;
         t1 = t2;
         t2 = tn;
@@ -557,7 +563,7 @@ __device__ __forceinline__ void fwd_tiles(Stage<Cfg<TS>::WG, F, false, P64>& S, 
         return kk < n ? (int)(sorted[rg.x + kk] - tet_base) : -1;
     };
     ChunkPipe<F> pipe;
-    pipe.start(entry, indices, verts, feats);
+    if (nchunks > 0) pipe.start(entry, indices, verts, feats);
     for (int c = 0; c < nchunks; ++c) {
         if (!__syncthreads_or(alive)) break;
         uint64_t blend_bits = 0;
