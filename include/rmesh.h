@@ -277,6 +277,7 @@ typedef struct rm_batch_info {
     int64_t plan_bytes;
     int32_t tile;
     int32_t n_features;
+    int64_t total_bin_blocks;   /* blocks of 8x8 tiles over all views (the binning's unit) */
 } rm_batch_info;
 
 int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_camera* cams,

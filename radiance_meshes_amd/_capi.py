@@ -35,7 +35,7 @@ class RmScene(C.Structure):
 class RmBatchInfo(C.Structure):
     _fields_ = [("n_views", C.c_int64), ("total_tets", C.c_int64), ("total_tiles", C.c_int64),
                 ("total_pixels", C.c_int64), ("total_workgroups", C.c_int64), ("plan_bytes", C.c_int64),
-                ("tile", C.c_int32), ("n_features", C.c_int32)]
+                ("tile", C.c_int32), ("n_features", C.c_int32), ("total_bin_blocks", C.c_int64)]
 
 
 _P = C.c_void_p

@@ -39,6 +39,7 @@ PlanHeader header_of(const rm_batch_info* info) {
     h.total_tiles = info->total_tiles;
     h.total_pixels = info->total_pixels;
     h.total_blocks = info->total_workgroups;
+    h.total_bin_blocks = info->total_bin_blocks;
     return h;
 }
 
@@ -80,6 +81,7 @@ extern "C" int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_
                      "rm_batch_plan: incomplete scene");
         RM_CHECK_ARG(((uintptr_t)sc.indices_dev & 15) == 0, "rm_batch_plan: indices must be 16-byte aligned");
         const int64_t blocks = batch_workgroups_per_view(h.tile, c.grid_height, c.grid_width);
+        const int64_t bin_blocks = bin_blocks_per_view(c.grid_height, c.grid_width);
         if (views) {
             ViewRec& r = views[v];
             r.cam = make_cam_args(c);
@@ -92,11 +94,13 @@ extern "C" int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_
             r.pixel_offset = h.total_pixels;
             r.block_offset = h.total_blocks;
             r.n_blocks = blocks;
+            r.bin_offset = h.total_bin_blocks;
         }
         h.total_tets += sc.n_tets;
         h.total_tiles += (int64_t)c.grid_height * c.grid_width;
         h.total_pixels += (int64_t)c.image_height * c.image_width;
         h.total_blocks += blocks;
+        h.total_bin_blocks += bin_blocks;
     }
     RM_CHECK_ARG(h.total_tets < (1ll << 31) && h.total_tiles < (1ll << 31), "rm_batch_plan: batch too large");
     if (plan_host) *reinterpret_cast<PlanHeader*>(plan_host) = h;
@@ -107,6 +111,7 @@ extern "C" int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_
     info->total_tiles = h.total_tiles;
     info->total_pixels = h.total_pixels;
     info->total_workgroups = h.total_blocks;
+    info->total_bin_blocks = h.total_bin_blocks;
     info->plan_bytes = (int64_t)need;
     return RM_OK;
 }
@@ -128,6 +133,8 @@ extern "C" int rm_batch_render(const void* plan_dev, const rm_batch_info* info, 
                  "rm_batch_render: null buffer");
     RM_CHECK_ARG(((uintptr_t)out_img_dev & 15) == 0 && ((uintptr_t)workspace_dev & 255) == 0,
                  "rm_batch_render: misaligned output or workspace");
+    RM_CHECK_ARG(info->total_bin_blocks > 0 && info->total_bin_blocks <= info->total_tiles,
+                 "rm_batch_render: total_bin_blocks must be set (rm_batch_plan's value or a capacity)");
     const PlanHeader h = header_of(info);
     const BatchLayout L = batch_layout(h, pair_capacity);
     if (workspace_bytes < L.total) {

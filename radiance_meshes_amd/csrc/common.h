@@ -81,11 +81,12 @@ struct ViewRec {
     int64_t pixel_offset;      // first pixel of the view in the batch image buffers
     int64_t block_offset;      // first compositing workgroup of the view
     int64_t n_blocks;
+    int64_t bin_offset;        // first global 8x8-tile binning block of the view
 };
 
 struct PlanHeader {
     int32_t n_views, tile, n_features, _pad;
-    int64_t total_tets, total_tiles, total_pixels, total_blocks;
+    int64_t total_tets, total_tiles, total_pixels, total_blocks, total_bin_blocks;
 };
 
 __host__ __device__ inline const ViewRec* plan_views(const void* plan) {
@@ -101,6 +102,8 @@ hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap
                      const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
                      int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st);
 int batch_workgroups_per_view(int tile, int gh, int gw);
+// Blocks of 8x8 tiles over a view's tile grid (the binning's unit).
+inline int64_t bin_blocks_per_view(int gh, int gw) { return (int64_t)((gw + 7) / 8) * ((gh + 7) / 8); }
 hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet,
                            int64_t pair_cap, const int32_t* ranges, float* img, float* dist, int32_t* nc,
                            hipStream_t st);

@@ -230,28 +230,22 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     cc[3 * t + 2] = ctr.z;
 }
 
-// Batched projection: lane g projects global tet g of the batch with its view's camera.
-__global__ __launch_bounds__(256) void batch_project_kernel(const void* __restrict__ plan, int64_t total,
-                                                            int32_t* __restrict__ tiles, int4* __restrict__ rect,
-                                                            float* __restrict__ depth, int32_t* __restrict__ tet_view) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const PlanHeader* h = static_cast<const PlanHeader*>(plan);
-    const ViewRec* views = plan_views(plan);
-    if (g >= h->total_tets) {  // launch sized for a capacity: slots past the plan's tets stay empty
-        tiles[g] = 0;
-        rect[g] = make_int4(0, 0, 0, 0);
-        depth[g] = 0.f;
-        tet_view[g] = 0;
-        return;
-    }
-    int lo = 0, hi = h->n_views - 1;  // last view with tet_offset <= g
+// Batched projection: lane g projects global tet g of the batch with its view's camera. A
+// workgroup whose tets all belong to one view (all but those that straddle two views) finds that
+// view once, so its camera is read with scalar loads instead of once per lane.
+__device__ inline int view_of_tet(const ViewRec* views, int n_views, int64_t g) {
+    int lo = 0, hi = n_views - 1;  // last view with tet_offset <= g
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (views[mid].tet_offset <= g) lo = mid;
         else hi = mid - 1;
     }
-    const ViewRec& v = views[lo];
+    return lo;
+}
+
+__device__ __forceinline__ void project_in_view(const ViewRec& v, int view, int64_t g, int32_t* __restrict__ tiles,
+                                       int4* __restrict__ rect, float* __restrict__ depth,
+                                       int32_t* __restrict__ tet_view) {
     const int64_t t = g - v.tet_offset;
     int32_t n;
     int4 r;
@@ -261,7 +255,33 @@ __global__ __launch_bounds__(256) void batch_project_kernel(const void* __restri
     tiles[g] = n;
     rect[g] = r;
     depth[g] = d;
-    tet_view[g] = lo;
+    tet_view[g] = view;
+}
+
+__global__ __launch_bounds__(256) void batch_project_kernel(const void* __restrict__ plan, int64_t total,
+                                                            int32_t* __restrict__ tiles, int4* __restrict__ rect,
+                                                            float* __restrict__ depth, int32_t* __restrict__ tet_view) {
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t g = g0 + threadIdx.x;
+    if (g >= total) return;
+    const PlanHeader* h = static_cast<const PlanHeader*>(plan);
+    const ViewRec* views = plan_views(plan);
+    const int64_t nt = h->total_tets;
+    if (g >= nt) {  // launch sized for a capacity: slots past the plan's tets stay empty
+        tiles[g] = 0;
+        rect[g] = make_int4(0, 0, 0, 0);
+        depth[g] = 0.f;
+        tet_view[g] = 0;
+        return;
+    }
+    const int64_t g1 = (g0 + (int64_t)blockDim.x < nt ? g0 + (int64_t)blockDim.x : nt) - 1;
+    const int v0 = view_of_tet(views, h->n_views, g0), v1 = view_of_tet(views, h->n_views, g1);
+    if (v0 == v1) {
+        project_in_view(views[v0], v0, g, tiles, rect, depth, tet_view);
+    } else {
+        const int v = view_of_tet(views, h->n_views, g);
+        project_in_view(views[v], v, g, tiles, rect, depth, tet_view);
+    }
 }
 
 // Adjoint of (clamped float(power), float3(center)) w.r.t. the four vertices, in FP64.

@@ -433,17 +433,12 @@ __device__ inline int64_t last_le(const uint32_t* __restrict__ a, int64_t n, uin
     return lo;
 }
 
-struct TileMapSingle {  // tile id in a single view's grid
+// Id maps of the pair expansion: for a tet, the (first id, ids per row) of its view's grid, so
+// that cell (x, y) of the tet's rect gets id first + y * row + x; looked up once per tet.
+struct TileMapSingle {  // one view's grid of gw cells per row
+    static constexpr bool kPerTet = false;
     int32_t gw;
-    __device__ uint32_t operator()(uint32_t, uint32_t x, uint32_t y) const { return y * (uint32_t)gw + x; }
-};
-struct TileMapBatch {   // global tile id of the tet's view in a batch
-    const int32_t* tet_view;
-    const ViewRec* views;
-    __device__ uint32_t operator()(uint32_t tet, uint32_t x, uint32_t y) const {
-        const ViewRec& v = views[tet_view[tet]];
-        return (uint32_t)v.tile_offset + y * (uint32_t)v.cam.gw + x;
-    }
+    __device__ uint2 operator()(uint32_t) const { return make_uint2(0u, (uint32_t)gw); }
 };
 
 struct RectTiles {  // the tet's rect of tiles
@@ -458,6 +453,7 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     __shared__ uint32_t s_off[kPairsPerBlock];
     __shared__ int4 s_rect[kPairsPerBlock];
     __shared__ uint32_t s_tet[kPairsPerBlock];
+    __shared__ uint2 s_map[TileMap::kPerTet ? kPairsPerBlock : 1];
     __shared__ int64_t s_lo;
     __shared__ int s_n;
     const int64_t nvis = eff_n(nvis_cap, nvis_dev);
@@ -479,6 +475,7 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
         s_off[j] = poff[lo + j];
         s_tet[j] = t;
         s_rect[j] = rect_of(rect[t]);
+        if constexpr (TileMap::kPerTet) s_map[j] = tile_of(t);
     }
     __syncthreads();
     for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
@@ -493,7 +490,8 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
         const uint32_t m = pp - s_off[a];
         const uint32_t wdt = (uint32_t)(r.z - r.x);
         const uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
-        ptile[p] = tile_of(s_tet[a], x, y);
+        const uint2 mp = TileMap::kPerTet ? s_map[a] : tile_of(0u);
+        ptile[p] = mp.x + y * mp.y + x;
         ptet[p] = s_tet[a];
     }
 }
@@ -536,6 +534,55 @@ __device__ inline int4 local_rect(int4 r, int bx, int by) {
     return make_int4(max(r.x - ox, 0), max(r.y - oy, 0), min(r.z - ox, kBinBlock), min(r.w - oy, kBinBlock));
 }
 
+// Tile grid of the view that a block or a tile belongs to: tile grid gw x gh, blocks per row bgw,
+// and the view's first global tile and block (0 for a single view; a batch numbers the tiles and
+// blocks of its views one view after the other).
+struct ViewGrid {
+    int32_t gw, gh, bgw;
+    int64_t tile_base, block_base;
+};
+struct GridSingle {
+    int32_t gw, gh, bgw;
+    __device__ ViewGrid of_block(int64_t) const { return {gw, gh, bgw, 0, 0}; }
+    __device__ ViewGrid of_tile(int64_t) const { return {gw, gh, bgw, 0, 0}; }
+    __device__ int64_t n_tiles() const { return (int64_t)gw * gh; }
+};
+struct GridBatch {   // views of a batch plan, looked up by global block or tile id
+    const void* plan;
+    __device__ ViewGrid at(int v) const {
+        const ViewRec& r = plan_views(plan)[v];
+        return {r.cam.gw, r.cam.gh, (r.cam.gw + kBinBlock - 1) / kBinBlock, r.tile_offset, r.bin_offset};
+    }
+    template <class Off>
+    __device__ int last_view(Off off, int64_t id) const {   // last view whose first id is <= id
+        const ViewRec* v = plan_views(plan);
+        int lo = 0, hi = static_cast<const PlanHeader*>(plan)->n_views - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off(v[mid]) <= id) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
+    __device__ ViewGrid of_block(int64_t b) const {
+        return at(last_view([](const ViewRec& r) { return r.bin_offset; }, b));
+    }
+    __device__ ViewGrid of_tile(int64_t t) const {
+        return at(last_view([](const ViewRec& r) { return r.tile_offset; }, t));
+    }
+    __device__ int64_t n_tiles() const { return static_cast<const PlanHeader*>(plan)->total_tiles; }
+};
+static_assert(kBinBlock == 8, "bin_blocks_per_view assumes 8x8-tile blocks");
+struct BlockMapBatch {  // blocks of the tet's view in a batch: first global block, blocks per row
+    static constexpr bool kPerTet = true;
+    const int32_t* tet_view;
+    const ViewRec* views;
+    __device__ uint2 operator()(uint32_t tet) const {
+        const ViewRec& v = views[tet_view[tet]];
+        return make_uint2((uint32_t)v.bin_offset, (uint32_t)((v.cam.gw + kBinBlock - 1) / kBinBlock));
+    }
+};
+
 // A block's list is cut into segments of kSegEntries; seg_first[b] (an exclusive scan over the
 // blocks) numbers them, so that every segment is one workgroup of the count and emit kernels.
 constexpr int kSegEntries = kThreads;
@@ -577,21 +624,24 @@ struct LoadSegColumn {
 };
 
 // Pairs of each of the block's 64 tiles within segment g, column-major: cnt[t * stride + g].
+template <class Grid>
 __global__ __launch_bounds__(kThreads) void seg_count_kernel(const uint32_t* __restrict__ blist,
                                                              const int4* __restrict__ seg_desc,
                                                              const int64_t* __restrict__ nseg_dev,
-                                                             const int4* __restrict__ rect, int32_t bgw,
+                                                             const int4* __restrict__ rect, Grid grid,
                                                              int64_t stride, uint32_t* __restrict__ cnt) {
     __shared__ uint32_t s_cnt[kBinLocal];
     const int64_t nseg = *nseg_dev;
     const int tid = threadIdx.x;
     for (int64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
         const int4 d = seg_desc[g];
+        const ViewGrid vg = grid.of_block(d.z);
+        const int lb = (int)(d.z - vg.block_base);
         if (tid < kBinLocal) s_cnt[tid] = 0;
         __syncthreads();
         const int64_t i = (int64_t)d.x + tid;
         if (i < d.y) {
-            const int4 l = local_rect(rect[blist[i]], d.z % bgw, d.z / bgw);
+            const int4 l = local_rect(rect[blist[i]], lb % vg.bgw, lb / vg.bgw);
             for (int y = l.y; y < l.w; ++y)
                 for (int x = l.x; x < l.z; ++x) atomicAdd(&s_cnt[y * kBinBlock + x], 1u);
         }
@@ -600,16 +650,24 @@ __global__ __launch_bounds__(kThreads) void seg_count_kernel(const uint32_t* __r
     }
 }
 
-// Pairs per tile (from the column prefix sums at the block's first and one-past-last segment).
+// Pairs per tile (from the column prefix sums at the block's first and one-past-last segment);
+// 0 for the tiles of a launch sized for a capacity past the views' tiles.
+template <class Grid>
 __global__ __launch_bounds__(kThreads) void tile_total_kernel(const uint32_t* __restrict__ cpre,
                                                               const int2* __restrict__ branges,
                                                               const uint32_t* __restrict__ seg_first,
-                                                              int32_t gw, int32_t gh, int32_t bgw, int64_t stride,
+                                                              Grid grid, int64_t G, int64_t stride,
                                                               uint32_t* __restrict__ tcount) {
     const int64_t tile = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (tile >= (int64_t)gw * gh) return;
-    const int tx = (int)(tile % gw), ty = (int)(tile / gw);
-    const int b = (ty / kBinBlock) * bgw + tx / kBinBlock;
+    if (tile >= G) return;
+    if (tile >= grid.n_tiles()) {
+        tcount[tile] = 0;
+        return;
+    }
+    const ViewGrid vg = grid.of_tile(tile);
+    const int64_t lt = tile - vg.tile_base;
+    const int tx = (int)(lt % vg.gw), ty = (int)(lt / vg.gw);
+    const int64_t b = vg.block_base + (ty / kBinBlock) * vg.bgw + tx / kBinBlock;
     const int t = (ty % kBinBlock) * kBinBlock + tx % kBinBlock;
     const int2 r = branges[b];
     const uint32_t f = seg_first[b], e = f + (uint32_t)((r.y - r.x + kSegEntries - 1) / kSegEntries);
@@ -650,11 +708,11 @@ __device__ inline uint32_t spread4(uint32_t b) {
 // Writes each tet of segment g into the lists of the block's tiles it covers. A tile's first free
 // position: its start, plus the pairs of the block's earlier segments (column prefix sums), plus
 // the earlier entries of this segment that cover it (packed wave scans).
+template <class Grid>
 __global__ __launch_bounds__(kThreads) void seg_emit_kernel(const uint32_t* __restrict__ blist,
                                                             const int4* __restrict__ seg_desc,
                                                             const int64_t* __restrict__ nseg_dev,
-                                                            const int4* __restrict__ rect, int32_t gw, int32_t gh,
-                                                            int32_t bgw, int64_t stride,
+                                                            const int4* __restrict__ rect, Grid grid, int64_t stride,
                                                             const uint32_t* __restrict__ cpre,
                                                             const uint32_t* __restrict__ tstart, int64_t n_pairs,
                                                             int32_t* __restrict__ out) {
@@ -665,7 +723,9 @@ __global__ __launch_bounds__(kThreads) void seg_emit_kernel(const uint32_t* __re
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (int64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
         const int4 d = seg_desc[g];
-        const int bx = d.z % bgw, by = d.z / bgw;
+        const ViewGrid vg = grid.of_block(d.z);
+        const int lb = (int)(d.z - vg.block_base);
+        const int bx = lb % vg.bgw, by = lb / vg.bgw;
         const int64_t i = (int64_t)d.x + tid;
         int4 l = make_int4(0, 0, 0, 0);
         uint32_t tet = 0;
@@ -676,8 +736,8 @@ __global__ __launch_bounds__(kThreads) void seg_emit_kernel(const uint32_t* __re
         if (tid < kBinLocal) {
             const int tx = bx * kBinBlock + (tid % kBinBlock), ty = by * kBinBlock + (tid / kBinBlock);
             uint32_t st = 0;
-            if (tx < gw && ty < gh)
-                st = tstart[(int64_t)ty * gw + tx] + cpre[tid * stride + g] - cpre[tid * stride + d.w];
+            if (tx < vg.gw && ty < vg.gh)
+                st = tstart[vg.tile_base + (int64_t)ty * vg.gw + tx] + cpre[tid * stride + g] - cpre[tid * stride + d.w];
             s_run[tid] = st;
         }
         const uint32_t xb = ((1u << l.z) - 1u) & ~((1u << l.x) - 1u);   // covered columns
@@ -771,44 +831,95 @@ inline int bit_length(int64_t v) {
     return b;
 }
 
+// Tile lists from the depth-ordered visible tets `order` (n_vis of them: nvis_cap, or the count at
+// nvis_dev), through blocks of 8x8 tiles: (block, tet) pair offsets in depth order (their count
+// stays on the device), the pairs expanded and stably sorted by block, the block lists cut into
+// segments, per segment and tile the pairs, prefix sums of those per tile over the segments,
+// pairs and first position of each tile, then every tet written into its tiles' lists. bmap gives
+// a tet's global block id from its block coordinates, grid the view of a block or tile; pair_cap
+// bounds the tile pairs (the exact count for a single view), G the tiles and NB the blocks.
+template <class BlockMap, class Grid>
+hipError_t block_bin_lists(const uint32_t* order, int64_t nvis_cap, const int64_t* nvis_dev, const int4* rect,
+                           const BinLayout& L, char* w, int64_t pair_cap, int64_t G, int64_t NB, BlockMap bmap,
+                           Grid grid, int32_t* sorted_tet, int32_t* ranges, hipStream_t st) {
+    auto W = [&](size_t off) { return reinterpret_cast<uint32_t*>(w + off); };
+    uint32_t* partial = W(L.partial);
+    uint32_t* rcounts = W(L.rcounts);
+    int64_t* nbp = reinterpret_cast<int64_t*>(w + L.counts2);   // [0] block pairs, [1] segments
+    device_scan(LoadBlockCount{(const uint32_t*)W(L.bcnt), order}, nvis_cap, W(L.poff), false, partial, nbp, st,
+                nvis_dev);
+    hipLaunchKernelGGL((emit_pairs_kernel<BlockMap, RectBlocks>), dim3((unsigned)ceil_div(pair_cap, kPairsPerBlock)),
+                       dim3(kThreads), 0, st, order, (const uint32_t*)W(L.poff), rect, nvis_cap, nvis_dev, pair_cap,
+                       (const int64_t*)nbp, bmap, W(L.ptile), W(L.ptet), RectBlocks{});
+    const int bbits = bit_length(NB - 1) > 0 ? bit_length(NB - 1) : 1;
+    hipError_t e = radix_sort<uint32_t>(W(L.ptile), W(L.ptet), W(L.ptile2), W(L.ptet2), W(L.ptile3), W(L.ptet3),
+                                        pair_cap, 0, bbits, rcounts, partial, st, nbp);
+    if (e != hipSuccess) return e;
+    int2* branges = reinterpret_cast<int2*>(w + L.branges);
+    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st, branges, NB);
+    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)std::min<int64_t>(ceil_div(pair_cap, kThreads), 8192)),
+                       dim3(kThreads), 0, st, TileOfU32{W(L.ptile2)}, pair_cap, (const int64_t*)nbp,
+                       (int32_t*)branges);
+    const uint32_t* blist = W(L.ptet2);
+    uint32_t* seg_first = W(L.seg_first);
+    int64_t* nsegp = nbp + 1;
+    device_scan(LoadSegCount{(const int2*)branges}, NB, seg_first, false, partial, nsegp, st);
+    const int64_t segcap = seg_capacity(pair_cap, G), stride = segcap + 1;
+    uint32_t* cnt = W(L.seg_cnt);
+    uint32_t* cpre = W(L.seg_pre);
+    int4* seg_desc = reinterpret_cast<int4*>(w + L.seg_block);
+    const unsigned sgrid = (unsigned)std::min(seg_grid(pair_cap, NB), segcap);
+    hipLaunchKernelGGL(seg_desc_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st,
+                       (const int2*)branges, (const uint32_t*)seg_first, NB, seg_desc);
+    hipLaunchKernelGGL((seg_count_kernel<Grid>), dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
+                       (const int64_t*)nsegp, rect, grid, stride, cnt);
+    device_scan(LoadSegColumn{cnt, stride, (const int64_t*)nsegp}, kBinLocal * stride, cpre, false, partial,
+                (int64_t*)nullptr, st);
+    uint32_t* tcount = W(L.tcount);
+    uint32_t* tstart = W(L.tstart);
+    hipLaunchKernelGGL((tile_total_kernel<Grid>), dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (const uint32_t*)cpre, (const int2*)branges, (const uint32_t*)seg_first, grid, G, stride,
+                       tcount);
+    device_scan(LoadU32{tcount}, G, tstart, false, partial, (int64_t*)nullptr, st);
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       (const uint32_t*)tcount, (const uint32_t*)tstart, G, ranges);
+    hipLaunchKernelGGL((seg_emit_kernel<Grid>), dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
+                       (const int64_t*)nsegp, rect, grid, stride, (const uint32_t*)cpre, (const uint32_t*)tstart,
+                       pair_cap, sorted_tet);
+    return hipGetLastError();
+}
+
 }  // namespace
 
-size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) { return bin_layout(h.total_tets, pair_cap).total; }
+size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) {
+    return bin_layout(h.total_tets, pair_cap, h.total_tiles).total;
+}
 
 // Binning of a whole batch with every size read from device memory: grids are sized by the
-// batch's tet count and the pair capacity; counts_dev receives {n_visible, n_pairs} (n_pairs may
-// exceed pair_cap: pairs beyond it are dropped and the caller re-renders with a larger capacity).
+// batch's tet, tile and block capacities and the pair capacity; counts_dev receives {n_visible,
+// n_pairs} (n_pairs may exceed pair_cap: the lists are then incomplete, ranges past the capacity
+// are dropped by the compositing, and the caller re-renders with a larger capacity). The views'
+// tiles and blocks are numbered one view after the other, so the lists equal those of a stable
+// sort of the (global tile, tet) pairs by tile.
 hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
                      const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
                      int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st) {
-    const int64_t T = h.total_tets, G = h.total_tiles;
-    BinLayout L = bin_layout(T, pair_cap);
+    const int64_t T = h.total_tets, G = h.total_tiles, NB = h.total_bin_blocks;
+    BinLayout L = bin_layout(T, pair_cap, G);
     char* w = static_cast<char*>(ws);
     auto W = [&](size_t off) { return reinterpret_cast<uint32_t*>(w + off); };
     const int64_t nb = scan_blocks(T);
-    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
-                       (int2*)ranges, G);
     hipLaunchKernelGGL(bin_count_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles, T, W(L.blk_vis),
                        W(L.blk_pairs));
     hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, W(L.blk_vis), nb, counts_dev);
     hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kThreads), 0, st, W(L.blk_pairs), nb, counts_dev + 1);
     hipLaunchKernelGGL(compact_kernel, dim3((unsigned)nb), dim3(kThreads), 0, st, tiles, depth, 1, T,
-                       (const uint32_t*)W(L.blk_vis), W(L.vkey), W(L.vidx));
-    uint32_t* partial = W(L.partial);
-    uint32_t* rcounts = W(L.rcounts);
-    radix_sort<uint32_t>(W(L.vkey), W(L.vidx), W(L.vkey2), W(L.vidx2), W(L.vkey3), W(L.vidx3), T, 0, 32, rcounts,
-                         partial, st, counts_dev);
-    const uint32_t* order = W(L.vidx2);
-    device_scan(LoadGather{tiles, order}, T, W(L.poff), false, partial, (int64_t*)nullptr, st, counts_dev);
-    hipLaunchKernelGGL(emit_pairs_kernel<TileMapBatch>, dim3((unsigned)ceil_div(pair_cap, kPairsPerBlock)),
-                       dim3(kThreads), 0, st, order, (const uint32_t*)W(L.poff), rect, T,
-                       (const int64_t*)counts_dev, pair_cap, (const int64_t*)(counts_dev + 1),
-                       TileMapBatch{tet_view, plan_views(plan_dev)}, W(L.ptile), W(L.ptet));
-    radix_sort<uint32_t>(W(L.ptile), W(L.ptet), W(L.ptile2), (uint32_t*)sorted_tet, W(L.ptile3), W(L.ptet3),
-                         pair_cap, 0, bit_length(G), rcounts, partial, st, counts_dev + 1);
-    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)ceil_div(pair_cap, kThreads)), dim3(kThreads), 0,
-                       st, TileOfU32{W(L.ptile2)}, pair_cap, (const int64_t*)(counts_dev + 1), ranges);
-    return hipGetLastError();
+                       (const uint32_t*)W(L.blk_vis), W(L.vkey), W(L.vidx), rect, W(L.bcnt));
+    hipError_t e = radix_sort<uint32_t>(W(L.vkey), W(L.vidx), W(L.vkey2), W(L.vidx2), W(L.vkey3), W(L.vidx3), T, 0,
+                                        32, W(L.rcounts), W(L.partial), st, counts_dev);
+    if (e != hipSuccess) return e;
+    return block_bin_lists(W(L.vidx2), T, counts_dev, rect, L, w, pair_cap, G, NB,
+                           BlockMapBatch{tet_view, plan_views(plan_dev)}, GridBatch{plan_dev}, sorted_tet, ranges, st);
 }
 
 }  // namespace rmesh
@@ -958,57 +1069,11 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     // 2. stable sort of the visible tets by depth key (32 bits)
     radix_sort<uint32_t>(WS(uint32_t, L.vkey), WS(uint32_t, L.vidx), WS(uint32_t, L.vkey2), WS(uint32_t, L.vidx2),
                          WS(uint32_t, L.vkey3), WS(uint32_t, L.vidx3), n_visible, 0, 32, rcounts, partial, st);
-    const uint32_t* order = WS(uint32_t, L.vidx2);
-    // 3. (block, tet) pair offsets in depth order; the pair count stays on the device (at most
-    //    n_pairs: a tet overlaps no more blocks than tiles)
+    // 3.-5. (block, tet) pairs in depth order, sorted by block; tile lists through block segments
     const int32_t bgw = (grid_width + kBinBlock - 1) / kBinBlock, bgh = (grid_height + kBinBlock - 1) / kBinBlock;
-    const int64_t NB = (int64_t)bgw * bgh;
-    int64_t* nbp = WS(int64_t, L.counts2);
-    device_scan(LoadBlockCount{(const uint32_t*)WS(uint32_t, L.bcnt), order}, n_visible, WS(uint32_t, L.poff), false,
-                partial, nbp, st);
-    // 4. expand (block, tet) pairs in depth order and sort them stably by block
-    hipLaunchKernelGGL((emit_pairs_kernel<TileMapSingle, RectBlocks>), dim3((unsigned)ceil_div(n_pairs, kPairsPerBlock)),
-                       dim3(kThreads), 0, st, order, (const uint32_t*)WS(uint32_t, L.poff), (const int4*)rect_dev,
-                       n_visible, (const int64_t*)nullptr, n_pairs, (const int64_t*)nbp, TileMapSingle{bgw},
-                       WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), RectBlocks{});
-    radix_sort<uint32_t>(WS(uint32_t, L.ptile), WS(uint32_t, L.ptet), WS(uint32_t, L.ptile2), WS(uint32_t, L.ptet2),
-                         WS(uint32_t, L.ptile3), WS(uint32_t, L.ptet3), n_pairs, 0, bit_length(NB - 1) > 0 ? bit_length(NB - 1) : 1, rcounts,
-                         partial, st, nbp);
-    int2* branges = WS(int2, L.branges);
-    hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st, branges, NB);
-    hipLaunchKernelGGL(ranges_kernel<TileOfU32>, dim3((unsigned)std::min<int64_t>(ceil_div(n_pairs, kThreads), 8192)),
-                       dim3(kThreads), 0, st, TileOfU32{WS(uint32_t, L.ptile2)}, n_pairs, (const int64_t*)nbp,
-                       (int32_t*)branges);
-    // 5. the block lists cut into segments; per segment and tile the pairs, prefix sums of those
-    //    per tile over the segments, pairs and first position of each tile, then every tet
-    //    written into its tiles' lists
-    const uint32_t* blist = WS(uint32_t, L.ptet2);
-    uint32_t* seg_first = WS(uint32_t, L.seg_first);
-    int64_t* nsegp = nbp + 1;
-    device_scan(LoadSegCount{(const int2*)branges}, NB, seg_first, false, partial, nsegp, st);
-    const int64_t segcap = seg_capacity(n_pairs, G), stride = segcap + 1;
-    uint32_t* cnt = WS(uint32_t, L.seg_cnt);
-    uint32_t* cpre = WS(uint32_t, L.seg_pre);
-    int4* seg_desc = WS(int4, L.seg_block);
-    const unsigned sgrid = (unsigned)std::min(seg_grid(n_pairs, NB), segcap);
-    hipLaunchKernelGGL(seg_desc_kernel, dim3((unsigned)ceil_div(NB, kThreads)), dim3(kThreads), 0, st,
-                       (const int2*)branges, (const uint32_t*)seg_first, NB, seg_desc);
-    hipLaunchKernelGGL(seg_count_kernel, dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
-                       (const int64_t*)nsegp, (const int4*)rect_dev, bgw, stride, cnt);
-    device_scan(LoadSegColumn{cnt, stride, (const int64_t*)nsegp}, kBinLocal * stride, cpre, false, partial,
-                (int64_t*)nullptr, st);
-    uint32_t* tcount = WS(uint32_t, L.tcount);
-    uint32_t* tstart = WS(uint32_t, L.tstart);
-    hipLaunchKernelGGL(tile_total_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
-                       (const uint32_t*)cpre, (const int2*)branges, (const uint32_t*)seg_first, grid_width,
-                       grid_height, bgw, stride, tcount);
-    device_scan(LoadU32{tcount}, G, tstart, false, partial, (int64_t*)nullptr, st);
-    hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
-                       (const uint32_t*)tcount, (const uint32_t*)tstart, G, ranges_dev);
-    hipLaunchKernelGGL(seg_emit_kernel, dim3(sgrid), dim3(kThreads), 0, st, blist, (const int4*)seg_desc,
-                       (const int64_t*)nsegp, (const int4*)rect_dev,
-                       grid_width, grid_height, bgw, stride, (const uint32_t*)cpre, (const uint32_t*)tstart, n_pairs,
-                       sorted_tet_dev);
+    block_bin_lists(WS(uint32_t, L.vidx2), n_visible, (const int64_t*)nullptr, (const int4*)rect_dev, L,
+                    static_cast<char*>(workspace_dev), n_pairs, G, (int64_t)bgw * bgh, TileMapSingle{bgw},
+                    GridSingle{grid_width, grid_height, bgw}, sorted_tet_dev, ranges_dev, st);
     RM_CHECK_LAUNCH("rm_bin_sort");
     return RM_OK;
 }

@@ -87,6 +87,7 @@ class Bucket:
         self.info = server.plan_info([(big, cam)] * slots)
         self.view_tiles = int(self.info.total_tiles) // slots
         self.view_blocks = int(self.info.total_workgroups) // slots
+        self.view_bin_blocks = int(self.info.total_bin_blocks) // slots
         self.scene_pairs = list(scene_pairs)     # pairs of each scene seen from the probe pose
         self.plan_bytes = int(self.info.plan_bytes)
         self.plan_host = torch.empty(self.plan_bytes, dtype=torch.uint8, pin_memory=True)
@@ -128,6 +129,7 @@ class Bucket:
         P = min(self.pair_cap, _pow2_at_least(est))
         info = RmBatchInfo(n_views=nv, total_tets=T, total_tiles=nv * self.view_tiles,
                            total_pixels=nv * self.H * self.W, total_workgroups=nv * self.view_blocks,
+                           total_bin_blocks=nv * self.view_bin_blocks,
                            plan_bytes=self.plan_bytes, tile=self.info.tile, n_features=self.info.n_features)
         assert lib().rm_batch_workspace_size(info, P) <= self.ws.numel()
         return (nv, T, P), info, P

@@ -239,3 +239,49 @@ def test_model_scene_views_match_render(tmp_path, use_graphs):
         assert d <= 1e-6
     # the two poses see different colours (view-dependent SH)
     assert (out[0][0] - out[2][0]).abs().max().item() > 1e-3
+
+
+@pytest.mark.parametrize("capacity", [False, True])
+def test_mixed_resolution_plan_matches_single_views(capacity):
+    """One rm_batch_render over a plan whose views differ in resolution (ragged tile and block
+    grids, one view wider than one 8x8-tile block row, one narrower): every view's image,
+    distortion and contributor counts equal the single-view path's. With `capacity` the launch
+    geometry is larger than the plan (tets, tiles, blocks and workgroups doubled, as a captured
+    graph of a larger class would launch)."""
+    import ctypes as C
+    import torch
+    from radiance_meshes_amd._capi import RmBatchInfo, check, lib, ptr, stream_of
+    print(f"mixed-resolution plan capacity={capacity}", flush=True)
+    srv = _server(False)
+    res = [(66, 90), (20, 28), (130, 200), (48, 64), (36, 36)]
+    rng = np.random.default_rng(11)
+    views = [(k % 3, scenes.orbit_camera(RADIUS[k % 3], rng.uniform(0, 360), rng.uniform(-30, 40), W, H, 50.0))
+             for k, (H, W) in enumerate(res)]
+    plan_host = torch.empty(srv.plan_info(views).plan_bytes, dtype=torch.uint8, pin_memory=True)
+    info = srv.plan_info(views, plan_host)
+    plan_dev = plan_host.cuda()
+    if capacity:
+        launch = RmBatchInfo(n_views=2 * info.n_views, total_tets=2 * info.total_tets,
+                             total_tiles=2 * info.total_tiles, total_pixels=2 * info.total_pixels,
+                             total_workgroups=2 * info.total_workgroups, plan_bytes=info.plan_bytes,
+                             tile=info.tile, n_features=info.n_features,
+                             total_bin_blocks=2 * info.total_bin_blocks)
+    else:
+        launch = info
+    cap = 1 << 22
+    ws = torch.empty(lib().rm_batch_workspace_size(launch, cap), dtype=torch.uint8, device="cuda")
+    npx = int(launch.total_pixels)
+    img = torch.empty((npx, 4), device="cuda")
+    dist = torch.empty((npx, 5), device="cuda")
+    nc = torch.empty((npx,), device="cuda", dtype=torch.int32)
+    counts = torch.zeros(2, device="cuda", dtype=torch.int64)
+    check(lib().rm_batch_render(ptr(plan_dev), C.byref(launch), cap, ptr(img), ptr(dist), ptr(nc), ptr(counts),
+                                ptr(ws), ws.numel(), stream_of()), "rm_batch_render")
+    torch.cuda.synchronize()
+    assert 0 < int(counts[1]) <= cap
+    off = 0
+    for (s, c), (H, W) in zip(views, res):
+        got = (img[off:off + H * W].view(H, W, 4), dist[off:off + H * W].view(H, W, 5), nc[off:off + H * W].view(H, W))
+        _assert_same(got, _single_full(srv, s, c), f"view {H}x{W} scene {s}")
+        off += H * W
+    assert off == int(info.total_pixels)
