@@ -270,7 +270,7 @@ typedef struct rm_scene {
 
 typedef struct rm_batch_info {
     int64_t n_views;
-    int64_t total_tets;
+    int64_t total_tets;         /* tet slots: each view's tets start at a multiple of 256 */
     int64_t total_tiles;
     int64_t total_pixels;
     int64_t total_workgroups;

@@ -11,7 +11,7 @@ namespace {
 inline size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct BatchLayout {
-    size_t tiles, rect, depth, tet_view, sorted, ranges, bin, total;
+    size_t tiles, rect, depth, tet_blocks, sorted, ranges, bin, total;
 };
 
 BatchLayout batch_layout(const PlanHeader& h, int64_t cap) {
@@ -21,7 +21,7 @@ BatchLayout batch_layout(const PlanHeader& h, int64_t cap) {
     L.tiles = o; o += a256(4 * (size_t)T);
     L.rect = o; o += a256(16 * (size_t)T);
     L.depth = o; o += a256(4 * (size_t)T);
-    L.tet_view = o; o += a256(4 * (size_t)T);
+    L.tet_blocks = o; o += a256(8 * (size_t)T);
     L.sorted = o; o += a256(4 * (size_t)(cap > 0 ? cap : 1));
     L.ranges = o; o += a256(8 * (size_t)(h.total_tiles > 0 ? h.total_tiles : 1));
     L.bin = o; o += a256(batch_bin_workspace(h, cap));
@@ -88,7 +88,7 @@ extern "C" int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_
             r.indices = sc.indices_dev;
             r.verts = sc.vertices_dev;
             r.feats = sc.features_dev;
-            r.tet_offset = h.total_tets;
+            r.tet_offset = kPlanTetAlign * ceil_div(h.total_tets, (int64_t)kPlanTetAlign);
             r.n_tets = sc.n_tets;
             r.tile_offset = h.total_tiles;
             r.pixel_offset = h.total_pixels;
@@ -96,7 +96,7 @@ extern "C" int rm_batch_plan(const rm_scene* scenes, int32_t n_scenes, const rm_
             r.n_blocks = blocks;
             r.bin_offset = h.total_bin_blocks;
         }
-        h.total_tets += sc.n_tets;
+        h.total_tets = kPlanTetAlign * ceil_div(h.total_tets, (int64_t)kPlanTetAlign) + sc.n_tets;
         h.total_tiles += (int64_t)c.grid_height * c.grid_width;
         h.total_pixels += (int64_t)c.image_height * c.image_width;
         h.total_blocks += blocks;
@@ -146,12 +146,13 @@ extern "C" int rm_batch_render(const void* plan_dev, const rm_batch_info* info, 
     int32_t* tiles = reinterpret_cast<int32_t*>(w + L.tiles);
     int4* rect = reinterpret_cast<int4*>(w + L.rect);
     float* depth = reinterpret_cast<float*>(w + L.depth);
-    int32_t* tet_view = reinterpret_cast<int32_t*>(w + L.tet_view);
+    uint2* tet_blocks = reinterpret_cast<uint2*>(w + L.tet_blocks);
     int32_t* sorted = reinterpret_cast<int32_t*>(w + L.sorted);
     int32_t* ranges = reinterpret_cast<int32_t*>(w + L.ranges);
-    hipError_t e = batch_project(plan_dev, h, tiles, rect, depth, tet_view, st);
+    hipError_t e = batch_project(plan_dev, h, tiles, rect, depth, tet_blocks, st);
     if (e == hipSuccess)
-        e = batch_bin(plan_dev, h, pair_capacity, tiles, rect, depth, tet_view, counts_dev, sorted, ranges, w + L.bin, st);
+        e = batch_bin(plan_dev, h, pair_capacity, tiles, rect, depth, tet_blocks, counts_dev, sorted, ranges, w + L.bin,
+                      st);
     if (e == hipSuccess)
         e = batch_composite(plan_dev, h, sorted, pair_capacity, ranges, out_img_dev, out_dist_dev, out_ncontrib_dev, st);
     if (e != hipSuccess) {

@@ -70,12 +70,17 @@ inline CamArgs make_cam_args(const rm_camera& c) {
 }
 
 // One view of a batched render (device-resident plan record, packed by rm_batch_plan).
+// A view's first global tet in a batch plan is a multiple of this (the batched projection's
+// workgroup size), so that no projection workgroup straddles two views; the slots between one
+// view's last tet and the next view's first stay empty.
+constexpr int kPlanTetAlign = 256;
+
 struct ViewRec {
     CamArgs cam;
     const int32_t* indices;    // [n_tets, 4] of the view's scene
     const float* verts;        // [V, 3]
     const float* feats;        // [n_tets, F]
-    int64_t tet_offset;        // first global tet id of this view in the batch
+    int64_t tet_offset;        // first global tet id of this view in the batch (kPlanTetAlign-aligned)
     int64_t n_tets;
     int64_t tile_offset;       // first global tile id
     int64_t pixel_offset;      // first pixel of the view in the batch image buffers
@@ -89,6 +94,19 @@ struct PlanHeader {
     int64_t total_tets, total_tiles, total_pixels, total_blocks, total_bin_blocks;
 };
 
+// A wave-uniform pointer read from memory (a plan record) marked as a global-memory pointer, so
+// that the compiler loads through it with global instructions, as through a kernel argument,
+// rather than flat ones (which also count against the LDS wait counter: an LDS wait then waits for
+// them too). The empty asm statement (no instruction) keeps the compiler from folding the two
+// address-space casts away; its operand lives in scalar registers, hence wave-uniform pointers only.
+template <typename T>
+__device__ inline const T* uniform_global_ptr(const T* p) {
+    using G = const __attribute__((address_space(1))) T*;
+    G g = (G)p;
+    asm volatile("" : "+s"(g));
+    return (const T*)g;
+}
+
 __host__ __device__ inline const ViewRec* plan_views(const void* plan) {
     return reinterpret_cast<const ViewRec*>(static_cast<const char*>(plan) + sizeof(PlanHeader));
 }
@@ -96,14 +114,16 @@ __host__ __device__ inline const ViewRec* plan_views(const void* plan) {
 // Batched stages (defined in project.hip, sort.hip, composite.hip), all on one stream, sized by
 // the plan and a pair capacity: no host synchronisation inside.
 hipError_t batch_project(const void* plan_dev, const PlanHeader& h, int32_t* tiles, int4* rect, float* depth,
-                         int32_t* tet_view, hipStream_t st);
+                         uint2* tet_blocks, hipStream_t st);
 size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap);
 hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
-                     const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
+                     const int4* rect, const float* depth, const uint2* tet_blocks, int64_t* counts_dev,
                      int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st);
 int batch_workgroups_per_view(int tile, int gh, int gw);
-// Blocks of 8x8 tiles over a view's tile grid (the binning's unit).
-inline int64_t bin_blocks_per_view(int gh, int gw) { return (int64_t)((gw + 7) / 8) * ((gh + 7) / 8); }
+// The binning's unit: blocks of kBinBlockTiles x kBinBlockTiles tiles over a view's tile grid.
+constexpr int kBinBlockTiles = 8;
+__host__ __device__ inline int32_t bin_blocks_per_row(int gw) { return (gw + kBinBlockTiles - 1) / kBinBlockTiles; }
+inline int64_t bin_blocks_per_view(int gh, int gw) { return (int64_t)bin_blocks_per_row(gw) * bin_blocks_per_row(gh); }
 hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet,
                            int64_t pair_cap, const int32_t* ranges, float* img, float* dist, int32_t* nc,
                            hipStream_t st);

@@ -662,8 +662,9 @@ __global__ __launch_bounds__(Cfg<TS>::WG) void batch_composite_fwd_kernel(
     }
     const ViewRec& v = views[lo];
     fwd_tiles<TS, F, P64>(S, s_chunks, v.cam, (int)(bg - v.block_offset), sorted, n_sorted, ranges, v.tile_offset,
-                          v.tet_offset, reinterpret_cast<const int4*>(v.indices), v.verts, v.feats, nullptr, out_img,
-                          out_dist, out_nc, v.pixel_offset);
+                          v.tet_offset, uniform_global_ptr(reinterpret_cast<const int4*>(v.indices)),
+                          uniform_global_ptr(v.verts), uniform_global_ptr(v.feats), nullptr, out_img, out_dist,
+                          out_nc, v.pixel_offset);
 }
 
 // ------------------------------------------------------------------------------ backward --

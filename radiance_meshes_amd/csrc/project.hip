@@ -230,58 +230,40 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     cc[3 * t + 2] = ctr.z;
 }
 
-// Batched projection: lane g projects global tet g of the batch with its view's camera. A
-// workgroup whose tets all belong to one view (all but those that straddle two views) finds that
-// view once, so its camera is read with scalar loads instead of once per lane.
-__device__ inline int view_of_tet(const ViewRec* views, int n_views, int64_t g) {
-    int lo = 0, hi = n_views - 1;  // last view with tet_offset <= g
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (views[mid].tet_offset <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ void project_in_view(const ViewRec& v, int view, int64_t g, int32_t* __restrict__ tiles,
-                                       int4* __restrict__ rect, float* __restrict__ depth,
-                                       int32_t* __restrict__ tet_view) {
-    const int64_t t = g - v.tet_offset;
-    int32_t n;
-    int4 r;
-    float d;
-    F3 ctr;
-    project_one(v.cam, reinterpret_cast<const int4*>(v.indices)[t], v.verts, n, r, d, ctr);
-    tiles[g] = n;
-    rect[g] = r;
-    depth[g] = d;
-    tet_view[g] = view;
-}
-
+// Batched projection: lane g projects global tet g of the batch with its view's camera. Views
+// start at multiples of the workgroup size, so a workgroup's tets belong to one view, found once
+// per workgroup: the camera is read with scalar loads rather than once per lane.
+static_assert(kPlanTetAlign == 256, "one projection workgroup per aligned block of a view's tets");
 __global__ __launch_bounds__(256) void batch_project_kernel(const void* __restrict__ plan, int64_t total,
                                                             int32_t* __restrict__ tiles, int4* __restrict__ rect,
-                                                            float* __restrict__ depth, int32_t* __restrict__ tet_view) {
+                                                            float* __restrict__ depth, uint2* __restrict__ tet_blocks) {
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t g = g0 + threadIdx.x;
     if (g >= total) return;
     const PlanHeader* h = static_cast<const PlanHeader*>(plan);
     const ViewRec* views = plan_views(plan);
-    const int64_t nt = h->total_tets;
-    if (g >= nt) {  // launch sized for a capacity: slots past the plan's tets stay empty
-        tiles[g] = 0;
-        rect[g] = make_int4(0, 0, 0, 0);
-        depth[g] = 0.f;
-        tet_view[g] = 0;
-        return;
+    int v = 0, hi = h->n_views - 1;  // last view with tet_offset <= g0 (uniform)
+    while (v < hi) {
+        const int mid = (v + hi + 1) >> 1;
+        if (views[mid].tet_offset <= g0) v = mid;
+        else hi = mid - 1;
     }
-    const int64_t g1 = (g0 + (int64_t)blockDim.x < nt ? g0 + (int64_t)blockDim.x : nt) - 1;
-    const int v0 = view_of_tet(views, h->n_views, g0), v1 = view_of_tet(views, h->n_views, g1);
-    if (v0 == v1) {
-        project_in_view(views[v0], v0, g, tiles, rect, depth, tet_view);
-    } else {
-        const int v = view_of_tet(views, h->n_views, g);
-        project_in_view(views[v], v, g, tiles, rect, depth, tet_view);
+    const ViewRec& r = views[v];
+    const int64_t t = g - r.tet_offset;
+    int32_t n = 0;
+    int4 rc = make_int4(0, 0, 0, 0);
+    float d = 0.f;
+    // slots past the view's tets (alignment gaps, or a launch sized for a capacity) stay empty
+    if (g < h->total_tets && t < r.n_tets) {
+        F3 ctr;
+        project_one(r.cam, uniform_global_ptr(reinterpret_cast<const int4*>(r.indices))[t],
+                    uniform_global_ptr(r.verts), n, rc, d, ctr);
     }
+    tiles[g] = n;
+    rect[g] = rc;
+    depth[g] = d;
+    // the view's binning blocks: first global block and blocks per row of its tile grid
+    tet_blocks[g] = make_uint2((uint32_t)r.bin_offset, (uint32_t)bin_blocks_per_row(r.cam.gw));
 }
 
 // Adjoint of (clamped float(power), float3(center)) w.r.t. the four vertices, in FP64.
@@ -363,10 +345,10 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
 }  // namespace
 
 hipError_t batch_project(const void* plan_dev, const PlanHeader& h, int32_t* tiles, int4* rect, float* depth,
-                         int32_t* tet_view, hipStream_t st) {
+                         uint2* tet_blocks, hipStream_t st) {
     if (h.total_tets <= 0) return hipSuccess;
     hipLaunchKernelGGL(batch_project_kernel, dim3((unsigned)ceil_div(h.total_tets, 256)), dim3(256), 0, st, plan_dev,
-                       h.total_tets, tiles, rect, depth, tet_view);
+                       h.total_tets, tiles, rect, depth, tet_blocks);
     return hipGetLastError();
 }
 

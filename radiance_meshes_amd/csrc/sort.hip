@@ -158,12 +158,19 @@ hipError_t device_scan(Load ld, int64_t n, Out* out, bool inclusive, uint32_t* p
 
 // ---------------------------------------------------------------- radix sort ------------
 
+// Per-tile digit counts, digit-major over the tiles in use: counts[d * nbu + tile] with nbu the
+// tiles that hold keys (all nb of a launch without a device count; with one, a launch sized for a
+// capacity writes and scans only the counts of the tiles in use, and len_out receives their number
+// times the digit count, the length of the scan).
 template <typename K>
 __global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__ keys, int64_t cap,
                                                            const int64_t* n_dev, int shift, int bits,
-                                                           uint32_t* counts, int64_t nb) {
+                                                           uint32_t* counts, int64_t* len_out) {
     __shared__ uint32_t h[256];
     const int64_t n = eff_n(cap, n_dev);
+    const int64_t nbu = (n + kRsTile - 1) / kRsTile;
+    if (len_out && blockIdx.x == 0 && threadIdx.x == 0) *len_out = ((int64_t)1 << bits) * nbu;
+    if ((int64_t)blockIdx.x >= nbu) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
@@ -174,15 +181,16 @@ __global__ __launch_bounds__(kThreads) void rs_hist_kernel(const K* __restrict__
         if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & mask], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < (1 << bits)) counts[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < (1 << bits)) counts[(int64_t)threadIdx.x * nbu + blockIdx.x] = h[threadIdx.x];
 }
 
 template <typename K>
 __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t cap, const int64_t* n_dev, int shift, int bits,
-    const uint32_t* __restrict__ offsets, int64_t nb) {
+    const uint32_t* __restrict__ offsets) {
     const int64_t n = eff_n(cap, n_dev);
+    const int64_t nbu = (n + kRsTile - 1) / kRsTile;   // tiles in use (the offsets' row length)
     __shared__ uint32_t s_cnt[256];
     __shared__ uint32_t s_wcnt[4][256];
     __shared__ uint32_t s_loc[256];
@@ -195,8 +203,9 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
     const uint32_t mask = (1u << bits) - 1u;
     const int nbins = 1 << bits;
     const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    if (base >= n) return;   // a launch sized for a capacity: nothing of this tile is in use
     s_cnt[tid] = 0;
-    s_glob[tid] = (tid < nbins) ? offsets[(int64_t)tid * nb + blockIdx.x] : 0u;
+    s_glob[tid] = (tid < nbins) ? offsets[(int64_t)tid * nbu + blockIdx.x] : 0u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     K key[kRsItems];
@@ -259,7 +268,10 @@ __global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
 }
 
 inline int64_t rs_blocks(int64_t n) { return ceil_div(n, kRsTile); }
-inline size_t rs_counts_bytes(int64_t n) { return align256(sizeof(uint32_t) * 256 * (size_t)rs_blocks(n)); }
+// the digit counts, then the scan length of a sort with a device count (one int64)
+inline size_t rs_counts_bytes(int64_t n) {
+    return align256(sizeof(uint32_t) * 256 * (size_t)rs_blocks(n) + sizeof(int64_t));
+}
 
 // Stable LSD sort of (key, value) over key bits [begin_bit, end_bit). kin/vin are not modified;
 // the result lands in kout/vout. ktmp/vtmp are a ping-pong buffer of n entries; all three
@@ -277,6 +289,7 @@ hipError_t radix_sort(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout
     }
     int passes = (nbits + 7) / 8;
     int64_t nb = rs_blocks(n);
+    int64_t* len_dev = n_dev ? reinterpret_cast<int64_t*>(counts + 256 * nb) : nullptr;
     const K* ks = kin;
     const uint32_t* vs = vin;
     int shift = begin_bit;
@@ -286,12 +299,12 @@ hipError_t radix_sort(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout
         K* kd = to_out ? kout : ktmp;
         uint32_t* vd = to_out ? vout : vtmp;
         hipLaunchKernelGGL(rs_hist_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, n, n_dev, shift,
-                           bits, counts, nb);
+                           bits, counts, len_dev);
         hipError_t e = device_scan(LoadU32{counts}, (int64_t)(1 << bits) * nb, counts, false, partial,
-                                   (int64_t*)nullptr, st);
+                                   (int64_t*)nullptr, st, (const int64_t*)len_dev);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3((unsigned)nb), dim3(kThreads), 0, st, ks, vs, kd, vd, n,
-                           n_dev, shift, bits, (const uint32_t*)counts, nb);
+                           n_dev, shift, bits, (const uint32_t*)counts);
         ks = kd;
         vs = vd;
         shift += bits;
@@ -416,12 +429,16 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(const int32_t* __rest
 }
 
 // Expands depth-ordered visible tets into (tile id, tet id) pairs, row-major over each rect.
-// Work is balanced by pairs: workgroup b owns pairs [b*K, (b+1)*K); its first lane finds the
-// tets covering that range by binary search over the exclusive pair offsets, their offsets and
-// rects are staged in LDS (at most K tets, each owning at least one pair), and lanes take the
-// range's pairs round-robin, locating their tet by binary search in LDS. A screen-filling tet
-// (tens of thousands of pairs) is thus spread over many workgroups.
+// Work is balanced by pairs: a workgroup takes chunks of K pairs [c*K, (c+1)*K) in grid strides;
+// its first lane finds the tets covering a chunk by binary search over the exclusive pair offsets,
+// their offsets and rects are staged in LDS (at most K tets, each owning at least one pair), and
+// lanes take the chunk's pairs round-robin, locating their tet by binary search in LDS. A
+// screen-filling tet (tens of thousands of pairs) is thus spread over many chunks.
 constexpr int kPairsPerBlock = 1024;
+// Workgroups of the grid-stride kernels whose launches are sized by a pair capacity (pair
+// expansion, block ranges, segment kernels): enough to fill the chip many times over, few enough
+// that a capacity far above the pairs in use launches no flood of idle workgroups.
+constexpr int64_t kMaxStrideGrid = 16384;
 
 __device__ inline int64_t last_le(const uint32_t* __restrict__ a, int64_t n, uint32_t v) {
     int64_t lo = 0, hi = n - 1;  // a[0] == 0 <= v
@@ -458,41 +475,44 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
     __shared__ int s_n;
     const int64_t nvis = eff_n(nvis_cap, nvis_dev);
     const int64_t P = eff_n(P_cap, P_dev);
-    const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
-    if (p0 >= P || nvis <= 0) return;
-    const int64_t p1 = (p0 + kPairsPerBlock < P) ? p0 + kPairsPerBlock : P;
-    if (threadIdx.x == 0) {
-        int64_t lo = last_le(poff, nvis, (uint32_t)p0);
-        int64_t hi = last_le(poff, nvis, (uint32_t)(p1 - 1));
-        s_lo = lo;
-        s_n = (int)(hi - lo + 1);
-    }
-    __syncthreads();
-    const int64_t lo = s_lo;
-    const int n = s_n;
-    for (int j = threadIdx.x; j < n; j += kThreads) {
-        uint32_t t = order[lo + j];
-        s_off[j] = poff[lo + j];
-        s_tet[j] = t;
-        s_rect[j] = rect_of(rect[t]);
-        if constexpr (TileMap::kPerTet) s_map[j] = tile_of(t);
-    }
-    __syncthreads();
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
-        const uint32_t pp = (uint32_t)p;
-        int a = 0, b = n - 1;  // last j with s_off[j] <= p
-        while (a < b) {
-            int mid = (a + b + 1) >> 1;
-            if (s_off[mid] <= pp) a = mid;
-            else b = mid - 1;
+    if (nvis <= 0) return;
+    // chunks of kPairsPerBlock pairs in grid strides (a launch sized for a capacity stays small)
+    for (int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock; p0 < P; p0 += (int64_t)gridDim.x * kPairsPerBlock) {
+        const int64_t p1 = (p0 + kPairsPerBlock < P) ? p0 + kPairsPerBlock : P;
+        if (threadIdx.x == 0) {
+            int64_t lo = last_le(poff, nvis, (uint32_t)p0);
+            int64_t hi = last_le(poff, nvis, (uint32_t)(p1 - 1));
+            s_lo = lo;
+            s_n = (int)(hi - lo + 1);
         }
-        const int4 r = s_rect[a];
-        const uint32_t m = pp - s_off[a];
-        const uint32_t wdt = (uint32_t)(r.z - r.x);
-        const uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
-        const uint2 mp = TileMap::kPerTet ? s_map[a] : tile_of(0u);
-        ptile[p] = mp.x + y * mp.y + x;
-        ptet[p] = s_tet[a];
+        __syncthreads();
+        const int64_t lo = s_lo;
+        const int n = s_n;
+        for (int j = threadIdx.x; j < n; j += kThreads) {
+            uint32_t t = order[lo + j];
+            s_off[j] = poff[lo + j];
+            s_tet[j] = t;
+            s_rect[j] = rect_of(rect[t]);
+            if constexpr (TileMap::kPerTet) s_map[j] = tile_of(t);
+        }
+        __syncthreads();
+        for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
+            const uint32_t pp = (uint32_t)p;
+            int a = 0, b = n - 1;  // last j with s_off[j] <= p
+            while (a < b) {
+                int mid = (a + b + 1) >> 1;
+                if (s_off[mid] <= pp) a = mid;
+                else b = mid - 1;
+            }
+            const int4 r = s_rect[a];
+            const uint32_t m = pp - s_off[a];
+            const uint32_t wdt = (uint32_t)(r.z - r.x);
+            const uint32_t y = (uint32_t)r.y + m / wdt, x = (uint32_t)r.x + m % wdt;
+            const uint2 mp = TileMap::kPerTet ? s_map[a] : tile_of(0u);
+            ptile[p] = mp.x + y * mp.y + x;
+            ptet[p] = s_tet[a];
+        }
+        __syncthreads();   // the chunk's LDS records are rewritten by the next one
     }
 }
 
@@ -508,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void emit_pairs_kernel(
 // 64 tiles as 16 registers of four 8-bit counters and a wave-wide inclusive scan of those
 // registers sums all 64 counters at once (a field never exceeds 255). The lists equal those of a
 // stable sort of the (tile, tet) pairs by tile.
-constexpr int kBinBlock = 8;                         // tiles per block side
+constexpr int kBinBlock = kBinBlockTiles;            // tiles per block side
 constexpr int kBinLocal = kBinBlock * kBinBlock;     // tiles per block
 constexpr int kBinRegs = kBinLocal / 4;              // packed 8-bit counters per lane
 
@@ -551,7 +571,7 @@ struct GridBatch {   // views of a batch plan, looked up by global block or tile
     const void* plan;
     __device__ ViewGrid at(int v) const {
         const ViewRec& r = plan_views(plan)[v];
-        return {r.cam.gw, r.cam.gh, (r.cam.gw + kBinBlock - 1) / kBinBlock, r.tile_offset, r.bin_offset};
+        return {r.cam.gw, r.cam.gh, bin_blocks_per_row(r.cam.gw), r.tile_offset, r.bin_offset};
     }
     template <class Off>
     __device__ int last_view(Off off, int64_t id) const {   // last view whose first id is <= id
@@ -572,15 +592,10 @@ struct GridBatch {   // views of a batch plan, looked up by global block or tile
     }
     __device__ int64_t n_tiles() const { return static_cast<const PlanHeader*>(plan)->total_tiles; }
 };
-static_assert(kBinBlock == 8, "bin_blocks_per_view assumes 8x8-tile blocks");
-struct BlockMapBatch {  // blocks of the tet's view in a batch: first global block, blocks per row
-    static constexpr bool kPerTet = true;
-    const int32_t* tet_view;
-    const ViewRec* views;
-    __device__ uint2 operator()(uint32_t tet) const {
-        const ViewRec& v = views[tet_view[tet]];
-        return make_uint2((uint32_t)v.bin_offset, (uint32_t)((v.cam.gw + kBinBlock - 1) / kBinBlock));
-    }
+struct BlockMapBatch {  // blocks of the tet's view in a batch (first global block, blocks per row),
+    static constexpr bool kPerTet = true;   // written per tet by the batched projection
+    const uint2* tet_blocks;
+    __device__ uint2 operator()(uint32_t tet) const { return tet_blocks[tet]; }
 };
 
 // A block's list is cut into segments of kSegEntries; seg_first[b] (an exclusive scan over the
@@ -613,7 +628,9 @@ __global__ __launch_bounds__(kThreads) void seg_desc_kernel(const int2* __restri
 // Workgroups of the segment kernels: one per segment up to an estimate of the segment count from
 // the pair count (the blocks hold about a quarter of the tile pairs at 8x8 tiles per block), each
 // looping over further segments in grid strides when there are more.
-inline int64_t seg_grid(int64_t n_pairs, int64_t nb) { return ceil_div(n_pairs, 4 * kSegEntries) + nb; }
+inline int64_t seg_grid(int64_t n_pairs, int64_t nb) {
+    return std::min<int64_t>(ceil_div(n_pairs, 4 * kSegEntries) + nb, kMaxStrideGrid);
+}
 
 // Column-major per-segment counts, 0 past the last segment (those entries are never written).
 struct LoadSegColumn {
@@ -848,7 +865,8 @@ hipError_t block_bin_lists(const uint32_t* order, int64_t nvis_cap, const int64_
     int64_t* nbp = reinterpret_cast<int64_t*>(w + L.counts2);   // [0] block pairs, [1] segments
     device_scan(LoadBlockCount{(const uint32_t*)W(L.bcnt), order}, nvis_cap, W(L.poff), false, partial, nbp, st,
                 nvis_dev);
-    hipLaunchKernelGGL((emit_pairs_kernel<BlockMap, RectBlocks>), dim3((unsigned)ceil_div(pair_cap, kPairsPerBlock)),
+    hipLaunchKernelGGL((emit_pairs_kernel<BlockMap, RectBlocks>),
+                       dim3((unsigned)std::min<int64_t>(ceil_div(pair_cap, kPairsPerBlock), kMaxStrideGrid)),
                        dim3(kThreads), 0, st, order, (const uint32_t*)W(L.poff), rect, nvis_cap, nvis_dev, pair_cap,
                        (const int64_t*)nbp, bmap, W(L.ptile), W(L.ptet), RectBlocks{});
     const int bbits = bit_length(NB - 1) > 0 ? bit_length(NB - 1) : 1;
@@ -902,7 +920,7 @@ size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) {
 // tiles and blocks are numbered one view after the other, so the lists equal those of a stable
 // sort of the (global tile, tet) pairs by tile.
 hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
-                     const int4* rect, const float* depth, const int32_t* tet_view, int64_t* counts_dev,
+                     const int4* rect, const float* depth, const uint2* tet_blocks, int64_t* counts_dev,
                      int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st) {
     const int64_t T = h.total_tets, G = h.total_tiles, NB = h.total_bin_blocks;
     BinLayout L = bin_layout(T, pair_cap, G);
@@ -919,7 +937,7 @@ hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap
                                         32, W(L.rcounts), W(L.partial), st, counts_dev);
     if (e != hipSuccess) return e;
     return block_bin_lists(W(L.vidx2), T, counts_dev, rect, L, w, pair_cap, G, NB,
-                           BlockMapBatch{tet_view, plan_views(plan_dev)}, GridBatch{plan_dev}, sorted_tet, ranges, st);
+                           BlockMapBatch{tet_blocks}, GridBatch{plan_dev}, sorted_tet, ranges, st);
 }
 
 }  // namespace rmesh
@@ -1070,7 +1088,7 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
     radix_sort<uint32_t>(WS(uint32_t, L.vkey), WS(uint32_t, L.vidx), WS(uint32_t, L.vkey2), WS(uint32_t, L.vidx2),
                          WS(uint32_t, L.vkey3), WS(uint32_t, L.vidx3), n_visible, 0, 32, rcounts, partial, st);
     // 3.-5. (block, tet) pairs in depth order, sorted by block; tile lists through block segments
-    const int32_t bgw = (grid_width + kBinBlock - 1) / kBinBlock, bgh = (grid_height + kBinBlock - 1) / kBinBlock;
+    const int32_t bgw = bin_blocks_per_row(grid_width), bgh = bin_blocks_per_row(grid_height);
     block_bin_lists(WS(uint32_t, L.vidx2), n_visible, (const int64_t*)nullptr, (const int4*)rect_dev, L,
                     static_cast<char*>(workspace_dev), n_pairs, G, (int64_t)bgw * bgh, TileMapSingle{bgw},
                     GridSingle{grid_width, grid_height, bgw}, sorted_tet_dev, ranges_dev, st);

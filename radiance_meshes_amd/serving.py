@@ -124,7 +124,8 @@ class Bucket:
             return ("full",), self.info, self.pair_cap
         scs = self.server.scenes
         nv = min(self.slots, _pow2_at_least(len(reqs)))
-        T = min(int(self.info.total_tets), _pow2_at_least(sum(scs[r.scene].rec.n_tets for r in reqs)))
+        # a plan starts every view's tets at a multiple of 256 (rm_batch_plan)
+        T = min(int(self.info.total_tets), _pow2_at_least(sum(-(-scs[r.scene].rec.n_tets // 256) * 256 for r in reqs)))
         est = sum(int(self.server.pair_margin * self.scene_pairs[r.scene]) + 4096 for r in reqs)
         P = min(self.pair_cap, _pow2_at_least(est))
         info = RmBatchInfo(n_views=nv, total_tets=T, total_tiles=nv * self.view_tiles,
