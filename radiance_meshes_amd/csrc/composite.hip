@@ -8,11 +8,12 @@
 // the one after which log-transmittance drops below -5.5.
 //
 // MI355X layout: at 8x8 and 16x16 tiles a workgroup of tile^2 lanes holds one tile, a lane per
-// pixel. At 4x4 tiles one wave takes a tile with the lanes on list entries instead (the forward
-// composite_fwd_tet_lanes_kernel, the backward and the error votes alike): a lane evaluates its
-// tet for each of the tile's 16 pixels and each pixel's walk state comes from prefix sums across
-// the lanes. Each chunk of a tile's list is staged as per-view plane records computed once per
-// (tile, tet): the four outward face planes n_i with offsets e_i = n_i . (cam - v_a), and
+// pixel. At 4x4 tiles one wave takes a tile: the forward gives each pixel a quad of lanes that
+// walk four entries per step (composite_fwd_quads_kernel); the backward and the error votes put
+// the lanes on list entries, a lane evaluating its tet for each of the tile's 16 pixels, with each
+// pixel's walk state from prefix sums across the lanes. Each chunk of a tile's list is staged as
+// per-view plane records computed once per (tile, tet): the four outward face planes n_i with
+// offsets e_i = n_i . (cam - v_a), and
 // the colour folded to c(s) = base + g.(cam - v0) + s * g.d, where s is the distance from the
 // camera centre along the unit ray. A plane crossing is then s_i = -e_i / (n_i . d) and the
 // reference's ray parameter is t = s - min_t (its origin is cam + min_t * d).
@@ -1189,30 +1190,6 @@ __device__ inline float wave_scan_incl(float x) {
 // the inclusive sum would leave a rounding error of the size of that term).
 __device__ inline float wave_scan_excl(float x) { return wave_scan_incl(dpp_rm<0x138, 0xF>(x)); }
 
-// N inclusive (or, with the values first shifted one lane up, exclusive) wave prefix sums at once:
-// the steps of the N chains interleaved, so that no DPP read waits on the write just before it.
-template <int N>
-__device__ inline void wave_scan_incl_n(float (&v)[N]) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x111, 0xF>(v[i]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x112, 0xF>(v[i]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x114, 0xF>(v[i]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x118, 0xF>(v[i]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x142, 0xA>(v[i]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += dpp_rm<0x143, 0xC>(v[i]);
-}
-template <int N>
-__device__ inline void wave_scan_excl_n(float (&v)[N]) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = dpp_rm<0x138, 0xF>(v[i]);
-    wave_scan_incl_n<N>(v);
-}
-
 // m ? a : b for a lane mask m of all ones or all zeros, as bit operations (kept branch-free)
 __device__ inline float bitsel(int m, float a, float b) {
     return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
@@ -1488,134 +1465,156 @@ __global__ __launch_bounds__(64, 3) void composite_bwd_tet_lanes_kernel(
     }
 }
 
-// ------------------------------------------------------------- forward, one lane per tet --
+// ------------------------------------------------------------- forward, pixel quads --
 //
-// The 4x4-tile forward with the roles of lanes and pixels exchanged, as in the backward above: one
-// wave per tile, a chunk holds the next 64 of the tile's list entries front to back, lane j the
-// j-th, its plane records in LDS, colour in registers. The wave then takes the tile's pixels that
-// are still walking; for pixel p every lane evaluates its tet, and the pixel's walk over the chunk
-// follows from wave scans: log T in front of entry j is the pixel's log T at the chunk's start
-// (FP64, lane p) minus the exclusive sum of the blended x = sigma dt of lanes 0..j-1, the walk ends
-// at the first blended entry behind which log T falls below -5.5 (as the sequential walk does),
-// and the colour and the distortion moments of the entries up to it are summed across the wave
-// (Z and W in front of an entry from exclusive scans of sigma and m1). n_contrib counts the
-// entries walked, blended or not; with flags, entry j's flag is set when some pixel blended it.
+// The 4x4-tile forward with four lanes per pixel: lane 4p + k walks pixel p over entry 4g + k of
+// each group g of four consecutive list entries, so a pixel's walk advances four entries per step.
+// Chunks of the tile's next 64 entries are staged in LDS, one entry per lane, through the load
+// pipeline of the 8x8 and 16x16 forward. Within a step the four entries' intersections and blend
+// terms do not depend on the pixel's state; the state in front of each entry (log T, Z, W) comes
+// from exclusive prefix sums over the quad of lanes, the walk ends at the first blended entry
+// behind which log T falls below -5.5 (as the sequential walk does), and sums over the quad of
+// the entries up to it update the pixel's state, which all four lanes hold.
+template <int CTRL>
+__device__ inline float dpp_q(float v) {   // a quad permutation (all lanes of a quad are active together)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// N sums over the quad, every lane receiving them (butterflies, the N chains interleaved).
+template <int N>
+__device__ inline void quad_sum_n(float (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_q<0xB1>(v[i]);   // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_q<0x4E>(v[i]);   // quad_perm [2,3,0,1]
+}
+// N exclusive prefix sums over the quad: the values shifted one lane up (quad_perm [0,0,1,2],
+// lane 0 of the quad takes 0), then the inclusive sum in two steps (lanes 1.. add lane k-1, lanes
+// 2.. add lane k-2, through quad_perm [0,0,0,1]). The lanes that take nothing are masked with bit
+// operations, never with a select: the compiler may turn a select into a branch and move the DPP
+// read into it, and a DPP read from a lane outside exec returns the old value, not the lane's.
+template <int N>
+__device__ inline void quad_scan_excl_n(float (&v)[N], int k) {
+    const int m1 = -(int)(k >= 1), m2 = -(int)(k >= 2);
+    auto keep = [](float x, int m) { return __int_as_float(__float_as_int(x) & m); };
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = keep(dpp_q<0x90>(v[i]), m1);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += keep(dpp_q<0x90>(v[i]), m1);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += keep(dpp_q<0x40>(v[i]), m2);
+}
+
 template <int F, bool P64, bool FLAGS>
-__device__ __forceinline__ void fwd_tet_lanes_tile(
-    const CamArgs& cam, int tile, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
-    int64_t tile_base, int64_t tet_base, const int4* __restrict__ indices, const float* __restrict__ verts,
-    const float* __restrict__ feats, const float2* __restrict__ jitter, float4* __restrict__ out_img,
-    float* __restrict__ out_dist, int* __restrict__ out_nc, int64_t pix_base, uint8_t* __restrict__ flags,
-    typename Prec<P64>::PL (&s_pl)[4][64]) {
+__device__ __forceinline__ void fwd_quads_tile(
+    Stage<64, F, false, P64>& S, const CamArgs& cam, int tile, const int* __restrict__ sorted, int64_t n_sorted,
+    const int2* __restrict__ ranges, int64_t tile_base, int64_t tet_base, const int4* __restrict__ indices,
+    const float* __restrict__ verts, const float* __restrict__ feats, const float2* __restrict__ jitter,
+    float4* __restrict__ out_img, float* __restrict__ out_dist, int* __restrict__ out_nc, int64_t pix_base,
+    uint8_t* __restrict__ flags) {
     constexpr int NGRD = NG_<F>::NGRD;
     using T = typename Prec<P64>::R;
     using PL = typename Prec<P64>::PL;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, k = lane & 3, p = lane >> 2;
     const int tx = tile % cam.gw, ty = tile / cam.gw;
     const int2 rg = load_range(ranges, true, (int)(tile_base + tile), n_sorted);
     const int n = rg.y - rg.x;
-
-    // pixel lane p < 16: its ray and walk state
-    const int px = tx * 4 + (lane & 3), py = ty * 4 + ((lane >> 2) & 3);
-    const bool inside = lane < 16 && px < cam.W && py < cam.H;
+    const int px = tx * 4 + (p & 3), py = ty * 4 + (p >> 2);
+    const bool inside = px < cam.W && py < cam.H;
     const int64_t pix = pix_base + (int64_t)py * cam.W + px;
-    RayT<T> pray = {0, 0, 1};
+    RayT<T> ray = {0, 0, 1};
     if (inside) {
         const float2 j = jitter ? jitter[pix - pix_base] : make_float2(0.5f, 0.5f);
-        pray = make_ray<T>(cam, px, py, j.x, j.y);
+        ray = make_ray<T>(cam, px, py, j.x, j.y);
     }
+    const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
+    const float mt = cam.min_t;
     float cr = 0.f, cg = 0.f, cb = 0.f, X = 0.f, Y = 0.f, Z = 0.f, W = 0.f, V = 0.f;
     double L = 0.0;
     int nc = 0;
-    uint32_t alive = (uint32_t)__ballot(inside) & 0xFFFFu;   // pixels still walking
-    const float mt = cam.min_t;
-
-    for (int c0 = 0; c0 < n && alive != 0; c0 += 64) {
-        const int cnt = n - c0 < 64 ? n - c0 : 64;
-        const bool valid = lane < cnt;
-        PL pl[4];
-        float4 fa = make_float4(0.f, 0.f, 0.f, 0.f);
-        float gr[NGRD];
-        if (valid) {
-            const int tet = (int)(sorted[rg.x + c0 + lane] - tet_base);
-            TetData<F> d;
-            load_tet_data<F>(d, indices[tet], tet, verts, feats);
-            tet_record<F, P64>(d, cam, pl, fa, gr);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {   // planes no ray crosses ahead of the camera: a miss
-                pl[i].x = 0; pl[i].y = 0; pl[i].z = 0; pl[i].w = 1;
-            }
-#pragma unroll
-            for (int k = 0; k < NGRD; ++k) gr[k] = 0.f;
-        }
-        __syncthreads();   // the previous chunk's records have been read
-#pragma unroll
-        for (int i = 0; i < 4; ++i) s_pl[i][lane] = pl[i];
-        const float sig = fa.x;
-        bool blended_any = false;
-
-        for (int p = 0; p < 16; ++p) {
-            if (!((alive >> p) & 1u)) continue;
-            const RayT<T> ray = {lane_f(pray.dx, p), lane_f(pray.dy, p), lane_f(pray.dz, p)};
-            const float rdx = (float)ray.dx, rdy = (float)ray.dy, rdz = (float)ray.dz;
-            const PL plp[4] = {s_pl[0][lane], s_pl[1][lane], s_pl[2][lane], s_pl[3][lane]};
-            const auto h = intersect(plp, ray, mt);
-            const float x = sig * h.dt;
-            const bool bl = valid && h.hit && x > 0.f;
-            const float te = h.se - mt, tx_ = h.sx - mt;
-            const float m1 = sig * (te + tx_) * 0.5f;
-            // the blended entries' x, sigma and m1, summed over the lanes in front: up to the entry
-            // that ends the walk, the entries in front are the blended ones
-            const float xa = bl ? x : 0.f, sa = bl ? sig : 0.f, ma = bl ? m1 : 0.f;
-            float sc[3] = {xa, sa, ma};
-            wave_scan_excl_n<3>(sc);
-            const float ex = sc[0], es = sc[1], em = sc[2];
-            const double Lp = lane_f(L, p);
-            const uint64_t stop = __ballot(bl && Lp - (double)(ex + xa) < -5.5);
-            const int first = stop ? (int)__builtin_ctzll(stop) : 64;
-            const bool act = bl && lane <= first;
-            blended_any |= act;
-            float c_r = 0.f, c_g = 0.f, c_b = 0.f, c_x = 0.f, c_y = 0.f, c_v = 0.f;
-            if (act) {
-                const float em1 = expm1_neg(x);
-                const float alpha = 1.0f + em1;
-                float w0, w1;
-                seg_weights(x, em1, alpha, w0, w1);
-                const float Tr = __expf((float)(Lp - (double)ex));
-                const float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
-                float gd1 = gd0, gd2 = gd0;
-                if (F == 13) {
-                    gd1 = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
-                    gd2 = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
+    bool alive = inside;
+    const int nchunks = (n + 63) / 64;
+    auto entry = [&](int cc) {
+        const int kk = cc * 64 + lane;
+        return kk < n ? (int)(sorted[rg.x + kk] - tet_base) : -1;
+    };
+    ChunkPipe<F> pipe;
+    if (nchunks > 0) pipe.start(entry, indices, verts, feats);
+    for (int c = 0; c < nchunks; ++c) {
+        if (!__syncthreads_or(alive)) break;
+        pipe.stage(S, lane, cam);
+        __syncthreads();
+        pipe.advance(entry, c, indices, verts, feats);
+        const int cnt = n - c * 64 < 64 ? n - c * 64 : 64;
+        uint32_t act_bits = 0;   // bit g: this lane's entry of group g was blended
+        for (int g = 0; 4 * g < cnt; ++g) {
+            if (__ballot(alive) == 0) break;
+            if (alive) {
+                const int e = 4 * g + k;
+                const bool valid = e < cnt;
+                const PL plp[4] = {S.pl[0][e], S.pl[1][e], S.pl[2][e], S.pl[3][e]};
+                const auto h = intersect(plp, ray, mt);
+                const float4 fa = S.fa[e];
+                const float sig = fa.x;
+                const float x = sig * h.dt;
+                const bool bl = valid && h.hit && x > 0.f;
+                const float te = h.se - mt, tx_ = h.sx - mt;
+                const float m1 = sig * (te + tx_) * 0.5f;
+                const float xa = bl ? x : 0.f;
+                float sc[3] = {xa, bl ? sig : 0.f, bl ? m1 : 0.f};
+                quad_scan_excl_n<3>(sc, k);
+                const float ex = sc[0], es = sc[1], em = sc[2];
+                const uint64_t stop = __ballot(bl && L - (double)(ex + xa) < -5.5);
+                const uint32_t qb = (uint32_t)(stop >> (lane & ~3)) & 0xFu;
+                const int first = qb ? (int)__builtin_ctz(qb) : 4;
+                const bool act = bl && k <= first;
+                act_bits |= (uint32_t)act << g;
+                float rd[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                if (act) {
+                    float gr[NGRD];
+                    read_grads(S, e, gr);
+                    const float em1 = expm1_neg(x);
+                    const float alpha = 1.0f + em1;
+                    float w0, w1;
+                    seg_weights(x, em1, alpha, w0, w1);
+                    const float Tr = __expf((float)(L - (double)ex));
+                    const float gd0 = gr[0] * rdx + gr[1] * rdy + gr[2] * rdz;
+                    float gd1 = gd0, gd2 = gd0;
+                    if (F == 13) {
+                        gd1 = gr[3 % NGRD] * rdx + gr[4 % NGRD] * rdy + gr[5 % NGRD] * rdz;
+                        gd2 = gr[6 % NGRD] * rdx + gr[7 % NGRD] * rdy + gr[8 % NGRD] * rdz;
+                    }
+                    const float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
+                    rd[0] = Tr * (wsum * fa.y + wsl * gd0);
+                    rd[1] = Tr * (wsum * fa.z + wsl * gd1);
+                    rd[2] = Tr * (wsum * fa.w + wsl * gd2);
+                    rd[3] = 2.0f * m1 * (Z + es);
+                    rd[4] = 2.0f * sig * (W + em);
+                    rd[5] = sig * sig * h.dt * (1.0f / 3.0f);
+                    rd[6] = x;
+                    rd[7] = sig;
+                    rd[8] = m1;
                 }
-                const float wsum = w0 + w1, wsl = w0 * h.sx + w1 * h.se;
-                c_r = Tr * (wsum * fa.y + wsl * gd0);
-                c_g = Tr * (wsum * fa.z + wsl * gd1);
-                c_b = Tr * (wsum * fa.w + wsl * gd2);
-                c_x = 2.0f * m1 * (lane_f(Z, p) + es);
-                c_y = 2.0f * sig * (lane_f(W, p) + em);
-                c_v = sig * sig * h.dt * (1.0f / 3.0f);
-            }
-            float rd[6] = {c_r, c_g, c_b, c_x, c_y, c_v};
-            wave_scan_incl_n<6>(rd);
-            const float tr = lane_f(rd[0], 63), tg = lane_f(rd[1], 63), tb = lane_f(rd[2], 63),
-                        tX = lane_f(rd[3], 63), tY = lane_f(rd[4], 63), tV = lane_f(rd[5], 63);
-            const int last = first < cnt ? first : cnt - 1;           // the last entry walked
-            const float tL = lane_f(ex + xa, last), tZ = lane_f(es + sa, last), tW = lane_f(em + ma, last);
-            if (first < 64) alive &= ~(1u << p);                      // the pixel stopped in this chunk
-            if (lane == p) {
-                cr += tr; cg += tg; cb += tb;
-                L -= (double)tL;
-                X += tX; Y += tY; Z += tZ; W += tW; V += tV;
-                nc += last + 1;
+                quad_sum_n<9>(rd);
+                cr += rd[0]; cg += rd[1]; cb += rd[2];
+                X += rd[3]; Y += rd[4]; V += rd[5];
+                L -= (double)rd[6];
+                Z += rd[7]; W += rd[8];
+                const int left = cnt - 4 * g;
+                nc += first < 4 ? first + 1 : (left < 4 ? left : 4);
+                if (first < 4) alive = false;
             }
         }
         if constexpr (FLAGS) {
-            if (valid) flags[rg.x + c0 + lane] = (uint8_t)blended_any;
+            // entry e of the chunk: 1 when some pixel blended it; lane e holds slot e % 4's bits
+            uint32_t any = act_bits;
+            any |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)any, 0x124, 0xF, 0xF, false);   // row_ror:4
+            any |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)any, 0x128, 0xF, 0xF, false);   // row_ror:8
+            any |= (uint32_t)__shfl_xor((int)any, 16, 64);
+            any |= (uint32_t)__shfl_xor((int)any, 32, 64);
+            if (lane < cnt) flags[rg.x + c * 64 + lane] = (uint8_t)((any >> (lane >> 2)) & 1u);
         }
     }
-    if (inside) {
+    if (inside && k == 0) {
         out_img[pix] = make_float4(cr, cg, cb, (float)L);
         float* d = out_dist + 5 * pix;
         d[0] = X; d[1] = Y; d[2] = Z; d[3] = W; d[4] = V;
@@ -1624,22 +1623,22 @@ __device__ __forceinline__ void fwd_tet_lanes_tile(
 }
 
 template <int F, bool P64, bool FLAGS>
-__global__ __launch_bounds__(64) void composite_fwd_tet_lanes_kernel(
+__global__ __launch_bounds__(64) void composite_fwd_quads_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
     int* __restrict__ out_nc, uint8_t* __restrict__ flags) {
-    __shared__ typename Prec<P64>::PL s_pl[4][64];
-    fwd_tet_lanes_tile<F, P64, FLAGS>(cam, remap_block(blockIdx.x, gridDim.x), sorted, n_sorted, ranges, 0, 0,
-                                      indices, verts, feats, jitter, out_img, out_dist, out_nc, 0, flags, s_pl);
+    __shared__ Stage<64, F, false, P64> S;
+    fwd_quads_tile<F, P64, FLAGS>(S, cam, remap_block(blockIdx.x, gridDim.x), sorted, n_sorted, ranges, 0, 0, indices,
+                                  verts, feats, jitter, out_img, out_dist, out_nc, 0, flags);
 }
 
 // Batched 4x4-tile forward: one workgroup per tile of every view of a plan (pixel-centre rays).
 template <int F, bool P64>
-__global__ __launch_bounds__(64) void batch_composite_fwd_tet_lanes_kernel(
+__global__ __launch_bounds__(64) void batch_composite_fwd_quads_kernel(
     const void* __restrict__ plan, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     float4* __restrict__ out_img, float* __restrict__ out_dist, int* __restrict__ out_nc) {
-    __shared__ typename Prec<P64>::PL s_pl[4][64];
+    __shared__ Stage<64, F, false, P64> S;
     const PlanHeader* h = static_cast<const PlanHeader*>(plan);
     const ViewRec* views = plan_views(plan);
     const int bg = remap_block(blockIdx.x, gridDim.x);
@@ -1651,10 +1650,10 @@ __global__ __launch_bounds__(64) void batch_composite_fwd_tet_lanes_kernel(
         else hi = mid - 1;
     }
     const ViewRec& v = views[lo];
-    fwd_tet_lanes_tile<F, P64, false>(v.cam, (int)(bg - v.block_offset), sorted, n_sorted, ranges, v.tile_offset,
-                                      v.tet_offset, uniform_global_ptr(reinterpret_cast<const int4*>(v.indices)),
-                                      uniform_global_ptr(v.verts), uniform_global_ptr(v.feats), nullptr, out_img,
-                                      out_dist, out_nc, v.pixel_offset, nullptr, s_pl);
+    fwd_quads_tile<F, P64, false>(S, v.cam, (int)(bg - v.block_offset), sorted, n_sorted, ranges, v.tile_offset,
+                                  v.tet_offset, uniform_global_ptr(reinterpret_cast<const int4*>(v.indices)),
+                                  uniform_global_ptr(v.verts), uniform_global_ptr(v.feats), nullptr, out_img, out_dist,
+                                  out_nc, v.pixel_offset, nullptr);
 }
 
 template <int TS, int F, bool P64>
@@ -1664,12 +1663,12 @@ int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
     using C = Cfg<TS>;
     if constexpr (TS == 4) {
         if (flags)
-            hipLaunchKernelGGL((composite_fwd_tet_lanes_kernel<F, P64, true>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
+            hipLaunchKernelGGL((composite_fwd_quads_kernel<F, P64, true>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
                                sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
                                (const float2*)jitter, (float4*)img, dist, nc, flags);
         else
-            hipLaunchKernelGGL((composite_fwd_tet_lanes_kernel<F, P64, false>), dim3(a.gw * a.gh), dim3(64), 0, st,
-                               a, sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
+            hipLaunchKernelGGL((composite_fwd_quads_kernel<F, P64, false>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
+                               sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
                                (const float2*)jitter, (float4*)img, dist, nc, flags);
     } else {
         int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
@@ -2143,7 +2142,7 @@ template <int TS, int F, bool P64>
 void launch_batch(const void* plan, const PlanHeader& h, const int32_t* sorted, int64_t cap, const int32_t* ranges,
                   float* img, float* dist, int32_t* nc, hipStream_t st) {
     if constexpr (TS == 4)
-        hipLaunchKernelGGL((batch_composite_fwd_tet_lanes_kernel<F, P64>), dim3((unsigned)h.total_blocks), dim3(64),
+        hipLaunchKernelGGL((batch_composite_fwd_quads_kernel<F, P64>), dim3((unsigned)h.total_blocks), dim3(64),
                            0, st, plan, sorted, cap, (const int2*)ranges, (float4*)img, dist, nc);
     else
         hipLaunchKernelGGL((batch_composite_fwd_kernel<TS, F, P64>), dim3((unsigned)h.total_blocks),
