@@ -133,14 +133,25 @@ int rm_composite_fwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_
 /* rm_composite_fwd that also records, for the backward, which list entries matter: for every entry
  * k (an index into sorted_tet_dev) that the pixels of its tile walked, pair_flags_dev[k] = 1 when some
  * pixel of the tile blended that tet (a hit with sigma * dt > 0) and 0 otherwise; entries no pixel
- * reached are left unwritten. pair_flags_dev holds n_sorted bytes. Square tiles 4 and 8 (a tile
- * within one wavefront); with tile 16 nothing is written. Same image outputs as rm_composite_fwd.
+ * reached are left unwritten. pair_flags_dev holds n_sorted bytes, or is null (nothing recorded).
+ * Square tiles 4 and 8 (a tile within one wavefront); with tile 16 nothing is written. With 4x4
+ * tiles, tile_order_dev (null, or the gw*gh tile ids in any order, e.g. from rm_tile_order) is the
+ * order in which the tiles are taken up. Same image outputs as rm_composite_fwd, whatever the order.
  * No counterpart in the reference (its backward revisits every entry). */
 int rm_composite_fwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
                            const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                            const float* features_dev, int32_t n_features, const float* jitter_dev,
                            float* out_img_dev, float* out_dist_dev, int32_t* out_ncontrib_dev,
-                           uint8_t* pair_flags_dev, rm_stream_t stream);
+                           uint8_t* pair_flags_dev, const int32_t* tile_order_dev, rm_stream_t stream);
+
+/* The tiles by decreasing list length (ranges [n_tiles, 2] as written by rm_bin_sort; lengths
+ * above 65535 count as 65535; tiles of equal length in increasing id) into order_dev [n_tiles]:
+ * a tile's walk runs on one wavefront, so taking the longest lists first keeps a late long walk
+ * from running alone at the end of a launch. The workspace holds rm_tile_order_workspace_size
+ * bytes (256-byte aligned). No counterpart in the reference. */
+size_t rm_tile_order_workspace_size(int64_t n_tiles);
+int rm_tile_order(const int32_t* ranges_dev, int64_t n_tiles, int32_t* order_dev, void* workspace_dev,
+                  size_t workspace_bytes, rm_stream_t stream);
 
 /* Precision of the per-pixel ray/plane intersection in rm_composite_fwd/bwd (process-wide):
  * 1 (default) = FP64 plane records and crossing distances, rounded to FP32 once the segment
@@ -181,14 +192,15 @@ int rm_composite_bwd_ws(const rm_camera* cam, const int32_t* sorted_tet_dev, int
  * list entries rm_composite_fwd_flags flagged for the same forward: an entry no pixel of its tile
  * blended contributes no gradient, so staging, intersecting and reducing it is skipped. Same
  * gradients as rm_composite_bwd_ws up to the order of float additions. With tile 16 the flags are
- * not read and every entry is walked. */
+ * not read and every entry is walked. tile_order_dev as for rm_composite_fwd_flags (4x4 tiles). */
 int rm_composite_bwd_flags(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
                            const int32_t* ranges_dev, const int32_t* indices_dev, int64_t n_tets,
                            const float* vertices_dev, const float* features_dev, int32_t n_features,
                            const float* jitter_dev, const float* img_dev, const float* dist_dev,
-                           const int32_t* ncontrib_dev, const uint8_t* pair_flags_dev, const float* grad_img_dev,
-                           const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
-                           void* workspace_dev, size_t workspace_bytes, rm_stream_t stream);
+                           const int32_t* ncontrib_dev, const uint8_t* pair_flags_dev,
+                           const int32_t* tile_order_dev, const float* grad_img_dev, const float* grad_dist_dev,
+                           float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
+                           size_t workspace_bytes, rm_stream_t stream);
 
 /* No counterpart in the reference: rm_composite_bwd_ws without a workspace argument, for callers
  * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that

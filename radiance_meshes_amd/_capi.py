@@ -60,7 +60,7 @@ SIGNATURES = {
     "rm_bin_sort": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, _I32, _I32, _P, _P, _P, _SZ, _P]),
     "rm_composite_fwd": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P]),
     "rm_composite_fwd_flags": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P,
-                                         _P]),
+                                         _P, _P]),
     "rm_set_intersect_precision": (C.c_int, [C.c_int]),
     "rm_composite_bwd": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P,
                                    _P, _P, _P, _P, _P]),
@@ -68,7 +68,9 @@ SIGNATURES = {
     "rm_composite_bwd_ws": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _I64, _P, _P, _I32, _P, _P, _P, _P,
                                       _P, _P, _P, _P, _P, _SZ, _P]),
     "rm_composite_bwd_flags": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _I64, _P, _P, _I32, _P, _P, _P, _P,
-                                         _P, _P, _P, _P, _P, _P, _SZ, _P]),
+                                         _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "rm_tile_order_workspace_size": (_SZ, [_I64]),
+    "rm_tile_order": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "rm_composite_bwd_noalloc_max_tets": (_I64, []),
     "rm_composite_bwd_noalloc": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _I64, _P, _P, _I32, _P, _P, _P,
                                            _P, _P, _P, _P, _P, _P]),

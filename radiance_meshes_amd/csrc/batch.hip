@@ -11,7 +11,7 @@ namespace {
 inline size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct BatchLayout {
-    size_t tiles, rect, depth, tet_blocks, sorted, ranges, bin, total;
+    size_t tiles, rect, depth, tet_blocks, sorted, ranges, order, bin, total;
 };
 
 BatchLayout batch_layout(const PlanHeader& h, int64_t cap) {
@@ -24,6 +24,7 @@ BatchLayout batch_layout(const PlanHeader& h, int64_t cap) {
     L.tet_blocks = o; o += a256(8 * (size_t)T);
     L.sorted = o; o += a256(4 * (size_t)(cap > 0 ? cap : 1));
     L.ranges = o; o += a256(8 * (size_t)(h.total_tiles > 0 ? h.total_tiles : 1));
+    L.order = o; o += a256(4 * (size_t)(h.total_tiles > 0 ? h.total_tiles : 1));
     L.bin = o; o += a256(batch_bin_workspace(h, cap));
     L.total = o;
     return L;
@@ -135,6 +136,8 @@ extern "C" int rm_batch_render(const void* plan_dev, const rm_batch_info* info, 
                  "rm_batch_render: misaligned output or workspace");
     RM_CHECK_ARG(info->total_bin_blocks > 0 && info->total_bin_blocks <= info->total_tiles,
                  "rm_batch_render: total_bin_blocks must be set (rm_batch_plan's value or a capacity)");
+    RM_CHECK_ARG(info->total_workgroups <= info->total_tiles,
+                 "rm_batch_render: total_workgroups exceeds total_tiles (one workgroup per tile)");
     const PlanHeader h = header_of(info);
     const BatchLayout L = batch_layout(h, pair_capacity);
     if (workspace_bytes < L.total) {
@@ -149,12 +152,14 @@ extern "C" int rm_batch_render(const void* plan_dev, const rm_batch_info* info, 
     uint2* tet_blocks = reinterpret_cast<uint2*>(w + L.tet_blocks);
     int32_t* sorted = reinterpret_cast<int32_t*>(w + L.sorted);
     int32_t* ranges = reinterpret_cast<int32_t*>(w + L.ranges);
+    int32_t* order = reinterpret_cast<int32_t*>(w + L.order);   // tiles by decreasing pair count
     hipError_t e = batch_project(plan_dev, h, tiles, rect, depth, tet_blocks, st);
     if (e == hipSuccess)
-        e = batch_bin(plan_dev, h, pair_capacity, tiles, rect, depth, tet_blocks, counts_dev, sorted, ranges, w + L.bin,
-                      st);
+        e = batch_bin(plan_dev, h, pair_capacity, tiles, rect, depth, tet_blocks, counts_dev, sorted, ranges, order,
+                      w + L.bin, st);
     if (e == hipSuccess)
-        e = batch_composite(plan_dev, h, sorted, pair_capacity, ranges, out_img_dev, out_dist_dev, out_ncontrib_dev, st);
+        e = batch_composite(plan_dev, h, sorted, pair_capacity, ranges, order, out_img_dev, out_dist_dev,
+                            out_ncontrib_dev, st);
     if (e != hipSuccess) {
         set_error(std::string("rm_batch_render: ") + hipGetErrorString(e));
         return RM_ERR_HIP;

@@ -118,14 +118,14 @@ hipError_t batch_project(const void* plan_dev, const PlanHeader& h, int32_t* til
 size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap);
 hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
                      const int4* rect, const float* depth, const uint2* tet_blocks, int64_t* counts_dev,
-                     int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st);
+                     int32_t* sorted_tet, int32_t* ranges, int32_t* tile_order_out, void* ws, hipStream_t st);
 int batch_workgroups_per_view(int tile, int gh, int gw);
 // The binning's unit: blocks of kBinBlockTiles x kBinBlockTiles tiles over a view's tile grid.
 constexpr int kBinBlockTiles = 8;
 __host__ __device__ inline int32_t bin_blocks_per_row(int gw) { return (gw + kBinBlockTiles - 1) / kBinBlockTiles; }
 inline int64_t bin_blocks_per_view(int gh, int gw) { return (int64_t)bin_blocks_per_row(gw) * bin_blocks_per_row(gh); }
 hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet,
-                           int64_t pair_cap, const int32_t* ranges, float* img, float* dist, int32_t* nc,
-                           hipStream_t st);
+                           int64_t pair_cap, const int32_t* ranges, const int32_t* tile_order, float* img,
+                           float* dist, int32_t* nc, hipStream_t st);
 
 }  // namespace rmesh

@@ -1208,7 +1208,7 @@ __global__ __launch_bounds__(64, 3) void composite_bwd_tet_lanes_kernel(
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, const float4* __restrict__ img, const float* __restrict__ dist,
     const int* __restrict__ ncontrib, const float4* __restrict__ gimg, const float* __restrict__ gdist,
-    float* __restrict__ rows, const uint8_t* __restrict__ flags) {
+    float* __restrict__ rows, const uint8_t* __restrict__ flags, const int* __restrict__ tile_order) {
     constexpr int NGRD = NG_<F>::NGRD;
     constexpr int NCH = F == 7 ? 1 : 3;
     constexpr int NV = 1 + 3 + NCH + 3 * NCH + 4 + 12;   // dsig, dbase[3], Sq[NCH], Rg[3*NCH], u[4], q[12]
@@ -1227,7 +1227,7 @@ __global__ __launch_bounds__(64, 3) void composite_bwd_tet_lanes_kernel(
     auto& s_sum = s_u.sum;
 
     const int lane = threadIdx.x;
-    const int tile = remap_block(blockIdx.x, gridDim.x);
+    const int tile = tile_order ? tile_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
     const int tx = tile % cam.gw, ty = tile / cam.gw;
     const int2 rg = load_range(ranges, true, tile, n_sorted);
     const int n = rg.y - rg.x;
@@ -1627,21 +1627,24 @@ __global__ __launch_bounds__(64) void composite_fwd_quads_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float2* __restrict__ jitter, float4* __restrict__ out_img, float* __restrict__ out_dist,
-    int* __restrict__ out_nc, uint8_t* __restrict__ flags) {
+    int* __restrict__ out_nc, uint8_t* __restrict__ flags, const int* __restrict__ tile_order) {
     __shared__ Stage<64, F, false, P64> S;
-    fwd_quads_tile<F, P64, FLAGS>(S, cam, remap_block(blockIdx.x, gridDim.x), sorted, n_sorted, ranges, 0, 0, indices,
-                                  verts, feats, jitter, out_img, out_dist, out_nc, 0, flags);
+    const int tile = tile_order ? tile_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
+    fwd_quads_tile<F, P64, FLAGS>(S, cam, tile, sorted, n_sorted, ranges, 0, 0, indices, verts, feats, jitter, out_img,
+                                  out_dist, out_nc, 0, flags);
 }
 
 // Batched 4x4-tile forward: one workgroup per tile of every view of a plan (pixel-centre rays).
 template <int F, bool P64>
 __global__ __launch_bounds__(64) void batch_composite_fwd_quads_kernel(
     const void* __restrict__ plan, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
-    float4* __restrict__ out_img, float* __restrict__ out_dist, int* __restrict__ out_nc) {
+    const int* __restrict__ tile_order, float4* __restrict__ out_img, float* __restrict__ out_dist,
+    int* __restrict__ out_nc) {
     __shared__ Stage<64, F, false, P64> S;
     const PlanHeader* h = static_cast<const PlanHeader*>(plan);
     const ViewRec* views = plan_views(plan);
-    const int bg = remap_block(blockIdx.x, gridDim.x);
+    // one workgroup per tile at 4x4 tiles, so a view's first workgroup is its first tile
+    const int bg = tile_order ? tile_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
     if (bg >= h->total_blocks) return;  // launch sized for a capacity
     int lo = 0, hi = h->n_views - 1;  // last view with block_offset <= bg
     while (lo < hi) {
@@ -1659,17 +1662,17 @@ __global__ __launch_bounds__(64) void batch_composite_fwd_quads_kernel(
 template <int TS, int F, bool P64>
 int launch_fwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, float* img, float* dist, int* nc, hipStream_t st,
-               uint8_t* flags = nullptr) {
+               uint8_t* flags = nullptr, const int* tile_order = nullptr) {
     using C = Cfg<TS>;
     if constexpr (TS == 4) {
         if (flags)
             hipLaunchKernelGGL((composite_fwd_quads_kernel<F, P64, true>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
                                sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
-                               (const float2*)jitter, (float4*)img, dist, nc, flags);
+                               (const float2*)jitter, (float4*)img, dist, nc, flags, tile_order);
         else
             hipLaunchKernelGGL((composite_fwd_quads_kernel<F, P64, false>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
                                sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
-                               (const float2*)jitter, (float4*)img, dist, nc, flags);
+                               (const float2*)jitter, (float4*)img, dist, nc, flags, tile_order);
     } else {
         int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
         hipLaunchKernelGGL((composite_fwd_kernel<TS, F, P64>), dim3(sgw * sgh), dim3(C::WG), 0, st, a, sorted,
@@ -1779,7 +1782,7 @@ template <int TS, int F, bool P64, bool ROWS>
 int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices, const float* verts,
                const float* feats, const float* jitter, const float* img, const float* dist, const int* nc,
                const float* gimg, const float* gdist, float* gverts, float* gfeats, float* rows, int64_t n_tets,
-               hipStream_t st, const uint8_t* flags = nullptr) {
+               hipStream_t st, const uint8_t* flags = nullptr, const int* tile_order = nullptr) {
     using C = Cfg<TS>;
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
     if (ROWS && n_tets > 0) {
@@ -1791,12 +1794,12 @@ int launch_bwd(const CamArgs& a, const int* sorted, int64_t n_sorted, const int*
             hipLaunchKernelGGL((composite_bwd_tet_lanes_kernel<F, P64, true>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
                                sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
                                (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist, rows,
-                               flags);
+                               flags, tile_order);
         else
             hipLaunchKernelGGL((composite_bwd_tet_lanes_kernel<F, P64, false>), dim3(a.gw * a.gh), dim3(64), 0, st, a,
                                sorted, n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats,
                                (const float2*)jitter, (const float4*)img, dist, nc, (const float4*)gimg, gdist, rows,
-                               (const uint8_t*)nullptr);
+                               (const uint8_t*)nullptr, tile_order);
     } else if constexpr (ROWS && C::GROUP <= 64) {
         if (flags) {
             hipLaunchKernelGGL((composite_bwd_kernel<TS, F, P64, ROWS, true>), dim3(sgw * sgh), dim3(C::WG), 0, st,
@@ -2140,10 +2143,10 @@ int g_fp64_planes = 1;
 
 template <int TS, int F, bool P64>
 void launch_batch(const void* plan, const PlanHeader& h, const int32_t* sorted, int64_t cap, const int32_t* ranges,
-                  float* img, float* dist, int32_t* nc, hipStream_t st) {
+                  const int32_t* order, float* img, float* dist, int32_t* nc, hipStream_t st) {
     if constexpr (TS == 4)
         hipLaunchKernelGGL((batch_composite_fwd_quads_kernel<F, P64>), dim3((unsigned)h.total_blocks), dim3(64),
-                           0, st, plan, sorted, cap, (const int2*)ranges, (float4*)img, dist, nc);
+                           0, st, plan, sorted, cap, (const int2*)ranges, order, (float4*)img, dist, nc);
     else
         hipLaunchKernelGGL((batch_composite_fwd_kernel<TS, F, P64>), dim3((unsigned)h.total_blocks),
                            dim3(Cfg<TS>::WG), 0, st, plan, sorted, cap, (const int2*)ranges, (float4*)img, dist, nc);
@@ -2155,10 +2158,13 @@ void launch_batch(const void* plan, const PlanHeader& h, const int32_t* sorted, 
 int batch_workgroups_per_view(int, int gh, int gw) { return gw * gh; }
 
 hipError_t batch_composite(const void* plan_dev, const PlanHeader& h, const int32_t* sorted_tet, int64_t pair_cap,
-                           const int32_t* ranges, float* img, float* dist, int32_t* nc, hipStream_t st) {
+                           const int32_t* ranges, const int32_t* tile_order, float* img, float* dist, int32_t* nc,
+                           hipStream_t st) {
     if (h.total_blocks <= 0) return hipSuccess;
-#define RM_BATCH(TS, F) (g_fp64_planes ? launch_batch<TS, F, true>(plan_dev, h, sorted_tet, pair_cap, ranges, img, dist, nc, st) \
-                                       : launch_batch<TS, F, false>(plan_dev, h, sorted_tet, pair_cap, ranges, img, dist, nc, st))
+#define RM_BATCH(TS, F) (g_fp64_planes ? launch_batch<TS, F, true>(plan_dev, h, sorted_tet, pair_cap, ranges, tile_order, \
+                                                                      img, dist, nc, st) \
+                                       : launch_batch<TS, F, false>(plan_dev, h, sorted_tet, pair_cap, ranges, tile_order, \
+                                                                    img, dist, nc, st))
     if (h.n_features == 7) {
         if (h.tile == 4) RM_BATCH(4, 7); else if (h.tile == 8) RM_BATCH(8, 7); else RM_BATCH(16, 7);
     } else {
@@ -2182,7 +2188,7 @@ int composite_fwd_entry(const rm_camera* cam, const int32_t* sorted_tet_dev, int
                         const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                         const float* features_dev, int32_t n_features, const float* jitter_dev, float* out_img_dev,
                         float* out_dist_dev, int32_t* out_ncontrib_dev, uint8_t* flags_dev, rm_stream_t stream,
-                        const char* who) {
+                        const char* who, const int32_t* tile_order_dev = nullptr) {
     const std::string w(who);
     RM_CHECK_ARG(n_sorted >= 0, w + ": negative n_sorted");
     if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
@@ -2196,7 +2202,7 @@ int composite_fwd_entry(const rm_camera* cam, const int32_t* sorted_tet_dev, int
     const int ts = cam->tile_width;
 #define RM_FWD(TS, F, P) launch_fwd<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
                                         features_dev, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, st, \
-                                        flags_dev)
+                                        flags_dev, tile_order_dev)
 #define RM_FWD_P(TS, F) (g_fp64_planes ? RM_FWD(TS, F, true) : RM_FWD(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_FWD_P(4, 7); else if (ts == 8) RM_FWD_P(8, 7); else RM_FWD_P(16, 7);
@@ -2224,11 +2230,11 @@ extern "C" int rm_composite_fwd_flags(const rm_camera* cam, const int32_t* sorte
                                       const int32_t* ranges_dev, const int32_t* indices_dev,
                                       const float* vertices_dev, const float* features_dev, int32_t n_features,
                                       const float* jitter_dev, float* out_img_dev, float* out_dist_dev,
-                                      int32_t* out_ncontrib_dev, uint8_t* pair_flags_dev, rm_stream_t stream) {
-    RM_CHECK_ARG(pair_flags_dev != nullptr || n_sorted == 0, "rm_composite_fwd_flags: null pair_flags");
+                                      int32_t* out_ncontrib_dev, uint8_t* pair_flags_dev,
+                                      const int32_t* tile_order_dev, rm_stream_t stream) {
     return composite_fwd_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, features_dev,
                                n_features, jitter_dev, out_img_dev, out_dist_dev, out_ncontrib_dev, pair_flags_dev,
-                               stream, "rm_composite_fwd_flags");
+                               stream, "rm_composite_fwd_flags", tile_order_dev);
 }
 
 extern "C" int rm_composite_bwd(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
@@ -2278,7 +2284,8 @@ int composite_bwd_rows_entry(const rm_camera* cam, const int32_t* sorted_tet_dev
                              const float* jitter_dev, const float* img_dev, const float* dist_dev,
                              const int32_t* ncontrib_dev, const float* grad_img_dev, const float* grad_dist_dev,
                              float* grad_vertices_dev, float* grad_features_dev, const uint8_t* flags_dev,
-                             void* workspace_dev, size_t workspace_bytes, rm_stream_t stream, const char* who) {
+                             void* workspace_dev, size_t workspace_bytes, rm_stream_t stream, const char* who,
+                             const int32_t* tile_order_dev = nullptr) {
     const std::string w(who);
     RM_CHECK_ARG(n_sorted >= 0, w + ": negative n_sorted");
     RM_CHECK_ARG(n_tets >= 0, w + ": negative n_tets");
@@ -2302,7 +2309,7 @@ int composite_bwd_rows_entry(const rm_camera* cam, const int32_t* sorted_tet_dev
 #define RM_BWD(TS, F, P) launch_bwd<TS, F, P, true>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, \
                                         vertices_dev, features_dev, jitter_dev, img_dev, dist_dev, ncontrib_dev, \
                                         grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev, rows, \
-                                        n_tets, st, flags_dev)
+                                        n_tets, st, flags_dev, tile_order_dev)
 #define RM_BWD_P(TS, F) (g_fp64_planes ? RM_BWD(TS, F, true) : RM_BWD(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_BWD_P(4, 7); else if (ts == 8) RM_BWD_P(8, 7); else RM_BWD_P(16, 7);
@@ -2334,15 +2341,15 @@ extern "C" int rm_composite_bwd_flags(const rm_camera* cam, const int32_t* sorte
                                       const float* vertices_dev, const float* features_dev, int32_t n_features,
                                       const float* jitter_dev, const float* img_dev, const float* dist_dev,
                                       const int32_t* ncontrib_dev, const uint8_t* pair_flags_dev,
-                                      const float* grad_img_dev, const float* grad_dist_dev,
-                                      float* grad_vertices_dev, float* grad_features_dev, void* workspace_dev,
-                                      size_t workspace_bytes, rm_stream_t stream) {
+                                      const int32_t* tile_order_dev, const float* grad_img_dev,
+                                      const float* grad_dist_dev, float* grad_vertices_dev, float* grad_features_dev,
+                                      void* workspace_dev, size_t workspace_bytes, rm_stream_t stream) {
     RM_CHECK_ARG(pair_flags_dev != nullptr || n_sorted == 0, "rm_composite_bwd_flags: null pair_flags");
     return composite_bwd_rows_entry(cam, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, n_tets, vertices_dev,
                                     features_dev, n_features, jitter_dev, img_dev, dist_dev, ncontrib_dev,
                                     grad_img_dev, grad_dist_dev, grad_vertices_dev, grad_features_dev,
                                     pair_flags_dev, workspace_dev, workspace_bytes, stream,
-                                    "rm_composite_bwd_flags");
+                                    "rm_composite_bwd_flags", tile_order_dev);
 }
 
 // ---- no-allocation variant of rm_composite_bwd_ws -------------------------------------------

@@ -790,6 +790,64 @@ __global__ __launch_bounds__(kThreads) void seg_emit_kernel(const uint32_t* __re
     }
 }
 
+// ---- tile order: tiles with longer lists first ----------------------------------------------
+//
+// The compositing kernels take their tiles in this order, so that the longest walks start first
+// and the short ones fill in behind them (a tile's walk runs on one wave; one late-starting
+// tile with a long list would otherwise keep the chip partly idle at the end). Keys 0xFFFF minus
+// the tile's count (clamped to 0xFFFF) in 16 bits, values the tile ids, sorted stably: tiles of
+// equal count stay in tile order.
+template <typename Count>
+__global__ __launch_bounds__(kThreads) void tile_order_keys_kernel(Count count_of, int64_t G, uint32_t* __restrict__ key,
+                                                                   uint32_t* __restrict__ val) {
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= G) return;
+    const uint32_t c = count_of(t);
+    key[t] = 0xFFFFu - (c < 0xFFFFu ? c : 0xFFFFu);
+    val[t] = (uint32_t)t;
+}
+struct CountOfRange {   // list length of tile t
+    const int2* ranges;
+    __device__ uint32_t operator()(int64_t t) const {
+        const int2 r = ranges[t];
+        return r.y > r.x ? (uint32_t)(r.y - r.x) : 0u;
+    }
+};
+struct CountOfArray {
+    const uint32_t* c;
+    __device__ uint32_t operator()(int64_t t) const { return c[t]; }
+};
+
+struct OrderLayout {
+    size_t key, val, key2, key3, val3, counts, partial, total;
+};
+inline OrderLayout order_layout(int64_t G) {
+    OrderLayout L;
+    size_t o = 0;
+    const int64_t g = G > 0 ? G : 1;
+    L.key = o; o += align256(4 * (size_t)g);
+    L.val = o; o += align256(4 * (size_t)g);
+    L.key2 = o; o += align256(4 * (size_t)g);
+    L.key3 = o; o += align256(4 * (size_t)g);
+    L.val3 = o; o += align256(4 * (size_t)g);
+    L.counts = o; o += rs_counts_bytes(g);
+    L.partial = o; o += scan_ws_bytes(256 * rs_blocks(g) > g ? 256 * rs_blocks(g) : g);
+    L.total = o;
+    return L;
+}
+
+// order[0..G) = the tiles by decreasing count of count_of, over a workspace of order_layout(G).
+template <typename Count>
+hipError_t tile_order(Count count_of, int64_t G, int32_t* order, char* w, hipStream_t st) {
+    if (G <= 0) return hipSuccess;
+    const OrderLayout L = order_layout(G);
+    auto W = [&](size_t off) { return reinterpret_cast<uint32_t*>(w + off); };
+    hipLaunchKernelGGL(tile_order_keys_kernel<Count>, dim3((unsigned)ceil_div(G, kThreads)), dim3(kThreads), 0, st,
+                       count_of, G, W(L.key), W(L.val));
+    return radix_sort<uint32_t>(W(L.key), W(L.val), W(L.key2), reinterpret_cast<uint32_t*>(order), W(L.key3),
+                                W(L.val3), G, 0, 16, W(L.counts), W(L.partial), st);
+}
+
 // Fused-path workspace layout (offsets in bytes). The count phase only uses the first region.
 struct BinLayout {
     size_t blk_vis, blk_pairs, partial, counts2, vkey, vidx, vkey2, vidx2, vkey3, vidx3, poff, ptile, ptet,
@@ -910,7 +968,7 @@ hipError_t block_bin_lists(const uint32_t* order, int64_t nvis_cap, const int64_
 }  // namespace
 
 size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) {
-    return bin_layout(h.total_tets, pair_cap, h.total_tiles).total;
+    return align256(bin_layout(h.total_tets, pair_cap, h.total_tiles).total) + order_layout(h.total_tiles).total;
 }
 
 // Binning of a whole batch with every size read from device memory: grids are sized by the
@@ -921,7 +979,7 @@ size_t batch_bin_workspace(const PlanHeader& h, int64_t pair_cap) {
 // sort of the (global tile, tet) pairs by tile.
 hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap, const int32_t* tiles,
                      const int4* rect, const float* depth, const uint2* tet_blocks, int64_t* counts_dev,
-                     int32_t* sorted_tet, int32_t* ranges, void* ws, hipStream_t st) {
+                     int32_t* sorted_tet, int32_t* ranges, int32_t* tile_order_out, void* ws, hipStream_t st) {
     const int64_t T = h.total_tets, G = h.total_tiles, NB = h.total_bin_blocks;
     BinLayout L = bin_layout(T, pair_cap, G);
     char* w = static_cast<char*>(ws);
@@ -936,8 +994,11 @@ hipError_t batch_bin(const void* plan_dev, const PlanHeader& h, int64_t pair_cap
     hipError_t e = radix_sort<uint32_t>(W(L.vkey), W(L.vidx), W(L.vkey2), W(L.vidx2), W(L.vkey3), W(L.vidx3), T, 0,
                                         32, W(L.rcounts), W(L.partial), st, counts_dev);
     if (e != hipSuccess) return e;
-    return block_bin_lists(W(L.vidx2), T, counts_dev, rect, L, w, pair_cap, G, NB,
-                           BlockMapBatch{tet_blocks}, GridBatch{plan_dev}, sorted_tet, ranges, st);
+    e = block_bin_lists(W(L.vidx2), T, counts_dev, rect, L, w, pair_cap, G, NB, BlockMapBatch{tet_blocks},
+                        GridBatch{plan_dev}, sorted_tet, ranges, st);
+    if (e != hipSuccess || !tile_order_out) return e;
+    // the global tiles by decreasing pair count, for the compositing (its capacity tiles count 0)
+    return tile_order(CountOfArray{W(L.tcount)}, G, tile_order_out, w + align256(L.total), st);
 }
 
 }  // namespace rmesh
@@ -1093,5 +1154,28 @@ extern "C" int rm_bin_sort(const int32_t* tiles_touched_dev, const int32_t* rect
                     static_cast<char*>(workspace_dev), n_pairs, G, (int64_t)bgw * bgh, TileMapSingle{bgw},
                     GridSingle{grid_width, grid_height, bgw}, sorted_tet_dev, ranges_dev, st);
     RM_CHECK_LAUNCH("rm_bin_sort");
+    return RM_OK;
+}
+
+extern "C" size_t rm_tile_order_workspace_size(int64_t n_tiles) { return order_layout(n_tiles).total; }
+
+extern "C" int rm_tile_order(const int32_t* ranges_dev, int64_t n_tiles, int32_t* order_dev, void* workspace_dev,
+                             size_t workspace_bytes, rm_stream_t stream) {
+    RM_CHECK_ARG(n_tiles >= 0 && n_tiles < (1ll << 31), "rm_tile_order: n_tiles out of range");
+    if (n_tiles == 0) return RM_OK;
+    RM_CHECK_ARG(ranges_dev && order_dev && workspace_dev, "rm_tile_order: null buffer");
+    RM_CHECK_ARG(((uintptr_t)ranges_dev & 7) == 0 && ((uintptr_t)workspace_dev & 255) == 0,
+                 "rm_tile_order: misaligned ranges or workspace");
+    if (workspace_bytes < order_layout(n_tiles).total) {
+        set_error("rm_tile_order: workspace too small");
+        return RM_ERR_WORKSPACE_TOO_SMALL;
+    }
+    hipError_t e = tile_order(CountOfRange{reinterpret_cast<const int2*>(ranges_dev)}, n_tiles, order_dev,
+                              static_cast<char*>(workspace_dev), as_stream(stream));
+    if (e != hipSuccess) {
+        set_error(std::string("rm_tile_order: ") + hipGetErrorString(e));
+        return RM_ERR_HIP;
+    }
+    RM_CHECK_LAUNCH("rm_tile_order");
     return RM_OK;
 }

@@ -234,6 +234,27 @@ def vertex_and_tile_shader(indices, vertices, cam, render_grid, fused=True):
     return sorted_idx, ranges, vs, cc, mask, rect
 
 
+# rm_tile_order workspace kept between calls, one per (device, stream) (calls on one stream run in
+# order, so reusing it there is safe).
+_order_ws = {}
+
+
+def tile_order(tile_ranges):
+    """int32 [G]: the tiles by decreasing list length (rm_tile_order), the order in which the 4x4-tile
+    compositing kernels take them so that the longest walks start first."""
+    dev = tile_ranges.device
+    G = tile_ranges.shape[0]
+    L = lib()
+    wsb = L.rm_tile_order_workspace_size(G)
+    key = (str(dev), torch.cuda.current_stream(dev).cuda_stream)
+    ws = _order_ws.get(key)
+    if ws is None or ws.numel() < wsb:
+        ws = _order_ws[key] = torch.empty((max(wsb, 1),), device=dev, dtype=torch.uint8)
+    order = torch.empty((G,), device=dev, dtype=torch.int32)
+    check(L.rm_tile_order(ptr(tile_ranges), G, ptr(order), ptr(ws), ws.numel(), stream_of(dev)), "rm_tile_order")
+    return order
+
+
 class AlphaBlendTiledRender(torch.autograd.Function):
     """Per-tile compositing. forward -> (output_img [H,W,4] = rgb + log-transmittance,
     distortion_img [H,W,5], tet_alive [T] bool (all False, as the reference never writes it))."""
@@ -266,11 +287,14 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         # then skips the entries no pixel blended
         P = sorted_tetra_idx.numel()
         flags = torch.empty((P if ts[0] <= 8 else 0,), device=dev, dtype=torch.uint8)
+        order = None
         with timing.span("composite_fwd"):
             if flags.numel():
+                # 4x4 tiles: the longest lists first (the kernels take any order; results do not depend on it)
+                order = tile_order(tile_ranges) if ts[0] == 4 else None
                 check(lib().rm_composite_fwd_flags(cam, ptr(sorted_tetra_idx), P, ptr(tile_ranges), ptr(indices),
                                                    ptr(vertices), ptr(tet_density), F, ptr(ray_jitter), ptr(output_img),
-                                                   ptr(distortion_img), ptr(n_contributors), ptr(flags),
+                                                   ptr(distortion_img), ptr(n_contributors), ptr(flags), ptr(order),
                                                    stream_of(dev)), "rm_composite_fwd_flags")
             else:
                 check(lib().rm_composite_fwd(cam, ptr(sorted_tetra_idx), P, ptr(tile_ranges), ptr(indices),
@@ -283,6 +307,7 @@ class AlphaBlendTiledRender(torch.autograd.Function):
         ctx.non_tensor_data = non_tensor
         ctx.render_grid = render_grid
         ctx.cam = cam
+        ctx.tile_order = order
         ctx.mark_non_differentiable(tet_alive)
         return output_img, distortion_img, tet_alive
 
@@ -308,7 +333,8 @@ class AlphaBlendTiledRender(torch.autograd.Function):
                                                ptr(tile_ranges), ptr(indices), n_tets, ptr(vertices), ptr(tet_density),
                                                tet_density.shape[1], ptr(ray_jitter), ptr(output_img),
                                                ptr(distortion_img), ptr(n_contributors), ptr(flags),
-                                               ptr(grad_output_img), ptr(grad_distortion_img), ptr(vertices_grad),
+                                               ptr(ctx.tile_order), ptr(grad_output_img), ptr(grad_distortion_img),
+                                               ptr(vertices_grad),
                                                ptr(tet_density_grad), ptr(ws), wsb, stream_of(vertices.device)),
                       "rm_composite_bwd_flags")
             else:

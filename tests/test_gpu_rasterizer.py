@@ -377,11 +377,12 @@ def test_projection_backward_fp64(oracle_lib, kind, tile):
 @pytest.mark.parametrize("fp64", [True, False])
 def test_flagged_pair_matches_workspace_path(kind, tile, nf, fp64):
     """rm_composite_fwd_flags writes the same image, distortion and n_contrib as rm_composite_fwd
-    (bit for bit); rm_composite_bwd_flags, which walks only the entries some pixel of the tile
-    blended, gives the gradients of rm_composite_bwd_ws (per element within 1e-5 of the largest:
-    the two differ only in the order of float additions); on the scene an entry is skipped for."""
+    (bit for bit), here with the tiles taken longest list first (rm_tile_order) at 4x4 tiles;
+    rm_composite_bwd_flags, which walks only the entries some pixel of the tile blended, gives the
+    gradients of rm_composite_bwd_ws (per element within 1e-5 of the largest: the two differ only
+    in the order of float additions); on the scene an entry is skipped for."""
     import torch
-    from radiance_meshes_amd.rasterizer import _cam, set_intersect_precision, vertex_and_tile_shader
+    from radiance_meshes_amd.rasterizer import _cam, set_intersect_precision, tile_order, vertex_and_tile_shader
     from radiance_meshes_amd._capi import check, lib, ptr, stream_of
     sc = scene(kind, tile, nf)
     print(f"flagged pair {kind} tile={tile} nf={nf} fp64={fp64}", flush=True)
@@ -410,11 +411,18 @@ def test_flagged_pair_matches_workspace_path(kind, tile, nf, fp64):
             gv = torch.zeros_like(verts)
             gf = torch.zeros_like(feats)
             if flagged:
+                order = tile_order(ranges) if tile == 4 else None
+                if order is not None:   # a permutation of the tiles, non-increasing list lengths
+                    n_of = (ranges[:, 1] - ranges[:, 0]).cpu().numpy()
+                    o = order.cpu().numpy()
+                    assert np.array_equal(np.sort(o), np.arange(len(o))) and (np.diff(n_of[o]) <= 0).all()
                 check(L.rm_composite_fwd_flags(cam, ptr(sidx), P, ptr(ranges), ptr(idx), ptr(verts), ptr(feats), nf,
-                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), st), "fwd_flags")
+                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), ptr(order), st),
+                      "fwd_flags")
                 check(L.rm_composite_bwd_flags(cam, ptr(sidx), P, ptr(ranges), ptr(idx), T, ptr(verts), ptr(feats), nf,
-                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), ptr(gimg),
-                                               ptr(gdist), ptr(gv), ptr(gf), ptr(ws), wsb, st), "bwd_flags")
+                                               ptr(jit), ptr(img), ptr(dist), ptr(nc), ptr(flags), ptr(order),
+                                               ptr(gimg), ptr(gdist), ptr(gv), ptr(gf), ptr(ws), wsb, st),
+                      "bwd_flags")
             else:
                 check(L.rm_composite_fwd(cam, ptr(sidx), P, ptr(ranges), ptr(idx), ptr(verts), ptr(feats), nf,
                                          ptr(jit), ptr(img), ptr(dist), ptr(nc), st), "fwd")
