@@ -250,12 +250,13 @@ int rm_activate_bwd(const float* camera_center_dev, const float* density_dev, co
  * and adds into tet_count_dev [n_tets,2] i32 (caller-zeroed): 0: blended pixels, 1: max int(65535 w).
  * tet_err_dev must be 16-byte aligned.
  * pixel_err_dev / ssim_err_dev are [H,W] f32. The reference's gt, output_img and n_contributors
- * arguments are read by neither kernel and do not cross this boundary. */
+ * arguments are read by neither kernel and do not cross this boundary. tile_order_dev as for
+ * rm_composite_fwd_flags (4x4 tiles; null takes the tiles in id order). */
 int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, int64_t n_sorted,
                const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                const float* features_dev, int32_t n_features, const float* pixel_err_dev,
                const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev, int64_t n_tets,
-               rm_stream_t stream);
+               const int32_t* tile_order_dev, rm_stream_t stream);
 
 /* ---- batched forward rendering (serving) -----------------------------------------------
  * No counterpart in the reference (which renders one view per call, evaluate.py:50-63 /

@@ -76,7 +76,7 @@ SIGNATURES = {
                                            _P, _P, _P, _P, _P, _P]),
     "rm_activate_fwd": (C.c_int, [_P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _P, _P]),
     "rm_activate_bwd": (C.c_int, [_P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
-    "rm_tet_err": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _I64, _P]),
+    "rm_tet_err": (C.c_int, [C.POINTER(RmCamera), _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _I64, _P, _P]),
     "rm_batch_plan": (C.c_int, [C.POINTER(RmScene), _I32, C.POINTER(RmCamera), _P, _I32, _P, _SZ,
                                 C.POINTER(RmBatchInfo)]),
     "rm_batch_workspace_size": (_SZ, [C.POINTER(RmBatchInfo), _I64]),

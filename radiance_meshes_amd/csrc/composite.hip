@@ -1982,14 +1982,14 @@ __global__ __launch_bounds__(64) void tet_err_tet_lanes_kernel(
     CamArgs cam, const int* __restrict__ sorted, int64_t n_sorted, const int2* __restrict__ ranges,
     const int4* __restrict__ indices, const float* __restrict__ verts, const float* __restrict__ feats,
     const float* __restrict__ pixel_err, const float* __restrict__ ssim_err, float* __restrict__ tet_err,
-    int* __restrict__ tet_count) {
+    int* __restrict__ tet_count, const int* __restrict__ tile_order) {
     using T = typename Prec<P64>::R;
     using PL = typename Prec<P64>::PL;
     constexpr int NVOTE = 12;                   // row channels 0-11 (3: the vote count, 4: none)
     __shared__ float s_row[64][NVOTE + 1];
     __shared__ int s_tet[64];
     const int lane = threadIdx.x;
-    const int tile = remap_block(blockIdx.x, gridDim.x);
+    const int tile = tile_order ? tile_order[blockIdx.x] : remap_block(blockIdx.x, gridDim.x);
     const int tx = tile % cam.gw, ty = tile / cam.gw;
     const int2 rg = load_range(ranges, true, tile, n_sorted);
     const int n = rg.y - rg.x;
@@ -2098,11 +2098,12 @@ __global__ __launch_bounds__(64) void tet_err_tet_lanes_kernel(
 template <int TS, int F, bool P64>
 int launch_tet_err(const CamArgs& a, const int* sorted, int64_t n_sorted, const int* ranges, const int* indices,
                    const float* verts, const float* feats, const float* perr, const float* serr, float* err,
-                   int* count, hipStream_t st) {
+                   int* count, hipStream_t st, const int* tile_order) {
     using C = Cfg<TS>;
     if constexpr (TS == 4) {
         hipLaunchKernelGGL((tet_err_tet_lanes_kernel<F, P64>), dim3(a.gw * a.gh), dim3(64), 0, st, a, sorted,
-                           n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, perr, serr, err, count);
+                           n_sorted, (const int2*)ranges, (const int4*)indices, verts, feats, perr, serr, err, count,
+                           tile_order);
         return 0;
     }
     int sgw = (a.gw + C::GX - 1) / C::GX, sgh = (a.gh + C::GY - 1) / C::GY;
@@ -2419,7 +2420,7 @@ extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, i
                           const int32_t* ranges_dev, const int32_t* indices_dev, const float* vertices_dev,
                           const float* features_dev, int32_t n_features, const float* pixel_err_dev,
                           const float* ssim_err_dev, float* tet_err_dev, int32_t* tet_count_dev,
-                          int64_t n_tets, rm_stream_t stream) {
+                          int64_t n_tets, const int32_t* tile_order_dev, rm_stream_t stream) {
     RM_CHECK_ARG(n_sorted >= 0, "rm_tet_err: negative n_sorted");
     RM_CHECK_ARG(n_tets >= 0, "rm_tet_err: negative n_tets");
     if (n_sorted == 0 && sorted_tet_dev == nullptr) sorted_tet_dev = ranges_dev;  // empty list
@@ -2432,7 +2433,8 @@ extern "C" int rm_tet_err(const rm_camera* cam, const int32_t* sorted_tet_dev, i
     hipStream_t st = as_stream(stream);
     const int ts = cam->tile_width;
 #define RM_ERRV(TS, F, P) launch_tet_err<TS, F, P>(a, sorted_tet_dev, n_sorted, ranges_dev, indices_dev, vertices_dev, \
-                                        features_dev, pixel_err_dev, ssim_err_dev, tet_err_dev, tet_count_dev, st)
+                                        features_dev, pixel_err_dev, ssim_err_dev, tet_err_dev, tet_count_dev, st, \
+                                        tile_order_dev)
 #define RM_ERRV_P(TS, F) (g_fp64_planes ? RM_ERRV(TS, F, true) : RM_ERRV(TS, F, false))
     if (n_features == 7) {
         if (ts == 4) RM_ERRV_P(4, 7); else if (ts == 8) RM_ERRV_P(8, 7); else RM_ERRV_P(16, 7);

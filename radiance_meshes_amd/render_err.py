@@ -19,7 +19,7 @@ import torch.nn.functional as F
 
 from . import timing
 from ._capi import check, lib, ptr, stream_of
-from .rasterizer import RenderGrid, _cam, _require, vertex_and_tile_shader
+from .rasterizer import RenderGrid, _cam, _require, tile_order, vertex_and_tile_shader
 
 
 def _gauss1d(size: int, sigma: float, device=None, dtype=torch.float32) -> torch.Tensor:
@@ -81,10 +81,12 @@ def calc_tet_err(sorted_tetra_idx, tile_ranges, indices, vertices, tet_density, 
     if tet_count.shape != (T, 2) or tet_count.dtype != torch.int32 or not tet_count.is_contiguous():
         raise RuntimeError("tet_count must be a contiguous int32 [T, 2] tensor")
     with timing.span("tet_err"):
+        # 4x4 tiles: the longest lists first (the votes do not depend on the order)
+        order = tile_order(tile_ranges.reshape(-1, 2)) if render_grid.tile_height == 4 else None
         check(lib().rm_tet_err(_cam(tcam, render_grid), ptr(sorted_tetra_idx), sorted_tetra_idx.numel(),
                                ptr(tile_ranges), ptr(indices), ptr(vertices), ptr(tet_density),
                                tet_density.shape[1], ptr(pixel_err), ptr(ssim_err), ptr(tet_err), ptr(tet_count),
-                               T, stream_of(dev)), "rm_tet_err")
+                               T, ptr(order), stream_of(dev)), "rm_tet_err")
     return tet_err, tet_count
 
 

@@ -328,7 +328,7 @@ class RenderServer:
         """One view through the single-view path (host sync for the pair count, as the reference):
         the [H,W,4] image, or with aux the tuple (image, [H,W,5] distortion, [H,W] n_contrib) that
         batched views return."""
-        from .rasterizer import RenderGrid, _cam, vertex_and_tile_shader
+        from .rasterizer import RenderGrid, _cam, tile_order, vertex_and_tile_shader
         sc = self.scenes[sid]
         H, W = camera.height, camera.width
         rg = RenderGrid(H, W, self.tile, self.tile)
@@ -340,9 +340,12 @@ class RenderServer:
             img = torch.empty((H, W, 4), device=self.device)
             dist = torch.empty((H, W, 5), device=self.device)
             nc = torch.empty((H, W), device=self.device, dtype=torch.int32)
-            check(lib().rm_composite_fwd(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(sc.indices),
-                                         ptr(sc.vertices), ptr(feats), feats.shape[1], ptr(jit),
-                                         ptr(img), ptr(dist), ptr(nc), stream_of(self.device)), "rm_composite_fwd")
+            # 4x4 tiles: the longest lists first, as the batched path does (same images in any order)
+            order = tile_order(ranges) if self.tile == 4 else None
+            check(lib().rm_composite_fwd_flags(_cam(tcam, rg), ptr(sidx), sidx.numel(), ptr(ranges), ptr(sc.indices),
+                                               ptr(sc.vertices), ptr(feats), feats.shape[1], ptr(jit), ptr(img),
+                                               ptr(dist), ptr(nc), None, ptr(order), stream_of(self.device)),
+                  "rm_composite_fwd_flags")
         return (img, dist, nc) if aux else img
 
     # ------------------------------------------------------------------ buckets
