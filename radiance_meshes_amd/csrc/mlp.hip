@@ -23,8 +23,9 @@
 // the transposed weights, and forms the weight gradients
 // gW = dH act^T as MFMA tiles contracted over the tile's points; those need the point index on
 // the lanes' k, so dH and act blocks are transposed through a 16x17 LDS block per 16 units.
-// Weight-gradient tiles accumulate in registers across all tiles of a wave, are summed across the
-// workgroup in LDS and added to global memory with one float atomic per element.
+// Weight-gradient tiles are split among the workgroup's waves (each owns one hidden block of 16
+// units and contracts it over the four tiles the waves work on together), accumulate in registers
+// across the workgroup's steps and are added to global memory with one float atomic per element.
 #include <atomic>
 
 #include "common.h"
@@ -266,10 +267,14 @@ __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(const float* __restri
     }
 }
 
-// KI * KO > 4 (the up-to-48-output heads): the weight-gradient tiles take more than 256
-// registers, so the kernel runs one wave per SIMD with the accumulators beside the VGPRs. The
-// 64-input, 16-output head runs two waves per SIMD with ~100 registers spilled: 4.50 -> 4.37 ms
-// over 6.66M queries against one wave per SIMD without spills.
+// KI * KO > 4 (the up-to-48-output heads) runs one wave per SIMD, the others two.
+// The workgroup's four waves take four consecutive 16-point tiles per step and split the
+// weight-gradient tiles among themselves: wave w holds gW3's column block w, gW2's row block w and
+// gW1's row block w, each contracted over all 64 points of the step from the transposed fragments
+// that every wave leaves in LDS. A wave so keeps 4 (KO + 4 + KI) accumulator registers (28 for the
+// 32-input, 16-output head) instead of four times as many, and the kernel needs no scratch. Five
+// workgroup barriers per step order the shared blocks: A2 in blocks 0-3 (read for gW3), then dH2 in
+// 4-7 and A1 in 0-3 (read for gW2), then dH1 in 4-7 (read for gW1).
 template <int P, int KI, int KO>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * KO > 4 ? 1 : 2))) void mlp_bwd_kernel(
     const float* __restrict__ x, int64_t n, const float* w1, const float* b1, const float* w2, const float* b2,
@@ -280,51 +285,59 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * K
     __shared__ Smem<KI, KO> S;
     __shared__ float TB[Sh::TB];
     static_assert(Sh::TB >= Sh::NGRAD && Sh::TB >= kWaves * kTBlocks * kTBlock, "reduction buffer too small");
+    static_assert(kWaves == 4, "one hidden block of 16 units per wave");
     load_weights<KI, KO>(S, w1, b1, w2, b2, w3, b3, n_out);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
     float* tb = TB + wave * kTBlocks * kTBlock;
-    // weight-gradient tiles (D layout): gW3 [OUT x 64], gW2 [64 x 64], gW1 [64 x IN]; bias partials
-    // from the point fragments: lane (q, c) sums unit 16*ot + c over its points 4q + s
-    f4 g3[KO][4], g2[4][4], g1[4][KI];
-    float gb1s[4], gb2s[4], gb3s[KO];
+    // block b of wave w's transpose area
+    auto blk = [&](int w, int b) -> const float* { return TB + (w * kTBlocks + b) * kTBlock; };
+    // weight-gradient tiles (D layout): g3[ob] = gW3[16 ob.., 16 wave..], g2[kt] = gW2[16 wave.., 16 kt..],
+    // g1[kt] = gW1[16 wave.., 16 kt..]; bias partials of lane (q, c) over its points 4q + s:
+    // gb1s, gb2s for unit 16 wave + c, gb3s[ob] for output 16 ob + c (every wave forms them, wave 0 adds)
+    f4 g3[KO], g2[4], g1[KI];
+    float gb1s = 0.f, gb2s = 0.f, gb3s[KO];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        gb1s[a] = 0.f;
-        gb2s[a] = 0.f;
+    for (int kt = 0; kt < 4; ++kt) g2[kt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int b = 0; b < 4; ++b) g2[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < KI; ++kt) g1[kt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int b = 0; b < KI; ++b) g1[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ob = 0; ob < KO; ++ob) g3[ob][a] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int ob = 0; ob < KO; ++ob) {
+        g3[ob] = f4{0.f, 0.f, 0.f, 0.f};
+        gb3s[ob] = 0.f;
     }
-#pragma unroll
-    for (int ob = 0; ob < KO; ++ob) gb3s[ob] = 0.f;
     const int64_t ntiles = (n + 15) / 16;
-    for (int64_t t = (int64_t)blockIdx.x * kWaves + wave; t < ntiles; t += (int64_t)gridDim.x * kWaves) {
+    const int64_t nsteps = (ntiles + kWaves - 1) / kWaves;
+    // the loop bound is uniform over the workgroup (every wave reaches every barrier); tiles past
+    // the end contribute zeros
+    for (int64_t step = blockIdx.x; step < nsteps; step += gridDim.x) {
         asm volatile("" ::: "memory");  // weights are re-read from LDS per tile, not held in registers
-        const int64_t p = t * 16 + c;
+        const int64_t t0 = step * kWaves;
+        const int64_t p = (t0 + wave) * 16 + c;
         f4 xin[KI], h1[4], h2[4], unused[KO];
         load_x_chain<KI>(x, p, n, xin);
         tile_forward<P, KI, KO, false, true>(S, xin, h1, h2, unused);
-        // dY in the chain layout (unit 16 ob + 4q + r of point c) and as point fragments (unit
-        // 16 ob + c, points 4q + s)
+        // dY of this wave's tile in the chain layout (unit 16 ob + 4q + r of point c), and as point
+        // fragments of all four tiles (output 16 ob + c, points 4q + s of tile t0 + w)
         f4 dyc[KO];
-        float dyp[KO][4];
+        float dyp[kWaves][KO][4];
 #pragma unroll
         for (int ob = 0; ob < KO; ++ob) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int o = 16 * ob + 4 * q + r;
                 dyc[ob][r] = (p < n && o < n_out) ? gy[p * n_out + o] : 0.f;
-                const int64_t pp = t * 16 + 4 * q + r;
-                const int oc = 16 * ob + c;
-                dyp[ob][r] = (pp < n && oc < n_out) ? gy[pp * n_out + oc] : 0.f;
             }
-            gb3s[ob] += (dyp[ob][0] + dyp[ob][1]) + (dyp[ob][2] + dyp[ob][3]);
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int64_t pp = (t0 + w) * 16 + 4 * q + s;
+                    const int oc = 16 * ob + c;
+                    dyp[w][ob][s] = (pp < n && oc < n_out) ? gy[pp * n_out + oc] : 0.f;
+                }
         }
-        // gW3 += dY A2^T
+        // A2 = selu(H2) to blocks 0-3
         f4 d2[4];
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
@@ -332,15 +345,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * K
             selu_pair(h2[kt], a, d2[kt]);
             put_block(tb + kt * kTBlock, a);
         }
-        wave_sync();
+        __syncthreads();
+        // gW3[:, 16 wave..] += dY A2^T over the 64 points
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
+        for (int w = 0; w < kWaves; ++w) {
             float a2p[4];
-            get_block(tb + kt * kTBlock, a2p);
+            get_block(blk(w, wave), a2p);
 #pragma unroll
-            for (int ob = 0; ob < KO; ++ob) g3[ob][kt] = outer<P>(dyp[ob], a2p, g3[ob][kt]);
+            for (int ob = 0; ob < KO; ++ob) {
+                gb3s[ob] += (dyp[w][ob][0] + dyp[w][ob][1]) + (dyp[w][ob][2] + dyp[w][ob][3]);
+                g3[ob] = outer<P>(dyp[w][ob], a2p, g3[ob]);
+            }
         }
-        // dH2 = (W3^T dY) * selu'(H2)
+        // dH2 = (W3^T dY) * selu'(H2) of this wave's tile
         f4 dh2[4];
 #pragma unroll
         for (int ot = 0; ot < 4; ++ot) {
@@ -348,29 +365,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * K
 #pragma unroll
             for (int r = 0; r < 4; ++r) dh2[ot][r] = d[r] * d2[ot][r];
         }
-        wave_sync();
-        // gW2 += dH2 A1^T
-        f4 d1[4];
+        f4 a1[4], d1[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            f4 a;
-            selu_pair(h1[b], a, d1[b]);
-            put_block(tb + b * kTBlock, dh2[b]);
-            put_block(tb + (4 + b) * kTBlock, a);
+            selu_pair(h1[b], a1[b], d1[b]);
+            put_block(tb + (4 + b) * kTBlock, dh2[b]);
         }
-        wave_sync();
-        {
-            // the dH2 fragments are read one block at a time, to keep the live registers down
-            float a1p[4][4];
+        __syncthreads();  // every wave is done with the A2 blocks
 #pragma unroll
-            for (int b = 0; b < 4; ++b) get_block(tb + (4 + b) * kTBlock, a1p[b]);
+        for (int b = 0; b < 4; ++b) put_block(tb + b * kTBlock, a1[b]);
+        __syncthreads();
+        // gW2[16 wave.., :] += dH2 A1^T over the 64 points
 #pragma unroll
-            for (int ot = 0; ot < 4; ++ot) {
-                float dh2p[4];
-                get_block(tb + ot * kTBlock, dh2p);
-                gb2s[ot] += (dh2p[0] + dh2p[1]) + (dh2p[2] + dh2p[3]);
+        for (int w = 0; w < kWaves; ++w) {
+            float dh2p[4];
+            get_block(blk(w, 4 + wave), dh2p);
+            gb2s += (dh2p[0] + dh2p[1]) + (dh2p[2] + dh2p[3]);
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt) g2[ot][kt] = outer<P>(dh2p, a1p[kt], g2[ot][kt]);
+            for (int kt = 0; kt < 4; ++kt) {
+                float a1p[4];
+                get_block(blk(w, kt), a1p);
+                g2[kt] = outer<P>(dh2p, a1p, g2[kt]);
             }
         }
         // dH1 = (W2^T dH2) * selu'(H1)
@@ -389,32 +404,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * K
                 if (p < n) *reinterpret_cast<float4*>(gx + p * IN + 16 * ot + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
             }
         }
-        wave_sync();
-        // gW1 += dH1 X^T
+        __syncthreads();  // every wave is done with the dH2 and A1 blocks
 #pragma unroll
-        for (int b = 0; b < 4; ++b) put_block(tb + b * kTBlock, dh1[b]);
-        wave_sync();
-        {
-            float xp[KI][4];
+        for (int b = 0; b < 4; ++b) put_block(tb + (4 + b) * kTBlock, dh1[b]);
+        __syncthreads();
+        // gW1[16 wave.., :] += dH1 X^T over the 64 points
 #pragma unroll
-            for (int kt = 0; kt < KI; ++kt)
+        for (int w = 0; w < kWaves; ++w) {
+            float dh1p[4];
+            get_block(blk(w, 4 + wave), dh1p);
+            gb1s += (dh1p[0] + dh1p[1]) + (dh1p[2] + dh1p[3]);
+#pragma unroll
+            for (int kt = 0; kt < KI; ++kt) {
+                float xp[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const int64_t pp = t * 16 + 4 * q + s;
-                    xp[kt][s] = pp < n ? x[pp * IN + 16 * kt + c] : 0.f;
+                    const int64_t pp = (t0 + w) * 16 + 4 * q + s;
+                    xp[s] = pp < n ? x[pp * IN + 16 * kt + c] : 0.f;
                 }
-#pragma unroll
-            for (int ot = 0; ot < 4; ++ot) {
-                float dh1p[4];
-                get_block(tb + ot * kTBlock, dh1p);
-                gb1s[ot] += (dh1p[0] + dh1p[1]) + (dh1p[2] + dh1p[3]);
-#pragma unroll
-                for (int kt = 0; kt < KI; ++kt) g1[ot][kt] = outer<P>(dh1p, xp[kt], g1[ot][kt]);
+                g1[kt] = outer<P>(dh1p, xp, g1[kt]);
             }
         }
-        wave_sync();
     }
-    // sum the waves' gradients in LDS (over the transpose blocks), then one global atomic per element
+    // each weight-gradient element has one owning wave; the bias partials are summed over the lanes
+    // in LDS; then one global atomic per element
     __syncthreads();
     float* G3 = TB;                       // [OUT][64]
     float* G2 = G3 + OUT * kHid;          // [64][64]
@@ -428,24 +441,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KI * K
     for (int r = 0; r < 4; ++r) {
         const int o = 4 * q + r;
 #pragma unroll
-        for (int ob = 0; ob < KO; ++ob)
+        for (int ob = 0; ob < KO; ++ob) G3[(16 * ob + o) * kHid + 16 * wave + c] = g3[ob][r];
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt) atomicAdd(&G3[(16 * ob + o) * kHid + 16 * kt + c], g3[ob][kt][r]);
+        for (int kt = 0; kt < 4; ++kt) G2[(16 * wave + o) * kHid + 16 * kt + c] = g2[kt][r];
 #pragma unroll
-        for (int ot = 0; ot < 4; ++ot) {
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) atomicAdd(&G2[(16 * ot + o) * kHid + 16 * kt + c], g2[ot][kt][r]);
-#pragma unroll
-            for (int kt = 0; kt < KI; ++kt) atomicAdd(&G1[(16 * ot + o) * IN + 16 * kt + c], g1[ot][kt][r]);
-        }
+        for (int kt = 0; kt < KI; ++kt) G1[(16 * wave + o) * IN + 16 * kt + c] = g1[kt][r];
     }
+    atomicAdd(&B1[16 * wave + c], gb1s);
+    atomicAdd(&B2[16 * wave + c], gb2s);
+    if (wave == 0)
 #pragma unroll
-    for (int ot = 0; ot < 4; ++ot) {
-        atomicAdd(&B1[16 * ot + c], gb1s[ot]);
-        atomicAdd(&B2[16 * ot + c], gb2s[ot]);
-    }
-#pragma unroll
-    for (int ob = 0; ob < KO; ++ob) atomicAdd(&B3[16 * ob + c], gb3s[ob]);
+        for (int ob = 0; ob < KO; ++ob) atomicAdd(&B3[16 * ob + c], gb3s[ob]);
     __syncthreads();
     for (int i = threadIdx.x; i < Sh::NGRAD; i += blockDim.x) {
         const float v = TB[i];
