@@ -7,6 +7,8 @@
 // reference's int64 arithmetic). pos = x * res + 0.5 rounded as two float ops, as torch does;
 // trilinear interpolation in the reference's x -> y -> z order. Output [N, L * n_feat],
 // level-major. One thread per (point, level): consecutive threads write consecutive features.
+#include <mutex>
+
 #include "common.h"
 
 namespace rmesh {
@@ -712,6 +714,41 @@ int hashgrid_bwd_levels(const float* x, int64_t n, const float* gout, const Grid
     return RM_OK;
 }
 
+// A second stream per device for the coarse-level aggregation, which then runs beside the
+// fine-level ownership kernels: the aggregation is bound by vector and LDS work, the ownership
+// kernels by HBM, and they write disjoint levels of the table. Created once per device (default
+// priority, non-blocking), with the fork and join events; the mutex keeps one caller's
+// record/wait sequence whole.
+constexpr int kMaxSideDevices = 64;
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_side_mu;
+SideStream g_side[kMaxSideDevices];
+
+// The device's side stream, or nullptr (run everything on the caller's stream) when it cannot be
+// made.
+SideStream* side_stream() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxSideDevices) return nullptr;
+    SideStream& s = g_side[dev];
+    if (s.stream) return &s;
+    hipStream_t st = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        (void)hipStreamDestroy(st);
+        return nullptr;
+    }
+    s.fork = a;
+    s.join = b;
+    s.stream = st;
+    return &s;
+}
+
 }  // namespace
 }  // namespace rmesh
 
@@ -790,9 +827,22 @@ extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* gr
     uint32_t* start = reinterpret_cast<uint32_t*>(w + P.start);
     uint16_t* rslot = reinterpret_cast<uint16_t*>(w + P.rslot);
     float2* rval = reinterpret_cast<float2*>(w + P.rval);
-    // coarse levels: workgroup aggregation
+    // coarse levels: workgroup aggregation, on the side stream between a fork and a join on st
+    // (the lock holds from the fork's record to the join's wait)
+    std::unique_lock<std::mutex> side_lock(g_side_mu);
+    SideStream* side = lp > 0 ? side_stream() : nullptr;
+    if (side && (hipEventRecord(side->fork, st) != hipSuccess ||
+                 hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)) {
+        (void)hipGetLastError();
+        side = nullptr;
+    }
+    if (!side) side_lock.unlock();
     if (lp > 0) {
-        int r = hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, lp, grad_table_dev, st);
+        int r = hashgrid_bwd_levels(x_dev, n, grad_out_dev, g, lp, grad_table_dev, side ? side->stream : st);
+        if (r == RM_OK && side && hipEventRecord(side->join, side->stream) != hipSuccess) {
+            set_error("rm_hashgrid_bwd_ws: recording the aggregation stream's join failed");
+            r = RM_ERR_HIP;
+        }
         if (r) return r;
     }
     hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)P.n_chunks), dim3(kPartPoints), 0, st, x_dev, n, g, lp,
@@ -804,6 +854,10 @@ extern "C" int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* gr
                        rslot, rval);
     hipLaunchKernelGGL(part_accumulate_kernel, dim3((unsigned)P.nb), dim3(kPartAccThreads), 0, st, (const uint32_t*)start,
                        (const uint16_t*)rslot, (const float2*)rval, g, lp, P.nb_level, grad_table_dev);
+    if (side && hipStreamWaitEvent(st, side->join, 0) != hipSuccess) {
+        set_error("rm_hashgrid_bwd_ws: waiting for the aggregation stream failed");
+        return RM_ERR_HIP;
+    }
     RM_CHECK_LAUNCH("rm_hashgrid_bwd_ws");
     return RM_OK;
 }
