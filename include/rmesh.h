@@ -317,7 +317,10 @@ int rm_hashgrid_bwd(const float* x_dev, int64_t n, const float* grad_out_dev, co
  * with a workspace: for n_feat 2, the hashed levels with res >= 512 are summed by slot ownership
  * (updates sorted into 4096-slot buckets, one workgroup per bucket, no float atomics) instead of
  * by atomics. rm_hashgrid_bwd_workspace_size returns the bytes needed (0: no level qualifies,
- * and rm_hashgrid_bwd_ws then runs rm_hashgrid_bwd's path without a workspace). */
+ * and rm_hashgrid_bwd_ws then runs rm_hashgrid_bwd's path without a workspace). The coarser levels
+ * are summed on a second stream of the library's own (one per device, made on first use, default
+ * priority) between an event recorded on `stream` and a wait of `stream` on its completion, so all
+ * of the call's work is ordered on `stream` as for any other entry point. */
 size_t rm_hashgrid_bwd_workspace_size(int64_t n, const int32_t* res, const int64_t* offs, int32_t n_levels,
                                       int32_t n_feat, int32_t log2_size);
 int rm_hashgrid_bwd_ws(const float* x_dev, int64_t n, const float* grad_out_dev, const int32_t* res,
