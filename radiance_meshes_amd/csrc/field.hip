@@ -361,7 +361,9 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
                 for (int f = 0; f < F; ++f) unsafeAtomicAdd(gtable + (g.offs[l] + (int64_t)h) * F + f, v[f]);
         }
         __syncthreads();
-        // consecutive lanes take the F features of one slot, so they share one atomic transaction
+        // consecutive lanes take the F features of one slot, so they share one atomic transaction;
+        // the lanes of a slot are in one wave and read its key in one instruction, so the first of
+        // them clears the key afterwards for the next level
         float* tb = gtable + g.offs[l] * F;
         for (int p = tid; p < kAggSlots * F; p += kAggPoints) {
             const int slot = p / F;
@@ -370,9 +372,8 @@ __global__ __launch_bounds__(kAggPoints) void hashgrid_bwd_agg_kernel(const floa
             const float v = (float)s_val[p];
             if (v != 0.f) unsafeAtomicAdd(tb + (int64_t)k * F + (p - slot * F), v);
             s_val[p] = 0.0;
+            if (p == slot * F) s_key[slot] = kEmptySlot;
         }
-        __syncthreads();
-        for (int p = tid; p < kAggSlots; p += kAggPoints) s_key[p] = kEmptySlot;
         __syncthreads();
     }
 }
